@@ -1,0 +1,367 @@
+// Host engine: problem tables, pipelines, Green / operator / curl applies and
+// the Born (CBS) iteration with device-resident convergence state.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "cbs_engine.h"
+
+namespace cbs {
+
+// ---------------------------------------------------------------------------
+// device buffers
+// ---------------------------------------------------------------------------
+DevBuf& DevBuf::operator=(DevBuf&& o) noexcept {
+  if (this != &o) {
+    release();
+    p = o.p;
+    bytes = o.bytes;
+    ctx = o.ctx;
+    o.p = nullptr;
+    o.bytes = 0;
+  }
+  return *this;
+}
+
+void DevBuf::alloc(Ctx* c, size_t n) {
+  release();
+  if (n == 0) return;
+  ctx = c;
+  if (c && c->budget && c->used.load() + n > c->budget)
+    throw Error(CBS_ENOMEM, "HBM budget exceeded: need " + std::to_string(n) + " bytes, " +
+                                std::to_string(c->budget - c->used.load()) + " left");
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, n);
+  if (e != hipSuccess)
+    throw Error(CBS_ENOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+  p = q;
+  bytes = n;
+  if (c) c->used += n;
+}
+
+void DevBuf::release() {
+  if (p) {
+    (void)hipFree(p);
+    if (ctx) ctx->used -= bytes;
+  }
+  p = nullptr;
+  bytes = 0;
+}
+
+Ctx::~Ctx() {
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+// ---------------------------------------------------------------------------
+// tables: twiddles, runtime plans, shift ramps, Green and curl k vectors
+// ---------------------------------------------------------------------------
+static inline void exp_m2pi(long long num, long long den, double& c, double& s) {
+  // exp(-2 pi i num/den) with the argument reduced exactly in integers
+  long long q = num % den;
+  if (q < 0) q += den;
+  const long double a = 2.0L * 3.141592653589793238462643383279502884L * (long double)q /
+                        (long double)den;
+  c = (double)cosl(a);
+  s = (double)(-sinl(a));
+}
+
+template <class T>
+static void upload(DevBuf& b, Ctx* c, const void* src, size_t n, hipStream_t s) {
+  b.alloc(c, n);
+  CBS_HIP(hipMemcpyAsync(b.p, src, n, hipMemcpyHostToDevice, s));
+}
+
+template <class T>
+Tables<T>& tables(Problem& p);
+template <>
+Tables<double>& tables<double>(Problem& p) {
+  return p.t64;
+}
+template <>
+Tables<float>& tables<float>(Problem& p) {
+  return p.t32;
+}
+template <>
+const Cx<double>* potential<double>(Problem& p) {
+  return p.V64.as<Cx<double>>();
+}
+template <>
+const Cx<float>* potential<float>(Problem& p) {
+  return p.V32.as<Cx<float>>();
+}
+
+static int green_index(int i, int N) {  // dyadic_green.jl:203-208 (0-based i)
+  const int h = N / 2;
+  return ((i + h) % N) - h;
+}
+static int curl_index(int i, int N) {  // curl.jl:84-86 (0-based i)
+  return (i + 1 <= N / 2) ? i : i - N;
+}
+
+template <class T>
+void fill_tables(Problem& p) {
+  Tables<T>& t = tables<T>(p);
+  if (t.built) return;
+  hipStream_t s = p.ctx->stream;
+  for (int d = 0; d < 3; ++d) {
+    const int N = p.n[d];
+    std::vector<Cx<T>> tw(N), rp(N);
+    for (int q = 0; q < N; ++q) {
+      double c, sn;
+      exp_m2pi(q, N, c, sn);
+      tw[q] = mk<T>((T)c, (T)sn);
+      // ramp exp(-2 pi i q s / N): s in {0, 0.25} -> exact rational q s / N
+      const long double a = 2.0L * 3.141592653589793238462643383279502884L * (long double)q *
+                            (long double)p.d.shift[d] / (long double)N;
+      rp[q] = mk<T>((T)(double)cosl(a), (T)(double)(-sinl(a)));
+    }
+    RadixPlan pl = make_plan(N);
+    std::vector<int> plan(1 + kMaxStages, 0);
+    plan[0] = pl.nst;
+    for (int k = 0; k < pl.nst; ++k) plan[1 + k] = pl.r[k];
+    const double dk = 2.0 * M_PI / ((double)N * p.d.res[d]);  // dyadic_green.jl:89, curl.jl:34
+    std::vector<double> kp(N), km(N), kc(N);
+    for (int i = 0; i < N; ++i) {
+      kp[i] = dk * ((double)green_index(i, N) + p.d.shift[d]);
+      km[i] = dk * ((double)green_index(i, N) - p.d.shift[d]);
+      kc[i] = dk * (double)curl_index(i, N);
+    }
+    upload<T>(t.tw[d], p.ctx, tw.data(), N * sizeof(Cx<T>), s);
+    upload<T>(t.ramp[d], p.ctx, rp.data(), N * sizeof(Cx<T>), s);
+    upload<T>(t.plan[d], p.ctx, plan.data(), plan.size() * sizeof(int), s);
+    upload<T>(t.kk[d][0], p.ctx, kp.data(), N * sizeof(double), s);
+    upload<T>(t.kk[d][1], p.ctx, km.data(), N * sizeof(double), s);
+    upload<T>(t.kc[d], p.ctx, kc.data(), N * sizeof(double), s);
+  }
+  CBS_HIP(hipStreamSynchronize(s));  // host vectors go out of scope
+  t.built = true;
+}
+template void fill_tables<double>(Problem&);
+template void fill_tables<float>(Problem&);
+
+template <class T>
+DevProb<T> make_desc(Problem& p, int pre, int zop, int post, const Cx<T>* in0, const Cx<T>* in1,
+                     Cx<T>* out0, Cx<T>* w, double* part, const RState* st) {
+  fill_tables<T>(p);
+  Tables<T>& t = tables<T>(p);
+  DevProb<T> d;
+  std::memset(&d, 0, sizeof(d));
+  for (int k = 0; k < 3; ++k) {
+    d.n[k] = p.n[k];
+    d.rt[k] = is_ct_length(p.n[k]) ? 0 : 1;
+    d.tw[k] = t.tw[k].template as<Cx<T>>();
+    d.plan[k] = t.plan[k].template as<int>();
+    d.ramp[k] = t.ramp[k].template as<Cx<T>>();
+    d.kk[k][0] = t.kk[k][0].template as<double>();
+    d.kk[k][1] = t.kk[k][1].template as<double>();
+    d.kc[k] = t.kc[k].template as<double>();
+  }
+  d.S = zop == ZOP_CURL ? 1 : p.S;
+  d.has_ramp = (zop == ZOP_CURL) ? 0 : (p.has_ramp ? 1 : 0);
+  d.pre = pre;
+  d.zop = zop;
+  d.post = post;
+  d.nvox = p.nvox;
+  d.k2c_re = p.d.k2_re;
+  d.k2c_im = p.d.k2_im;
+  d.zs_re = 1.0 / (double)p.nvox;
+  d.zs_im = 0.0;
+  d.eps_inv = p.eps_inv;
+  d.a_re = 1.0;
+  d.V = potential<T>(p);
+  d.in0 = in0;
+  d.in1 = in1;
+  d.out0 = out0;
+  d.w = w;
+  d.part = part;
+  d.st = st;
+  count_tiles(d);
+  return d;
+}
+template DevProb<double> make_desc<double>(Problem&, int, int, int, const Cx<double>*,
+                                           const Cx<double>*, Cx<double>*, Cx<double>*, double*,
+                                           const RState*);
+template DevProb<float> make_desc<float>(Problem&, int, int, int, const Cx<float>*,
+                                         const Cx<float>*, Cx<float>*, Cx<float>*, double*,
+                                         const RState*);
+
+template <class T>
+void Pipe<T>::build(Ctx* c, hipStream_t s) {
+  const int np = (int)h.size();
+  CBS_REQUIRE(np > 0, "empty pipeline");
+  std::vector<int> bs(4 * (np + 1), 0);
+  L = PipeLaunch<T>{};
+  L.S = h[0].S;
+  for (int pass = 0; pass < 4; ++pass) {
+    int acc = 0;
+    size_t lds = 0;
+    for (int i = 0; i < np; ++i) {
+      CBS_REQUIRE(h[i].S == L.S, "pipeline mixes averaging and non-averaging problems");
+      bs[pass * (np + 1) + i] = acc;
+      acc += h[i].tiles[pass];
+      lds = std::max(lds, (size_t)tile_lds_bytes(h[i], pass));
+    }
+    bs[pass * (np + 1) + np] = acc;
+    L.blocks[pass] = acc;
+    // the xinv epilogue reuses the tile LDS for its block reduction
+    L.lds[pass] = std::max(lds, (size_t)((kThreads / 64) * kNumPart * sizeof(double)));
+  }
+  for (int pass = 0; pass < 4; ++pass)
+    if (L.lds[pass] > 160 * 1024)
+      throw Error(CBS_EINVAL, "grid axis too long for an LDS tile (" +
+                                  std::to_string(L.lds[pass]) + " bytes)");
+  d_probs.ensure(c, np * sizeof(DevProb<T>));
+  d_bstart.ensure(c, bs.size() * sizeof(int));
+  CBS_HIP(hipMemcpyAsync(d_probs.p, h.data(), np * sizeof(DevProb<T>), hipMemcpyHostToDevice, s));
+  CBS_HIP(hipMemcpyAsync(d_bstart.p, bs.data(), bs.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  CBS_HIP(hipStreamSynchronize(s));
+  L.probs = d_probs.as<DevProb<T>>();
+  L.bstart = d_bstart.as<int>();
+  L.nprob = np;
+}
+template struct Pipe<double>;
+template struct Pipe<float>;
+
+void StateSet::build(Ctx* c, const std::vector<RState*>& st, const std::vector<double*>& ph,
+                     const std::vector<int>& nt, hipStream_t s) {
+  n = (int)st.size();
+  sts.ensure(c, n * sizeof(RState*));
+  parts.ensure(c, n * sizeof(double*));
+  ntiles.ensure(c, n * sizeof(int));
+  sums.ensure(c, n * kNumPart * sizeof(double));
+  CBS_HIP(hipMemcpyAsync(sts.p, st.data(), n * sizeof(RState*), hipMemcpyHostToDevice, s));
+  CBS_HIP(hipMemcpyAsync(parts.p, ph.data(), n * sizeof(double*), hipMemcpyHostToDevice, s));
+  CBS_HIP(hipMemcpyAsync(ntiles.p, nt.data(), n * sizeof(int), hipMemcpyHostToDevice, s));
+  CBS_HIP(hipStreamSynchronize(s));
+}
+
+template <class T>
+void Work<T>::ensure(Ctx* c, long long nvox, int s, int ntiles, bool need_r) {
+  const size_t f = (size_t)nvox * 3 * sizeof(Cx<T>);
+  X[0].ensure(c, f);
+  X[1].ensure(c, f);
+  B.ensure(c, f);
+  W.ensure(c, f * s);
+  if (need_r) R.ensure(c, f);
+  const long long chunk = (long long)kThreads * 16;
+  const size_t np = std::max<size_t>((size_t)ntiles, (size_t)((nvox + chunk - 1) / chunk));
+  part.ensure(c, np * kNumPart * sizeof(double));
+  st.ensure(c, sizeof(RState));
+  nv = nvox;
+  S = s;
+}
+template struct Work<double>;
+template struct Work<float>;
+
+// ---------------------------------------------------------------------------
+// single applies (Green, operator, curl)
+// ---------------------------------------------------------------------------
+template <class T>
+static Work<T>& work(Problem& p);
+template <>
+Work<double>& work<double>(Problem& p) {
+  return p.w64;
+}
+template <>
+Work<float>& work<float>(Problem& p) {
+  return p.w32;
+}
+
+template <class T>
+void green_apply_dev(Problem& p, Cx<T>* field, hipStream_t s) {
+  Work<T>& w = work<T>(p);
+  DevProb<T> probe = make_desc<T>(p, PRE_COPY, ZOP_GREEN, POST_STORE, field, nullptr, field,
+                                  nullptr, nullptr, nullptr);
+  w.ensure(p.ctx, p.nvox, p.S, probe.tiles[3], false);
+  Pipe<T> pipe;
+  pipe.h.push_back(make_desc<T>(p, PRE_COPY, ZOP_GREEN, POST_STORE, field, nullptr, field,
+                                w.W.template as<Cx<T>>(), nullptr, nullptr));
+  pipe.build(p.ctx, s);
+  pipe.launch(s);
+  CBS_HIP(hipGetLastError());
+  CBS_HIP(hipStreamSynchronize(s));
+}
+
+template <class T>
+void operator_apply_dev(Problem& p, Cx<T>* y, const Cx<T>* x, double ar, double ai, double br,
+                        double bi, hipStream_t s) {
+  Work<T>& w = work<T>(p);
+  DevProb<T> d = make_desc<T>(p, PRE_V, ZOP_GREEN, POST_OPERATOR, x, nullptr, y, nullptr,
+                              nullptr, nullptr);
+  w.ensure(p.ctx, p.nvox, p.S, d.tiles[3], false);
+  d.w = w.W.template as<Cx<T>>();
+  d.a_re = ar;
+  d.a_im = ai;
+  d.b_re = br;
+  d.b_im = bi;
+  Pipe<T> pipe;
+  pipe.h.push_back(d);
+  pipe.build(p.ctx, s);
+  pipe.launch(s);
+  CBS_HIP(hipGetLastError());
+  CBS_HIP(hipStreamSynchronize(s));
+}
+
+template <class T>
+void curl_apply_dev(Problem& p, const Cx<T>* E, Cx<T>* H, double sr, double si, hipStream_t s) {
+  Work<T>& w = work<T>(p);
+  DevProb<T> d =
+      make_desc<T>(p, PRE_COPY, ZOP_CURL, POST_STORE, E, nullptr, H, nullptr, nullptr, nullptr);
+  w.ensure(p.ctx, p.nvox, p.S, d.tiles[3], false);
+  d.w = w.W.template as<Cx<T>>();
+  // H = scale * IFFT(i k x FFT(E)); the 1/N of the inverse transform is folded in
+  d.zs_re = sr / (double)p.nvox;
+  d.zs_im = si / (double)p.nvox;
+  Pipe<T> pipe;
+  pipe.h.push_back(d);
+  pipe.build(p.ctx, s);
+  pipe.launch(s);
+  CBS_HIP(hipGetLastError());
+  CBS_HIP(hipStreamSynchronize(s));
+}
+
+#define CBS_APPLY_INST(T)                                                                   \
+  template void green_apply_dev<T>(Problem&, Cx<T>*, hipStream_t);                          \
+  template void operator_apply_dev<T>(Problem&, Cx<T>*, const Cx<T>*, double, double, double, \
+                                      double, hipStream_t);                                 \
+  template void curl_apply_dev<T>(Problem&, const Cx<T>*, Cx<T>*, double, double, hipStream_t);
+CBS_APPLY_INST(double)
+CBS_APPLY_INST(float)
+
+// ---------------------------------------------------------------------------
+// validation / footprint
+// ---------------------------------------------------------------------------
+void validate_desc(const cbs_problem_desc& d) {
+  for (int k = 0; k < 3; ++k) {
+    CBS_REQUIRE(d.n[k] > 0, "all grid_size components must be positive");
+    CBS_REQUIRE(d.n[k] <= 4096, "grid axis longer than 4096 is not supported");
+    CBS_REQUIRE(d.res[k] > 0, "all resolution components must be positive");
+    CBS_REQUIRE(d.shift[k] == 0.0 || d.shift[k] == 0.25 || d.shift[k] == -0.25,
+                "subpixel shift must be 0 or 0.25");
+  }
+  CBS_REQUIRE(d.eps_imag > 0 && std::isfinite(d.eps_imag), "eps_imag must be positive");
+  CBS_REQUIRE(std::isfinite(d.k2_re) && std::isfinite(d.k2_im), "k_square must be finite");
+  CBS_REQUIRE(d.precision >= CBS_FP64 && d.precision <= CBS_FP32_REFINE, "unknown precision");
+  CBS_REQUIRE(d.averaging == 0 || d.averaging == 1, "averaging must be 0 or 1");
+}
+
+uint64_t footprint_bytes(const cbs_problem_desc& d, const cbs_solve_opts& o) {
+  const uint64_t nv = (uint64_t)d.n[0] * d.n[1] * d.n[2];
+  const uint64_t S = d.averaging ? 2 : 1;
+  const uint64_t f64 = nv * 3 * 16, f32 = nv * 3 * 8;
+  uint64_t b = nv * 16 /*V64*/ + f64 /*S*/ + f64 /*E*/;
+  if (o.precision == CBS_FP64) {
+    b += f64 * (3 + S);
+  } else {
+    b += nv * 8 /*V32*/ + f32 * (3 + S) + f32;
+    if (o.precision == CBS_FP32_REFINE) b += f64 * (3 + S) + f64;
+  }
+  if (o.method == CBS_BICGSTAB) b += (o.precision == CBS_FP64 ? f64 : f32) * 6;
+  if (o.method == CBS_GMRES) {
+    const uint64_t m = o.gmres_restart > 0 ? (uint64_t)o.gmres_restart : (uint64_t)o.maxiters;
+    b += (o.precision == CBS_FP64 ? f64 : f32) * (m + 3);
+  }
+  return b + (4u << 20);
+}
+
+}  // namespace cbs

@@ -1,0 +1,258 @@
+// Stockham auto-sort FFT over tiles held in LDS.
+//
+// A tile is Q pencils of length n.  Element m of pencil q lives at
+// s[m * QS + q] (QS >= Q), so lanes with consecutive q touch consecutive LDS
+// words.  Stage (radix R, span NS): butterfly t of pencil q reads
+// x[t + r*n/R] (r < R), multiplies by w^(j r) with j = t mod NS and
+// w = exp(-+2 pi i/(NS R)), runs a radix-R DFT and writes
+// y[(t/NS)*NS*R + j + r*NS]; after the last stage the pencil is in natural
+// order (Govindaraju et al., "High performance discrete Fourier transforms on
+// graphics processors", SC 2008).
+//
+// DIR = -1: forward exp(-2 pi i ...), unnormalised; DIR = +1: inverse without
+// the 1/n factor (the callers fold 1/N into the k-space multiply).
+#pragma once
+
+#include "cbs_internal.h"
+
+namespace cbs {
+
+template <class T>
+struct K {
+  static constexpr T s3 = T(0.86602540378443864676372317075294);  // sin(2 pi/3)
+  static constexpr T r2 = T(0.70710678118654752440084436210485);  // 1/sqrt(2)
+  static constexpr T c51 = T(0.30901699437494742410229341718282);   // cos(2 pi/5)
+  static constexpr T c52 = T(-0.80901699437494742410229341718282);  // cos(4 pi/5)
+  static constexpr T s51 = T(0.95105651629515357211643933337938);   // sin(2 pi/5)
+  static constexpr T s52 = T(0.58778525229247312916870595463907);   // sin(4 pi/5)
+};
+
+// multiply by DIR*i
+template <int DIR, class T>
+__device__ __forceinline__ Cx<T> mul_di(Cx<T> a) {
+  return DIR > 0 ? mk<T>(-a.im, a.re) : mk<T>(a.im, -a.re);
+}
+
+template <int DIR, class T>
+__device__ __forceinline__ void dft2(Cx<T>* v) {
+  const Cx<T> a = v[0], b = v[1];
+  v[0] = a + b;
+  v[1] = a - b;
+}
+
+template <int DIR, class T>
+__device__ __forceinline__ void dft3(Cx<T>* v) {
+  const Cx<T> t = v[1] + v[2];
+  const Cx<T> u = mul_di<DIR>(K<T>::s3 * (v[1] - v[2]));
+  const Cx<T> m = v[0] - T(0.5) * t;
+  v[0] = v[0] + t;
+  v[1] = m + u;
+  v[2] = m - u;
+}
+
+template <int DIR, class T>
+__device__ __forceinline__ void dft4(Cx<T>* v) {
+  const Cx<T> t0 = v[0] + v[2], t1 = v[0] - v[2];
+  const Cx<T> t2 = v[1] + v[3], t3 = mul_di<DIR>(v[1] - v[3]);
+  v[0] = t0 + t2;
+  v[2] = t0 - t2;
+  v[1] = t1 + t3;
+  v[3] = t1 - t3;
+}
+
+template <int DIR, class T>
+__device__ __forceinline__ void dft5(Cx<T>* v) {
+  const Cx<T> t1 = v[1] + v[4], t2 = v[2] + v[3];
+  const Cx<T> t3 = v[1] - v[4], t4 = v[2] - v[3];
+  const Cx<T> a1 = v[0] + K<T>::c51 * t1 + K<T>::c52 * t2;
+  const Cx<T> a2 = v[0] + K<T>::c52 * t1 + K<T>::c51 * t2;
+  const Cx<T> b1 = mul_di<DIR>(K<T>::s51 * t3 + K<T>::s52 * t4);
+  const Cx<T> b2 = mul_di<DIR>(K<T>::s52 * t3 - K<T>::s51 * t4);
+  v[0] = v[0] + t1 + t2;
+  v[1] = a1 + b1;
+  v[4] = a1 - b1;
+  v[2] = a2 + b2;
+  v[3] = a2 - b2;
+}
+
+template <int DIR, class T>
+__device__ __forceinline__ void dft8(Cx<T>* v) {
+  Cx<T> e[4] = {v[0], v[2], v[4], v[6]};
+  Cx<T> o[4] = {v[1], v[3], v[5], v[7]};
+  dft4<DIR>(e);
+  dft4<DIR>(o);
+  // o[k] *= w8^k,  w8 = exp(DIR 2 pi i / 8)
+  o[1] = K<T>::r2 * (o[1] + mul_di<DIR>(o[1]));
+  o[2] = mul_di<DIR>(o[2]);
+  o[3] = K<T>::r2 * (mul_di<DIR>(o[3]) - o[3]);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    v[k] = e[k] + o[k];
+    v[k + 4] = e[k] - o[k];
+  }
+}
+
+// generic radix R (any R), roots of unity from the length-n twiddle table
+template <int DIR, int R, class T>
+__device__ __forceinline__ void dftg(Cx<T>* v, const Cx<T>* __restrict__ tw, int n) {
+  Cx<T> out[R];
+  const int step = n / R;
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    Cx<T> acc = v[0];
+#pragma unroll
+    for (int m = 1; m < R; ++m) {
+      Cx<T> w = tw[((k * m) % R) * step];
+      if (DIR > 0) w.im = -w.im;
+      acc = acc + v[m] * w;
+    }
+    out[k] = acc;
+  }
+#pragma unroll
+  for (int k = 0; k < R; ++k) v[k] = out[k];
+}
+
+template <int DIR, int R, class T>
+__device__ __forceinline__ void dft(Cx<T>* v, const Cx<T>* __restrict__ tw, int n) {
+  if constexpr (R == 2) dft2<DIR>(v);
+  else if constexpr (R == 3) dft3<DIR>(v);
+  else if constexpr (R == 4) dft4<DIR>(v);
+  else if constexpr (R == 5) dft5<DIR>(v);
+  else if constexpr (R == 8) dft8<DIR>(v);
+  else dftg<DIR, R>(v, tw, n);
+}
+
+// ---------------------------------------------------------------------------
+// compile-time transform: in place, registers hold one stage's inputs
+// ---------------------------------------------------------------------------
+template <class T, int N, int Q, int QS, int DIR, int R, int NS>
+__device__ __forceinline__ void stage_ct(Cx<T>* __restrict__ s, const Cx<T>* __restrict__ tw,
+                                         int tid) {
+  constexpr int NB = N / R;
+  constexpr int TOTAL = Q * NB;
+  constexpr int RD = (TOTAL + kThreads - 1) / kThreads;
+  constexpr bool EXACT = (TOTAL % kThreads) == 0;
+  Cx<T> v[RD][R];
+#pragma unroll
+  for (int k = 0; k < RD; ++k) {
+    const int b = tid + k * kThreads;
+    if (EXACT || b < TOTAL) {
+      const int q = b % Q, t = b / Q;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[k][r] = s[(t + r * NB) * QS + q];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < RD; ++k) {
+    const int b = tid + k * kThreads;
+    if (EXACT || b < TOTAL) {
+      const int q = b % Q, t = b / Q;
+      const int j = t % NS;
+      if constexpr (NS > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          Cx<T> w = tw[j * r * (N / (NS * R))];
+          if (DIR > 0) w.im = -w.im;
+          v[k][r] = v[k][r] * w;
+        }
+      }
+      dft<DIR, R>(v[k], tw, N);
+      const int d = (t / NS) * NS * R + j;
+#pragma unroll
+      for (int r = 0; r < R; ++r) s[(d + r * NS) * QS + q] = v[k][r];
+    }
+  }
+  __syncthreads();
+}
+
+template <class T, int N, int Q, int QS, int DIR, int S = 0, int NS = 1>
+__device__ __forceinline__ void fft_ct(Cx<T>* __restrict__ s, const Cx<T>* __restrict__ tw,
+                                       int tid) {
+  if constexpr (S < plan_nst(N)) {
+    constexpr int R = plan_radix(N, S);
+    stage_ct<T, N, Q, QS, DIR, R, NS>(s, tw, tid);
+    fft_ct<T, N, Q, QS, DIR, S + 1, NS * R>(s, tw, tid);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// runtime-plan transform: ping-pong between two LDS buffers; returns the
+// buffer holding the result.  Used for lengths without a specialisation.
+// ---------------------------------------------------------------------------
+template <class T, int DIR, int R>
+__device__ __forceinline__ void bfly_rt(const Cx<T>* __restrict__ a, Cx<T>* __restrict__ b,
+                                        const Cx<T>* __restrict__ tw, int n, int QS, int q,
+                                        int t, int NS) {
+  const int NB = n / R;
+  const int j = t % NS;
+  const int tws = n / (NS * R);
+  Cx<T> v[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    v[r] = a[(t + r * NB) * QS + q];
+    if (r > 0 && NS > 1) {
+      Cx<T> w = tw[j * r * tws];
+      if (DIR > 0) w.im = -w.im;
+      v[r] = v[r] * w;
+    }
+  }
+  dft<DIR, R>(v, tw, n);
+  const int d = (t / NS) * NS * R + j;
+#pragma unroll
+  for (int r = 0; r < R; ++r) b[(d + r * NS) * QS + q] = v[r];
+}
+
+template <class T, int DIR>
+__device__ __forceinline__ void bfly_rt_generic(const Cx<T>* __restrict__ a,
+                                                Cx<T>* __restrict__ b,
+                                                const Cx<T>* __restrict__ tw, int n, int QS,
+                                                int q, int t, int NS, int R) {
+  const int NB = n / R;
+  const int j = t % NS;
+  const int tws = n / (NS * R);
+  const int step = n / R;
+  const int d = (t / NS) * NS * R + j;
+  for (int k = 0; k < R; ++k) {
+    Cx<T> acc = mk<T>(T(0), T(0));
+    for (int m = 0; m < R; ++m) {
+      Cx<T> x = a[(t + m * NB) * QS + q];
+      // stage twiddle w^(j m) times DFT root w_R^(k m)
+      const int e = (j * m * tws + ((k * m) % R) * step) % n;
+      Cx<T> w = tw[e];
+      if (DIR > 0) w.im = -w.im;
+      acc = acc + x * w;
+    }
+    b[(d + k * NS) * QS + q] = acc;
+  }
+}
+
+template <class T, int DIR>
+__device__ Cx<T>* fft_rt(Cx<T>* a, Cx<T>* b, int n, int Q, int QS, const int* __restrict__ plan,
+                         const Cx<T>* __restrict__ tw, int tid) {
+  const int nst = plan[0];
+  int NS = 1;
+  for (int st = 0; st < nst; ++st) {
+    const int R = plan[1 + st];
+    const int total = Q * (n / R);
+    for (int bi = tid; bi < total; bi += kThreads) {
+      const int q = bi % Q, t = bi / Q;
+      switch (R) {
+        case 2: bfly_rt<T, DIR, 2>(a, b, tw, n, QS, q, t, NS); break;
+        case 3: bfly_rt<T, DIR, 3>(a, b, tw, n, QS, q, t, NS); break;
+        case 4: bfly_rt<T, DIR, 4>(a, b, tw, n, QS, q, t, NS); break;
+        case 5: bfly_rt<T, DIR, 5>(a, b, tw, n, QS, q, t, NS); break;
+        case 8: bfly_rt<T, DIR, 8>(a, b, tw, n, QS, q, t, NS); break;
+        default: bfly_rt_generic<T, DIR>(a, b, tw, n, QS, q, t, NS, R); break;
+      }
+    }
+    __syncthreads();
+    Cx<T>* tmp = a;
+    a = b;
+    b = tmp;
+    NS *= R;
+  }
+  return a;
+}
+
+}  // namespace cbs

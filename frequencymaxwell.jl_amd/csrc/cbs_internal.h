@@ -1,0 +1,256 @@
+// Internal declarations of the MI355X CBS engine (device descriptors, complex
+// arithmetic, FFT radix plans, tile geometry).  Shared by every .hip unit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cbs {
+
+constexpr int kThreads = 256;     // threads per block of every pass kernel
+constexpr int kMaxStages = 12;    // radix stages of one axis transform
+constexpr int kLdsBudget = 65536; // LDS bytes a pass tile may use (two blocks per CU)
+constexpr int kNumPart = 5;       // doubles of partial sums written per xinv tile
+
+// ---------------------------------------------------------------------------
+// complex arithmetic (interleaved re, im; 8 or 16 bytes, naturally aligned)
+// ---------------------------------------------------------------------------
+template <class T>
+struct alignas(2 * sizeof(T)) Cx {
+  T re, im;
+};
+
+template <class T>
+__host__ __device__ __forceinline__ Cx<T> mk(T r, T i) {
+  Cx<T> c;
+  c.re = r;
+  c.im = i;
+  return c;
+}
+template <class T>
+__host__ __device__ __forceinline__ Cx<T> operator+(Cx<T> a, Cx<T> b) {
+  return mk<T>(a.re + b.re, a.im + b.im);
+}
+template <class T>
+__host__ __device__ __forceinline__ Cx<T> operator-(Cx<T> a, Cx<T> b) {
+  return mk<T>(a.re - b.re, a.im - b.im);
+}
+template <class T>
+__host__ __device__ __forceinline__ Cx<T> operator*(Cx<T> a, Cx<T> b) {
+  return mk<T>(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
+}
+template <class T>
+__host__ __device__ __forceinline__ Cx<T> operator*(T s, Cx<T> a) {
+  return mk<T>(s * a.re, s * a.im);
+}
+template <class T>
+__host__ __device__ __forceinline__ Cx<T> conj(Cx<T> a) {
+  return mk<T>(a.re, -a.im);
+}
+// conj(a) * b
+template <class T>
+__host__ __device__ __forceinline__ Cx<T> cdot(Cx<T> a, Cx<T> b) {
+  return mk<T>(a.re * b.re + a.im * b.im, a.re * b.im - a.im * b.re);
+}
+template <class T>
+__host__ __device__ __forceinline__ T abs2(Cx<T> a) {
+  return a.re * a.re + a.im * a.im;
+}
+// multiply by i
+template <class T>
+__host__ __device__ __forceinline__ Cx<T> mul_i(Cx<T> a) {
+  return mk<T>(-a.im, a.re);
+}
+template <class D, class S>
+__host__ __device__ __forceinline__ Cx<D> cvt(Cx<S> a) {
+  return mk<D>(static_cast<D>(a.re), static_cast<D>(a.im));
+}
+__host__ __device__ __forceinline__ Cx<double> cdiv(Cx<double> a, Cx<double> b) {
+  const double d = b.re * b.re + b.im * b.im;
+  return mk<double>((a.re * b.re + a.im * b.im) / d, (a.im * b.re - a.re * b.im) / d);
+}
+
+// ---------------------------------------------------------------------------
+// radix plans: n = r0 * r1 * ... ; powers of two use radix 8 with one or two
+// radix-4 stages, odd factors 3, 5, 7 and any remaining prime as a generic
+// radix.  The same function runs on the host (runtime plans) and at compile
+// time (specialised lengths).
+// ---------------------------------------------------------------------------
+struct RadixPlan {
+  int nst;
+  int r[kMaxStages];
+};
+
+__host__ __device__ constexpr RadixPlan make_plan(int n) {
+  RadixPlan p{0, {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0}};
+  int e = 0;
+  while (n % 2 == 0 && n > 1) {
+    n /= 2;
+    ++e;
+  }
+  // odd factors first (stage 1 carries no twiddles)
+  const int odd[3] = {3, 5, 7};
+  for (int k = 0; k < 3; ++k)
+    while (n % odd[k] == 0 && n > 1) {
+      p.r[p.nst++] = odd[k];
+      n /= odd[k];
+    }
+  for (int f = 11; n > 1 && f * f <= n; f += 2)
+    while (n % f == 0) {
+      p.r[p.nst++] = f;
+      n /= f;
+    }
+  if (n > 1) p.r[p.nst++] = n;
+  int n8 = e / 3, rem = e % 3;
+  if (rem == 1 && n8 > 0) {  // 2^(3k+1) = 8^(k-1) * 4 * 4
+    --n8;
+    p.r[p.nst++] = 4;
+    p.r[p.nst++] = 4;
+  } else if (rem == 1) {
+    p.r[p.nst++] = 2;
+  } else if (rem == 2) {
+    p.r[p.nst++] = 4;
+  }
+  for (int k = 0; k < n8; ++k) p.r[p.nst++] = 8;
+  return p;
+}
+
+__host__ __device__ constexpr int plan_nst(int n) { return make_plan(n).nst; }
+__host__ __device__ constexpr int plan_radix(int n, int s) { return make_plan(n).r[s]; }
+__host__ __device__ constexpr int plan_maxradix(int n) {
+  RadixPlan p = make_plan(n);
+  int m = 1;
+  for (int s = 0; s < p.nst; ++s) m = p.r[s] > m ? p.r[s] : m;
+  return m;
+}
+
+// Lengths with compile-time specialised transforms.  Any other length runs
+// the runtime-plan transform (ping-pong LDS, generic radices).
+__host__ __device__ constexpr bool is_ct_length(int n) {
+  return n == 8 || n == 16 || n == 32 || n == 64 || n == 96 || n == 128 || n == 160 ||
+         n == 192 || n == 256 || n == 288;
+}
+
+// ---------------------------------------------------------------------------
+// tile geometry.  Tiles of the strided axes (y, z) are P consecutive x
+// columns; tiles of the x axis are P consecutive lines.  P is the largest
+// power of two (<= pmax) whose tile fits the LDS budget.
+// ---------------------------------------------------------------------------
+__host__ __device__ constexpr int tile_p(int n, int elem_bytes, int pencils_per_p, int pmax,
+                                         bool runtime_plan, int pad) {
+  int p = pmax;
+  const int bufs = runtime_plan ? 2 : 1;
+  while (p > 1 && bufs * (n * (pencils_per_p * p + pad)) * elem_bytes > kLdsBudget) p /= 2;
+  return p;
+}
+// x-axis passes: S streams * 3 components * P lines per tile
+__host__ __device__ constexpr int xlines(int nx, int eb, int S, bool rt) {
+  return tile_p(nx, eb, 3 * S, 16, rt, 1);
+}
+// y pass: P columns of one component
+__host__ __device__ constexpr int ycols(int ny, int eb, bool rt) {
+  return tile_p(ny, eb, 1, eb == 8 ? 32 : 16, rt, 0);
+}
+// z pass: P columns of three components (Green / curl need all three)
+__host__ __device__ constexpr int zcols(int nz, int eb, bool rt) {
+  return tile_p(nz, eb, 3, eb == 8 ? 16 : 8, rt, 0);
+}
+__host__ __device__ constexpr int xpad_stride(int q) { return q + 1; }
+
+// ---------------------------------------------------------------------------
+// device descriptors
+// ---------------------------------------------------------------------------
+enum PreOp { PRE_COPY = 0, PRE_V = 1 };
+enum ZOp { ZOP_GREEN = 0, ZOP_CURL = 1 };
+enum PostOp {
+  POST_STORE = 0,    // out0 = w
+  POST_OPERATOR = 1, // out0 = alpha (in0 - w) + beta out0
+  POST_BORN = 2,     // r = gamma (in1 - in0 + w); out0 = in0 + r; part: |r|^2
+  POST_RESID = 3,    // out0 = in1 - in0 + w;  part: |gamma out0|^2
+  POST_PMATVEC = 4   // out0 = gamma (in0 - w); part: <in1,out0>, <out0,in0>, |out0|^2
+};
+
+// Per right-hand-side solver state kept on the device so that an iteration
+// never needs the host (and can be captured in a graph).
+struct RState {
+  double bnorm2;   // ||gamma b||^2
+  double thr2;     // stopping threshold squared (< 0: set from the first residual)
+  double r2;       // last residual norm squared
+  double tol, abstol;
+  int rule;        // CBS_STOP_*
+  int iters;       // operator applications
+  int maxit;
+  int done;        // 0 running, 1 converged, 2 stopped at maxit, 3 numeric failure
+  int ans;         // Born: index of the buffer holding the answer
+  int pad_;
+  // BiCGStab scalars
+  double rho_re, rho_im, alpha_re, alpha_im, omega_re, omega_im;
+  double hist[512];  // residual history (relative), first 512 entries
+};
+
+template <class T>
+struct DevProb {
+  int n[3];
+  int S;         // 1 or 2 pipelines (averaging over +/- shift)
+  int has_ramp;  // any axis shift non-zero
+  int pre, zop, post;
+  int rt[3];     // axis uses the runtime-plan transform
+  long long nvox;
+  const Cx<T>* tw[3];   // twiddles exp(-2 pi i q / n_d), q < n_d
+  const int* plan[3];   // runtime plans {nst, r0, r1, ...}
+  const Cx<T>* ramp[3]; // exp(-2 pi i idx s_d / n_d)  (+ shift pipeline)
+  const double* kk[3][2];  // Green k along each axis for the +/- pipelines
+  const double* kc[3];     // curl k along each axis
+  double k2c_re, k2c_im;   // k_square of the Green operator
+  double zs_re, zs_im;     // scale folded into the z-pass multiply
+  double eps_inv;          // 1 / eps  (gamma = i V / eps)
+  double a_re, a_im, b_re, b_im;  // POST_OPERATOR coefficients
+  const Cx<T>* V;
+  const Cx<T>* in0;
+  const Cx<T>* in1;
+  Cx<T>* out0;
+  Cx<T>* w;         // spectrum workspace [S][3][nvox]
+  double* part;     // partial sums of this RHS, kNumPart per xinv tile
+  const RState* st; // skip the launch work when st->done != 0 (may be null)
+  int tiles[4];     // tiles of the xfwd, y, z and xinv passes
+};
+
+// host-side tile counts (must match the kernels)
+template <class T>
+inline void count_tiles(DevProb<T>& p) {
+  const int eb = (int)sizeof(Cx<T>);
+  const int nx = p.n[0], ny = p.n[1], nz = p.n[2];
+  const int px = xlines(nx, eb, p.S, p.rt[0] != 0);
+  const int py = ycols(ny, eb, p.rt[1] != 0);
+  const int pz = zcols(nz, eb, p.rt[2] != 0);
+  const int lines = ny * nz;
+  p.tiles[0] = (lines + px - 1) / px;
+  p.tiles[1] = p.S * 3 * nz * ((nx + py - 1) / py);
+  p.tiles[2] = p.S * ny * ((nx + pz - 1) / pz);
+  p.tiles[3] = p.tiles[0];
+}
+
+// LDS bytes a pass tile of problem p needs
+template <class T>
+inline int tile_lds_bytes(const DevProb<T>& p, int pass) {
+  const int eb = (int)sizeof(Cx<T>);
+  const int nx = p.n[0], ny = p.n[1], nz = p.n[2];
+  switch (pass) {
+    case 0:
+    case 3: {
+      const bool rt = p.rt[0] != 0;
+      const int P = xlines(nx, eb, p.S, rt);
+      return (rt ? 2 : 1) * nx * xpad_stride(3 * p.S * P) * eb;
+    }
+    case 1: {
+      const bool rt = p.rt[1] != 0;
+      return (rt ? 2 : 1) * ny * ycols(ny, eb, rt) * eb;
+    }
+    default: {
+      const bool rt = p.rt[2] != 0;
+      return (rt ? 2 : 1) * nz * 3 * zcols(nz, eb, rt) * eb;
+    }
+  }
+}
+
+}  // namespace cbs

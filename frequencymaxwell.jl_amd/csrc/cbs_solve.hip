@@ -1,0 +1,475 @@
+// Iterative solvers of the CBS system (I - G_avg V) x = G_avg S with the left
+// preconditioner gamma = i V / eps (src/solvers/convergent_born.jl:703-774):
+//   * Born   -- preconditioned Richardson x <- x + gamma (b - A x), one
+//               fused five-sweep pipeline + one finalize launch per iteration,
+//               convergence decided on the device (graph-capturable);
+//   * FP32_REFINE -- FP64 outer loop r = b - A x (FP64 pipeline), FP32 inner
+//               Born solve of A d = r, x += d;
+//   * BiCGStab / GMRES -- see cbs_krylov.hip.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "cbs_engine.h"
+#include "cbs_solve.h"
+
+namespace cbs {
+
+template <class T>
+std::vector<std::unique_ptr<Work<T>>>& slots(Problem& p);
+template <>
+std::vector<std::unique_ptr<Work<double>>>& slots<double>(Problem& p) {
+  return p.wk64;
+}
+template <>
+std::vector<std::unique_ptr<Work<float>>>& slots<float>(Problem& p) {
+  return p.wk32;
+}
+
+template <class T>
+Work<T>& work_slot(Problem& p, int k) {
+  auto& v = slots<T>(p);
+  while ((int)v.size() <= k) v.emplace_back(new Work<T>());
+  return *v[k];
+}
+template Work<double>& work_slot<double>(Problem&, int);
+template Work<float>& work_slot<float>(Problem&, int);
+
+static unsigned nb256(long long n) { return (unsigned)((n + 255) / 256); }
+
+// ---------------------------------------------------------------------------
+// helpers shared by the solvers
+// ---------------------------------------------------------------------------
+template <class T>
+void copy_dev(Cx<T>* dst, const Cx<T>* src, long long n, hipStream_t s) {
+  CBS_HIP(hipMemcpyAsync(dst, src, (size_t)n * sizeof(Cx<T>), hipMemcpyDeviceToDevice, s));
+}
+
+// rhs_i = G_avg src_i for every item, one pipeline launch
+template <class T>
+void green_set(Ctx* ctx, std::vector<SolveItem<T>>& it, std::vector<const Cx<T>*>& src,
+               std::vector<Cx<T>*>& dst, hipStream_t s) {
+  Pipe<T> pipe;
+  for (size_t i = 0; i < it.size(); ++i)
+    pipe.h.push_back(make_desc<T>(*it[i].p, PRE_COPY, ZOP_GREEN, POST_STORE, src[i], nullptr,
+                                  dst[i], it[i].w->W.template as<Cx<T>>(), nullptr, nullptr));
+  pipe.build(ctx, s);
+  pipe.launch(s);
+  CBS_HIP(hipGetLastError());
+}
+
+// initialise the device state of every item from ||gamma b||^2
+template <class T>
+void init_states(Ctx* ctx, std::vector<SolveItem<T>>& it, double tol, double abstol, int rule,
+                 int maxit, hipStream_t s, StateSet& gn) {
+  std::vector<RState*> st;
+  std::vector<double*> parts;
+  std::vector<int> nt;
+  for (auto& x : it) {
+    const int nb = launch_gnorm_parts<T>(potential<T>(*x.p), x.b, x.p->nvox, x.p->eps_inv,
+                                         x.w->part.template as<double>(), s);
+    st.push_back(x.w->st.template as<RState>());
+    parts.push_back(x.w->part.template as<double>());
+    nt.push_back(nb);
+  }
+  gn.build(ctx, st, parts, nt, s);
+  launch_sum_parts(gn.parts.as<double*>(), gn.ntiles.as<int>(), gn.n, 1, gn.sums.as<double>(),
+                   s);
+  for (size_t i = 0; i < it.size(); ++i)
+    launch_state_init(st[i], gn.sums.as<double>() + i * kNumPart, tol, abstol, rule, maxit, s);
+  CBS_HIP(hipGetLastError());
+}
+
+template <class T>
+void read_states(std::vector<SolveItem<T>>& it, std::vector<RState>& out, hipStream_t s) {
+  out.resize(it.size());
+  for (size_t i = 0; i < it.size(); ++i)
+    CBS_HIP(hipMemcpyAsync(&out[i], it[i].w->st.p, sizeof(RState), hipMemcpyDeviceToHost, s));
+  CBS_HIP(hipStreamSynchronize(s));
+}
+
+// ---------------------------------------------------------------------------
+// Born iteration over a set of right-hand sides
+// ---------------------------------------------------------------------------
+template <class T>
+void born_solve(Ctx* ctx, std::vector<SolveItem<T>>& it, const IterOpts& io, hipStream_t s,
+                std::vector<RState>& states, std::vector<const Cx<T>*>& result) {
+  const int n = (int)it.size();
+  StateSet gn, fin;
+  // x0 = b (reference, convergent_born.jl:727-728) or x0 = 0 -> x1 = gamma b
+  for (auto& x : it) {
+    Cx<T>* X0 = x.w->X[0].template as<Cx<T>>();
+    if (io.x0_zero)
+      launch_gamma_mul<T>(X0, x.b, potential<T>(*x.p), x.p->nvox, x.p->eps_inv, s);
+    else
+      copy_dev<T>(X0, x.b, 3 * x.p->nvox, s);
+  }
+  init_states<T>(ctx, it, io.tol, io.abstol, io.rule, io.maxit, s, gn);
+
+  Pipe<T> pipe[2];
+  std::vector<RState*> sts;
+  std::vector<double*> parts;
+  std::vector<int> nt;
+  for (int par = 0; par < 2; ++par) {
+    for (auto& x : it) {
+      Cx<T>* Xi = x.w->X[par].template as<Cx<T>>();
+      Cx<T>* Xo = x.w->X[1 - par].template as<Cx<T>>();
+      pipe[par].h.push_back(make_desc<T>(*x.p, PRE_V, ZOP_GREEN, POST_BORN, Xi, x.b, Xo,
+                                         x.w->W.template as<Cx<T>>(),
+                                         x.w->part.template as<double>(),
+                                         x.w->st.template as<RState>()));
+    }
+    pipe[par].build(ctx, s);
+  }
+  for (int i = 0; i < n; ++i) {
+    sts.push_back(it[i].w->st.template as<RState>());
+    parts.push_back(it[i].w->part.template as<double>());
+    nt.push_back(pipe[0].h[i].tiles[3]);
+  }
+  fin.build(ctx, sts, parts, nt, s);
+  DevBuf gath;
+  gath.alloc(ctx, (size_t)n * 4 * sizeof(double));
+  std::vector<double> hg((size_t)n * 4);
+
+  auto step = [&](int par) {
+    pipe[par].launch(s);
+    launch_born_finalize(fin.sts.as<RState*>(), fin.parts.as<double*>(), fin.ntiles.as<int>(), n,
+                         par, s);
+  };
+  const int chunk = std::max(1, io.check_every);
+  const int chunk2 = io.use_graph ? 2 * ((chunk + 1) / 2) : chunk;
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+  if (io.use_graph) {
+    CBS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < chunk2; ++k) step(k & 1);
+    CBS_HIP(hipStreamEndCapture(s, &graph));
+    CBS_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+  }
+  int par = 0;
+  long long launched = 0;
+  const long long cap = (long long)io.maxit + 2 * chunk2 + 2;
+  try {
+    while (true) {
+      if (gexec) {
+        CBS_HIP(hipGraphLaunch(gexec, s));
+      } else {
+        for (int k = 0; k < chunk2; ++k) {
+          step(par);
+          par ^= 1;
+        }
+      }
+      CBS_HIP(hipGetLastError());
+      launched += chunk2;
+      launch_gather_state(fin.sts.as<RState*>(), n, gath.as<double>(), s);
+      CBS_HIP(hipMemcpyAsync(hg.data(), gath.p, hg.size() * sizeof(double),
+                             hipMemcpyDeviceToHost, s));
+      CBS_HIP(hipStreamSynchronize(s));
+      bool all = true;
+      for (int i = 0; i < n; ++i) all = all && hg[4 * i] != 0.0;
+      if (all || launched >= cap) break;
+    }
+  } catch (...) {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    throw;
+  }
+  if (gexec) (void)hipGraphExecDestroy(gexec);
+  if (graph) (void)hipGraphDestroy(graph);
+  read_states<T>(it, states, s);
+  result.resize(n);
+  for (int i = 0; i < n; ++i)
+    result[i] = it[i].w->X[states[i].ans & 1].template as<Cx<T>>();
+}
+
+// ---------------------------------------------------------------------------
+// FP64 residual r = b - A x over a set; returns ||gamma r||^2 per item
+// ---------------------------------------------------------------------------
+void resid64(Ctx* ctx, std::vector<SolveItem<double>>& it, std::vector<const Cx<double>*>& x,
+             std::vector<Cx<double>*>& r, std::vector<double>& g2, hipStream_t s) {
+  const int n = (int)it.size();
+  Pipe<double> pipe;
+  std::vector<RState*> sts;
+  std::vector<double*> parts;
+  std::vector<int> nt;
+  for (int i = 0; i < n; ++i) {
+    pipe.h.push_back(make_desc<double>(*it[i].p, PRE_V, ZOP_GREEN, POST_RESID, x[i], it[i].b,
+                                       r[i], it[i].w->W.as<Cx<double>>(),
+                                       it[i].w->part.as<double>(), nullptr));
+    sts.push_back(it[i].w->st.as<RState>());
+    parts.push_back(it[i].w->part.as<double>());
+  }
+  pipe.build(ctx, s);
+  for (int i = 0; i < n; ++i) nt.push_back(pipe.h[i].tiles[3]);
+  StateSet ss;
+  ss.build(ctx, sts, parts, nt, s);
+  pipe.launch(s);
+  launch_sum_parts(ss.parts.as<double*>(), ss.ntiles.as<int>(), n, 1, ss.sums.as<double>(), s);
+  CBS_HIP(hipGetLastError());
+  std::vector<double> h((size_t)n * kNumPart);
+  CBS_HIP(hipMemcpyAsync(h.data(), ss.sums.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  CBS_HIP(hipStreamSynchronize(s));
+  g2.resize(n);
+  for (int i = 0; i < n; ++i) g2[i] = h[(size_t)i * kNumPart];
+}
+
+// ||gamma v||^2 per item (FP64)
+void gnorm64(Ctx* ctx, std::vector<SolveItem<double>>& it, std::vector<double>& g2,
+             hipStream_t s) {
+  StateSet gn;
+  std::vector<RState*> st;
+  std::vector<double*> parts;
+  std::vector<int> nt;
+  for (auto& x : it) {
+    nt.push_back(launch_gnorm_parts<double>(potential<double>(*x.p), x.b, x.p->nvox,
+                                            x.p->eps_inv, x.w->part.as<double>(), s));
+    st.push_back(x.w->st.as<RState>());
+    parts.push_back(x.w->part.as<double>());
+  }
+  gn.build(ctx, st, parts, nt, s);
+  launch_sum_parts(gn.parts.as<double*>(), gn.ntiles.as<int>(), gn.n, 1, gn.sums.as<double>(), s);
+  std::vector<double> h((size_t)gn.n * kNumPart);
+  CBS_HIP(hipMemcpyAsync(h.data(), gn.sums.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  CBS_HIP(hipStreamSynchronize(s));
+  g2.resize(it.size());
+  for (size_t i = 0; i < it.size(); ++i) g2[i] = h[i * kNumPart];
+}
+
+// ---------------------------------------------------------------------------
+// solve_set: the C-ABI solves route here
+// ---------------------------------------------------------------------------
+static double algo_bytes_per_iter(const Problem& p, int prec) {
+  const double s = prec == CBS_FP64 ? 16.0 : 8.0;
+  return (p.S == 2 ? 62.0 : 38.0) * s * (double)p.nvox;
+}
+
+static void ensure_v32(Problem& p, hipStream_t s) {
+  if (!p.V32.p) {
+    p.V32.alloc(p.ctx, (size_t)p.nvox * sizeof(Cx<float>));
+    launch_convert<float, double>(p.V32.as<Cx<float>>(), p.V64.as<Cx<double>>(), p.nvox, s);
+    CBS_HIP(hipGetLastError());
+  }
+}
+
+template <class T>
+static void prepare_items(std::vector<RhsIO>& io, std::vector<SolveItem<T>>& it, bool need_r) {
+  std::map<Problem*, int> used;
+  it.clear();
+  for (auto& r : io) {
+    const int k = used[r.p]++;
+    Work<T>& w = work_slot<T>(*r.p, k);
+    DevProb<T> d = make_desc<T>(*r.p, PRE_V, ZOP_GREEN, POST_BORN, nullptr, nullptr, nullptr,
+                                nullptr, nullptr, nullptr);
+    w.ensure(r.p->ctx, r.p->nvox, r.p->S, d.tiles[3], need_r);
+    SolveItem<T> x;
+    x.p = r.p;
+    x.w = &w;
+    x.b = w.B.template as<Cx<T>>();
+    it.push_back(x);
+  }
+}
+
+static IterOpts iter_opts(const cbs_solve_opts& o, int maxit) {
+  IterOpts io;
+  io.tol = o.tol;
+  io.abstol = o.abstol;
+  io.rule = o.stop_rule;
+  io.maxit = maxit;
+  io.check_every = o.check_every;
+  io.use_graph = o.use_graph;
+  io.x0_zero = o.x0_zero;
+  io.restart = o.gmres_restart;
+  return io;
+}
+
+static void fill_stats(cbs_solve_stats& st, const RState& rs, const Problem& p, int prec) {
+  st.iters += rs.iters;
+  if (prec == CBS_FP64) st.iters_fp64 += rs.iters;
+  st.converged = rs.done == 1;
+  st.resid = rs.bnorm2 > 0 ? std::sqrt(rs.r2 / rs.bnorm2) : 0.0;
+  st.bytes_moved += algo_bytes_per_iter(p, prec) * rs.iters;
+}
+
+template <class T>
+void run_method(Ctx* ctx, std::vector<SolveItem<T>>& it, const IterOpts& io, int method,
+                hipStream_t s, std::vector<RState>& states, std::vector<const Cx<T>*>& result) {
+  if (method == CBS_BORN) born_solve<T>(ctx, it, io, s, states, result);
+  else if (method == CBS_BICGSTAB) bicgstab_solve<T>(ctx, it, io, s, states, result);
+  else gmres_solve<T>(ctx, it, io, s, states, result);
+}
+
+void solve_set(Ctx* ctx, std::vector<RhsIO>& io, const cbs_solve_opts& o,
+               std::vector<cbs_solve_stats>& stats, hipStream_t s) {
+  const int n = (int)io.size();
+  CBS_REQUIRE(n > 0, "no right-hand sides");
+  CBS_REQUIRE(o.tol >= 0 && o.tol <= 1, "tolerance must be in [0, 1]");
+  CBS_REQUIRE(o.maxiters >= 1, "maxiters must be >= 1");
+  CBS_REQUIRE(o.method >= CBS_BORN && o.method <= CBS_GMRES, "unknown method");
+  CBS_REQUIRE(o.precision >= CBS_FP64 && o.precision <= CBS_FP32_REFINE, "unknown precision");
+  CBS_REQUIRE(o.stop_rule == CBS_STOP_RHS || o.stop_rule == CBS_STOP_KRYLOV, "unknown stop rule");
+  for (auto& r : io) CBS_REQUIRE(r.p->S == io[0].p->S, "a set mixes averaging and periodic problems");
+  stats.assign(n, cbs_solve_stats{});
+  hipEvent_t e0, e1;
+  CBS_HIP(hipEventCreate(&e0));
+  CBS_HIP(hipEventCreate(&e1));
+  CBS_HIP(hipEventRecord(e0, s));
+
+  if (o.precision == CBS_FP64) {
+    std::vector<SolveItem<double>> it;
+    prepare_items<double>(io, it, false);
+    std::vector<const Cx<double>*> src;
+    std::vector<Cx<double>*> dst;
+    for (int i = 0; i < n; ++i) {
+      src.push_back(io[i].S);
+      dst.push_back(it[i].w->B.as<Cx<double>>());
+    }
+    green_set<double>(ctx, it, src, dst, s);  // rhs = G_avg S  (convergent_born.jl:715-723)
+    std::vector<RState> st;
+    std::vector<const Cx<double>*> res;
+    run_method<double>(ctx, it, iter_opts(o, o.maxiters), o.method, s, st, res);
+    for (int i = 0; i < n; ++i) {
+      copy_dev<double>(io[i].E, res[i], 3 * io[i].p->nvox, s);
+      fill_stats(stats[i], st[i], *io[i].p, CBS_FP64);
+    }
+  } else if (o.precision == CBS_FP32) {
+    for (auto& r : io) ensure_v32(*r.p, s);
+    std::vector<SolveItem<float>> it;
+    prepare_items<float>(io, it, false);
+    std::vector<const Cx<float>*> src;
+    std::vector<Cx<float>*> dst;
+    for (int i = 0; i < n; ++i) {
+      Cx<float>* tmp = it[i].w->X[1].as<Cx<float>>();
+      launch_convert<float, double>(tmp, io[i].S, 3 * io[i].p->nvox, s);
+      src.push_back(tmp);
+      dst.push_back(it[i].w->B.as<Cx<float>>());
+    }
+    green_set<float>(ctx, it, src, dst, s);
+    std::vector<RState> st;
+    std::vector<const Cx<float>*> res;
+    run_method<float>(ctx, it, iter_opts(o, o.maxiters), o.method, s, st, res);
+    for (int i = 0; i < n; ++i) {
+      launch_convert<double, float>(io[i].E, res[i], 3 * io[i].p->nvox, s);
+      fill_stats(stats[i], st[i], *io[i].p, CBS_FP32);
+    }
+  } else {
+    // FP32 bulk inside an FP64 iterative-refinement loop
+    for (auto& r : io) ensure_v32(*r.p, s);
+    std::vector<SolveItem<double>> it64;
+    prepare_items<double>(io, it64, true);
+    std::vector<SolveItem<float>> it32;
+    prepare_items<float>(io, it32, false);
+    std::vector<const Cx<double>*> src;
+    std::vector<Cx<double>*> dst, rs, xs;
+    for (int i = 0; i < n; ++i) {
+      src.push_back(io[i].S);
+      dst.push_back(it64[i].w->B.as<Cx<double>>());
+    }
+    green_set<double>(ctx, it64, src, dst, s);  // rhs in FP64
+    std::vector<double> bn2;
+    gnorm64(ctx, it64, bn2, s);
+    for (int i = 0; i < n; ++i) {
+      Cx<double>* X = io[i].E;  // the FP64 iterate lives in the output buffer
+      if (o.x0_zero) launch_fill<double>(X, 3 * io[i].p->nvox, s);
+      else copy_dev<double>(X, it64[i].b, 3 * io[i].p->nvox, s);
+      xs.push_back(X);
+      rs.push_back(it64[i].w->R.as<Cx<double>>());
+    }
+    std::vector<double> thr2(n, -1.0), g2;
+    std::vector<int> active(n, 1), used(n, 0);
+    const int max_refine = o.max_refine > 0 ? o.max_refine : 20;
+    for (int k = 0;; ++k) {
+      // FP64 true residual of every active item
+      std::vector<SolveItem<double>> a64;
+      std::vector<const Cx<double>*> ax;
+      std::vector<Cx<double>*> ar;
+      std::vector<int> idx;
+      for (int i = 0; i < n; ++i)
+        if (active[i]) {
+          a64.push_back(it64[i]);
+          ax.push_back(xs[i]);
+          ar.push_back(rs[i]);
+          idx.push_back(i);
+        }
+      if (idx.empty()) break;
+      resid64(ctx, a64, ax, ar, g2, s);
+      std::vector<int> go;
+      for (size_t j = 0; j < idx.size(); ++j) {
+        const int i = idx[j];
+        cbs_solve_stats& stt = stats[i];
+        stt.iters += 1;
+        stt.iters_fp64 += 1;
+        used[i] += 1;
+        stt.bytes_moved += algo_bytes_per_iter(*io[i].p, CBS_FP64);
+        if (thr2[i] < 0) {
+          const double t = o.stop_rule == CBS_STOP_RHS ? o.tol * std::sqrt(bn2[i])
+                                                       : o.abstol + o.tol * std::sqrt(g2[j]);
+          thr2[i] = t * t;
+        }
+        stt.resid = bn2[i] > 0 ? std::sqrt(g2[j] / bn2[i]) : 0.0;
+        stt.resid_true_fp64 = stt.resid;
+        if (bn2[i] == 0.0 || g2[j] <= thr2[i]) {
+          stt.converged = 1;
+          active[i] = 0;
+        } else if (!(g2[j] == g2[j])) {
+          active[i] = 0;
+        } else if (k >= max_refine || used[i] >= o.maxiters) {
+          active[i] = 0;
+        } else {
+          go.push_back(i);
+          stt.refine_steps += 1;
+        }
+      }
+      if (go.empty()) break;
+      // inner FP32 solve of A d = r (d0 = 0) to a relative tolerance eta
+      std::vector<SolveItem<float>> a32;
+      int inner_max = 0;
+      // eta: the reduction the outer test still needs (x 0.3), never below
+      // what an FP32 solve resolves (1e-5) nor above 0.1
+      double need = 1.0;
+      for (size_t j = 0; j < idx.size(); ++j)
+        if (active[idx[j]]) need = std::min(need, std::sqrt(thr2[idx[j]] / std::max(g2[j], 1e-300)));
+      for (int i : go) {
+        launch_convert<float, double>(it32[i].w->B.as<Cx<float>>(), rs[i], 3 * io[i].p->nvox, s);
+        a32.push_back(it32[i]);
+        inner_max = std::max(inner_max, o.maxiters - used[i]);
+      }
+      const double eta = o.inner_tol > 0 ? o.inner_tol : std::min(0.1, std::max(1e-5, 0.3 * need));
+      IterOpts io32 = iter_opts(o, std::max(1, inner_max));
+      io32.tol = eta;
+      io32.abstol = 0.0;
+      io32.rule = CBS_STOP_RHS;
+      io32.x0_zero = 1;
+      std::vector<RState> st;
+      std::vector<const Cx<float>*> res;
+      run_method<float>(ctx, a32, io32, o.method, s, st, res);
+      for (size_t j = 0; j < go.size(); ++j) {
+        const int i = go[j];
+        launch_accum<double, float>(xs[i], res[j], 3 * io[i].p->nvox, s);
+        stats[i].iters += st[j].iters;
+        used[i] += st[j].iters;
+        stats[i].bytes_moved += algo_bytes_per_iter(*io[i].p, CBS_FP32) * st[j].iters;
+      }
+      CBS_HIP(hipGetLastError());
+    }
+  }
+  CBS_HIP(hipEventRecord(e1, s));
+  CBS_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  CBS_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  for (int i = 0; i < n; ++i) {
+    stats[i].time_ms = ms;
+    stats[i].voxel_iters = (int64_t)stats[i].iters * io[i].p->nvox;
+  }
+}
+
+template void born_solve<double>(Ctx*, std::vector<SolveItem<double>>&, const IterOpts&,
+                                 hipStream_t, std::vector<RState>&,
+                                 std::vector<const Cx<double>*>&);
+template void born_solve<float>(Ctx*, std::vector<SolveItem<float>>&, const IterOpts&,
+                                hipStream_t, std::vector<RState>&,
+                                std::vector<const Cx<float>*>&);
+
+}  // namespace cbs

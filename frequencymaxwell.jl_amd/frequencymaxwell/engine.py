@@ -1,0 +1,207 @@
+"""Thin object layer over the C ABI: one ``CBSProblem`` per padded potential.
+
+Every method calls ``libcbs_mi355x.so``; there is no other compute path.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib as L
+
+_PREC = {"fp64": L.CBS_FP64, "fp32": L.CBS_FP32, "fp32_refine": L.CBS_FP32_REFINE}
+_METHOD = {"born": L.CBS_BORN, "bicgstab": L.CBS_BICGSTAB, "gmres": L.CBS_GMRES}
+_RULE = {"rhs": L.CBS_STOP_RHS, "krylov": L.CBS_STOP_KRYLOV}
+
+
+def precision_code(p) -> int:
+    if isinstance(p, int):
+        return p
+    try:
+        return _PREC[str(p).lower()]
+    except KeyError:
+        raise ValueError(f"unknown precision {p!r} (fp64, fp32, fp32_refine)") from None
+
+
+def method_code(m) -> int:
+    if isinstance(m, int):
+        return m
+    key = str(m).lower().replace("krylovjl_", "").replace("()", "")
+    try:
+        return _METHOD[key]
+    except KeyError:
+        raise ValueError(f"unknown linear solver {m!r} (gmres, bicgstab, born)") from None
+
+
+def make_opts(*, method="born", precision="fp64", tol=1e-6, abstol=0.0, stop_rule="rhs",
+              maxiters=1000, gmres_restart=0, max_refine=20, inner_tol=0.0, check_every=1,
+              use_graph=False, x0_zero=False, h_scale=0j, compute_h=True) -> L.SolveOpts:
+    o = L.default_opts()
+    o.method = method_code(method)
+    o.precision = precision_code(precision)
+    o.stop_rule = _RULE[stop_rule] if isinstance(stop_rule, str) else int(stop_rule)
+    o.maxiters = int(maxiters)
+    o.tol = float(tol)
+    o.abstol = float(abstol)
+    o.gmres_restart = int(gmres_restart)
+    o.max_refine = int(max_refine)
+    o.inner_tol = float(inner_tol)
+    o.check_every = int(check_every)
+    o.use_graph = int(bool(use_graph))
+    o.x0_zero = int(bool(x0_zero))
+    o.compute_h = int(bool(compute_h))
+    o.h_scale_re = float(complex(h_scale).real)
+    o.h_scale_im = float(complex(h_scale).imag)
+    return o
+
+
+@dataclass
+class GreenSpec:
+    """Parameters of the dyadic Green operator (src/core/dyadic_green.jl:45-92)."""
+
+    shape: tuple          # padded grid (Nx, Ny, Nz)
+    resolution: tuple
+    shifts: tuple         # per-axis subpixel shift
+    use_averaging: bool
+    k_square: complex     # k_bg^2 + i eps
+    eps_imag: float
+
+
+class CBSProblem:
+    """A padded potential V uploaded to the GPU plus its Green operator."""
+
+    def __init__(self, V: np.ndarray, spec: GreenSpec, precision="fp64", ctx: L.Context | None = None):
+        self.ctx = ctx or L.default_context()
+        self._lib = L.load()
+        self.spec = spec
+        self.shape = tuple(int(s) for s in spec.shape)
+        V = L.as_field(V, self.shape)
+        d = L.ProblemDesc()
+        for k in range(3):
+            d.n[k] = self.shape[k]
+            d.res[k] = float(spec.resolution[k])
+            d.shift[k] = float(spec.shifts[k])
+        d.averaging = int(bool(spec.use_averaging))
+        d.k2_re = float(complex(spec.k_square).real)
+        d.k2_im = float(complex(spec.k_square).imag)
+        d.eps_imag = float(spec.eps_imag)
+        d.precision = precision_code(precision)
+        self.desc = d
+        h = C.c_void_p()
+        L.check(self._lib.cbs_problem_create(self.ctx.handle, C.byref(d), L.dptr(V), C.byref(h)))
+        self.handle = h
+
+    @property
+    def field_shape(self):
+        return self.shape + (3,)
+
+    def close(self):
+        if self.handle:
+            L.check(self._lib.cbs_problem_destroy(self.handle))
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None):
+                self._lib.cbs_problem_destroy(self.handle)
+        except Exception:
+            pass
+
+    # -- operator layer --------------------------------------------------
+    def green_apply(self, field) -> np.ndarray:
+        """``conv!(G, src)`` on a copy: returns G_avg field."""
+        f = np.array(L.as_field(field, self.field_shape), order="F", copy=True)
+        L.check(self._lib.cbs_green_apply(self.handle, L.dptr(f)))
+        return f
+
+    def operator_apply(self, x, y=None, alpha=1.0, beta=0.0) -> np.ndarray:
+        """``mul!(y, op, x, alpha, beta)``: returns alpha (x - G_avg V x) + beta y."""
+        x = L.as_field(x, self.field_shape)
+        if y is None:
+            y = np.zeros(self.field_shape, dtype=np.complex128, order="F")
+        else:
+            y = np.array(L.as_field(y, self.field_shape), order="F", copy=True)
+        a, b = complex(alpha), complex(beta)
+        L.check(self._lib.cbs_operator_apply(self.handle, L.dptr(y), L.dptr(x), a.real, a.imag,
+                                             b.real, b.imag))
+        return y
+
+    def precond_apply(self, x) -> np.ndarray:
+        """``ldiv!(y, Pl, x)``: returns (i/eps) V x."""
+        x = L.as_field(x, self.field_shape)
+        y = np.empty(self.field_shape, dtype=np.complex128, order="F")
+        L.check(self._lib.cbs_precond_apply(self.handle, L.dptr(y), L.dptr(x)))
+        return y
+
+    def curl_apply(self, field, scale=1.0 + 0j) -> np.ndarray:
+        """``scale * conv(Curl, field)``."""
+        f = np.array(L.as_field(field, self.field_shape), order="F", copy=True)
+        s = complex(scale)
+        L.check(self._lib.cbs_curl_apply(self.handle, L.dptr(f), s.real, s.imag))
+        return f
+
+    # -- solves ------------------------------------------------------------
+    def solve(self, S, opts: L.SolveOpts, want_h: bool = True):
+        S = L.as_field(S, self.field_shape)
+        E = np.empty(self.field_shape, dtype=np.complex128, order="F")
+        H = np.empty(self.field_shape, dtype=np.complex128, order="F") if want_h else None
+        st = L.SolveStats()
+        L.check(self._lib.cbs_solve(self.handle, L.dptr(S), L.dptr(E),
+                                    L.dptr(H) if H is not None else None, C.byref(opts),
+                                    C.byref(st)))
+        return E, H, st
+
+    def set_source(self, S):
+        S = L.as_field(S, self.field_shape)
+        L.check(self._lib.cbs_problem_set_source(self.handle, L.dptr(S)))
+
+    def run(self, opts: L.SolveOpts) -> L.SolveStats:
+        st = L.SolveStats()
+        L.check(self._lib.cbs_problem_run(self.handle, C.byref(opts), C.byref(st)))
+        return st
+
+    def get_fields(self, h_scale=None):
+        E = np.empty(self.field_shape, dtype=np.complex128, order="F")
+        H = None
+        hs = 0j
+        if h_scale is not None:
+            H = np.empty(self.field_shape, dtype=np.complex128, order="F")
+            hs = complex(h_scale)
+        L.check(self._lib.cbs_problem_get_fields(self.handle, L.dptr(E),
+                                                 L.dptr(H) if H is not None else None,
+                                                 hs.real, hs.imag))
+        return E, H
+
+    def solve_multi(self, S_list, opts: L.SolveOpts, want_h: bool = True):
+        """Many right-hand sides of this problem in one set of launches per sweep."""
+        n = len(S_list)
+        Ss = [L.as_field(s, self.field_shape) for s in S_list]
+        Es = [np.empty(self.field_shape, dtype=np.complex128, order="F") for _ in range(n)]
+        Hs = [np.empty(self.field_shape, dtype=np.complex128, order="F") for _ in range(n)] if want_h else None
+        ArrD = L._D * n
+        sts = (L.SolveStats * n)()
+        L.check(self._lib.cbs_solve_multi(
+            self.handle, n, ArrD(*[L.dptr(a) for a in Ss]), ArrD(*[L.dptr(a) for a in Es]),
+            ArrD(*[L.dptr(a) for a in Hs]) if want_h else None, C.byref(opts), sts))
+        return Es, Hs, list(sts)
+
+
+def solve_batch(problems, S_list, opts: L.SolveOpts, want_h: bool = True, ctx=None):
+    """Problems of possibly different grids, one RHS each, swept together."""
+    ctx = ctx or problems[0].ctx
+    lib = L.load()
+    n = len(problems)
+    Ss = [L.as_field(s, p.field_shape) for p, s in zip(problems, S_list)]
+    Es = [np.empty(p.field_shape, dtype=np.complex128, order="F") for p in problems]
+    Hs = [np.empty(p.field_shape, dtype=np.complex128, order="F") for p in problems] if want_h else None
+    ArrD = L._D * n
+    ArrP = C.c_void_p * n
+    sts = (L.SolveStats * n)()
+    L.check(lib.cbs_solve_batch(ctx.handle, n, ArrP(*[p.handle for p in problems]),
+                                ArrD(*[L.dptr(a) for a in Ss]), ArrD(*[L.dptr(a) for a in Es]),
+                                ArrD(*[L.dptr(a) for a in Hs]) if want_h else None,
+                                C.byref(opts), sts))
+    return Es, Hs, list(sts)

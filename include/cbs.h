@@ -1,0 +1,201 @@
+/*
+ * cbs.h -- C ABI of the MI355X convergent-Born-series (CBS) engine.
+ *
+ * The library replaces the device side of FrequencyMaxwell.jl's solve path:
+ * the dyadic Green operator, the CBS linear operator and preconditioner, the
+ * curl used for H, and the Krylov loop that LinearSolve.jl runs.  Host code
+ * (potential build, incident plane wave, source term, crop) stays with the
+ * caller, exactly as in the reference.
+ *
+ * Conventions
+ *   - Complex numbers are interleaved (re, im) doubles: a Julia
+ *     Ptr{ComplexF64} passes as `double *`.
+ *   - Arrays are Julia column-major.  A scalar grid array (the potential) has
+ *     element (i,j,k) at  (k*Ny + j)*Nx + i;  a field array (nx,ny,nz,3) has
+ *     element (i,j,k,c) at ((c*Nz + k)*Ny + j)*Nx + i  (0-based indices).
+ *   - Complex scalars pass as separate re / im doubles.
+ *   - Every entry point returns a cbs_status (0 = success).  No C++ exception
+ *     crosses the ABI.  cbs_last_error() returns a thread-local message for
+ *     the last failing call on the calling thread.
+ *   - Non-convergence is NOT an error: the call returns CBS_OK with
+ *     stats->converged = 0 (the reference only warns,
+ *     src/solvers/convergent_born.jl:753-755).
+ *   - The caller owns every host buffer; the library copies in and out and
+ *     keeps no host pointer after a call returns.  The library owns every
+ *     device allocation; *_destroy frees it.
+ *   - A cbs_problem is used by one host thread at a time.  A cbs_ctx may be
+ *     shared by threads for the serving calls (cbs_submit / cbs_poll /
+ *     cbs_wait).
+ */
+#ifndef CBS_MI355X_H
+#define CBS_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t cbs_status;
+typedef struct cbs_ctx cbs_ctx;
+typedef struct cbs_problem cbs_problem;
+
+enum {
+  CBS_OK = 0,
+  CBS_EINVAL = -1,    /* bad argument: maps to Julia ArgumentError */
+  CBS_ENOMEM = -2,    /* device allocation failed / HBM budget refused */
+  CBS_EDEVICE = -3,   /* HIP runtime error */
+  CBS_ENOTFOUND = -4, /* unknown ticket */
+  CBS_EBUSY = -5,     /* serving queue full */
+  CBS_ECANCELED = -6, /* request was cancelled */
+  CBS_ENUMERIC = -7   /* NaN / Inf detected in an input or iterate */
+};
+
+/* Working precision of the iteration. */
+enum {
+  CBS_FP64 = 0,        /* complex FP64 everywhere ("plain FP64" path) */
+  CBS_FP32 = 1,        /* complex FP32 bulk, FP64 twiddles/ramps/norms */
+  CBS_FP32_REFINE = 2  /* FP32 bulk inside an FP64 iterative-refinement loop */
+};
+
+/* Iterative method. */
+enum {
+  CBS_BORN = 0,     /* CBS / preconditioned Richardson: x <- x + gamma (b - A x) */
+  CBS_BICGSTAB = 1, /* left-preconditioned BiCGStab (KrylovJL_BICGSTAB) */
+  CBS_GMRES = 2     /* left-preconditioned GMRES (KrylovJL_GMRES, the default) */
+};
+
+/* Stopping rule on the preconditioned residual r = gamma (b - A x). */
+enum {
+  CBS_STOP_RHS = 0,   /* ||r|| <= tol * ||gamma b||                       */
+  CBS_STOP_KRYLOV = 1 /* ||r|| <= abstol + tol * ||r0|| (Krylov.jl rule)  */
+};
+
+/* Problem description: the padded FFT grid and the DyadicGreen parameters
+ * (src/core/dyadic_green.jl:67-92, src/solvers/convergent_born.jl:619-675). */
+typedef struct {
+  int64_t n[3];        /* padded grid Nx, Ny, Nz                           */
+  double res[3];       /* voxel size dx, dy, dz (m)                        */
+  double shift[3];     /* subpixel shift per axis (0 periodic, 0.25 absorbing) */
+  int32_t averaging;   /* 1: G_avg = (G(+s) + G(-s))/2 (any axis absorbing) */
+  double k2_re;        /* k_square = k_bg^2 + i eps: real part             */
+  double k2_im;        /*                           imaginary part (= eps) */
+  double eps_imag;     /* eps of the preconditioner gamma = i V / eps      */
+  int32_t precision;   /* CBS_FP64 | CBS_FP32 | CBS_FP32_REFINE            */
+} cbs_problem_desc;
+
+typedef struct {
+  int32_t method;       /* CBS_BORN | CBS_BICGSTAB | CBS_GMRES              */
+  int32_t precision;    /* CBS_FP64 | CBS_FP32 | CBS_FP32_REFINE            */
+  int32_t stop_rule;    /* CBS_STOP_RHS | CBS_STOP_KRYLOV                   */
+  int32_t maxiters;     /* bound on operator applications                  */
+  double tol;           /* relative tolerance (reltol)                     */
+  double abstol;        /* absolute tolerance (CBS_STOP_KRYLOV only)       */
+  int32_t gmres_restart;/* 0 = no restart (Krylov.jl default)              */
+  int32_t max_refine;   /* FP32_REFINE: bound on FP64 refinement steps     */
+  double inner_tol;     /* FP32_REFINE: inner relative tolerance (0 = auto)*/
+  int32_t check_every;  /* host convergence check cadence (>=1)            */
+  int32_t use_graph;    /* capture check_every iterations as a hipGraph    */
+  int32_t x0_zero;      /* 1: x0 = 0; 0: x0 = rhs (reference, :727-728)    */
+  int32_t compute_h;    /* cbs_solve: also return H_scat                   */
+  double h_scale_re;    /* H = h_scale * curl(E): -i lambda/(2 pi 377)     */
+  double h_scale_im;
+} cbs_solve_opts;
+
+typedef struct {
+  int32_t iters;          /* operator applications (voxel-iterations / N_vox) */
+  int32_t iters_fp64;     /* of which in FP64                                */
+  int32_t refine_steps;   /* FP64 refinement steps (FP32_REFINE)             */
+  int32_t converged;      /* 1 when the stopping rule was met                */
+  double resid;           /* ||gamma (b - A x)|| / ||gamma b|| (FP64) of the returned x */
+  double resid_true_fp64; /* same, recomputed in FP64 after the solve (0 if not run) */
+  double time_ms;         /* device time of the solve                        */
+  double bytes_moved;     /* algorithmic HBM bytes of the solve              */
+  int64_t voxel_iters;    /* iters * N_vox                                   */
+} cbs_solve_stats;
+
+/* ---- context ------------------------------------------------------------ */
+cbs_status cbs_ctx_create(cbs_ctx **out, int32_t device, uint64_t hbm_budget_bytes);
+cbs_status cbs_ctx_destroy(cbs_ctx *ctx);
+const char *cbs_last_error(void);
+const char *cbs_version(void);
+void cbs_default_opts(cbs_solve_opts *opts);
+
+/* ---- problem ------------------------------------------------------------ */
+/* Upload the padded, attenuated potential V (complex, Nx*Ny*Nz) -- the
+ * `potential_device` of src/solvers/convergent_born.jl:650 -- and build the
+ * Green operator of src/core/dyadic_green.jl:67-92. */
+cbs_status cbs_problem_create(cbs_ctx *ctx, const cbs_problem_desc *desc, const double *V,
+                              cbs_problem **out);
+cbs_status cbs_problem_destroy(cbs_problem *p);
+
+/* conv!(::DyadicGreen, src): field <- G_avg field, in place.
+ * Replaces src/core/dyadic_green.jl:127-151. */
+cbs_status cbs_green_apply(cbs_problem *p, double *field_inout);
+
+/* mul!(y, ::CBSLinearOperator, x, alpha, beta): y <- alpha (x - G_avg(V x)) + beta y.
+ * Replaces src/solvers/convergent_born.jl:482-512. */
+cbs_status cbs_operator_apply(cbs_problem *p, double *y, const double *x, double a_re,
+                              double a_im, double b_re, double b_im);
+
+/* ldiv!(y, ::CBSPreconditioner, x): y <- (i/eps) V x.
+ * Replaces src/solvers/convergent_born.jl:398-410. */
+cbs_status cbs_precond_apply(cbs_problem *p, double *y, const double *x);
+
+/* conv(::Curl, E) scaled: field <- scale * curl(field).  With scale =
+ * -i lambda/(2 pi 377) this is _compute_magnetic_field,
+ * src/solvers/convergent_born.jl:784-800 / src/core/curl.jl:55-116. */
+cbs_status cbs_curl_apply(cbs_problem *p, double *field_inout, double scale_re,
+                          double scale_im);
+
+/* _solve_cbs_scattering: rhs = G_avg S, x0 = rhs, iterate to tolerance, then
+ * H = h_scale * curl(x).  Replaces src/solvers/convergent_born.jl:703-774
+ * (the LinearSolve.solve call at :750 and everything it drives).
+ * S, E_scat, H_scat: padded fields (Nx,Ny,Nz,3).  H_scat may be NULL. */
+cbs_status cbs_solve(cbs_problem *p, const double *S, double *E_scat, double *H_scat,
+                     const cbs_solve_opts *opts, cbs_solve_stats *stats);
+
+/* Device-resident form of cbs_solve, for callers that solve repeatedly or
+ * time the iteration alone: set the source (host -> device), run, fetch. */
+cbs_status cbs_problem_set_source(cbs_problem *p, const double *S);
+cbs_status cbs_problem_run(cbs_problem *p, const cbs_solve_opts *opts, cbs_solve_stats *stats);
+cbs_status cbs_problem_get_fields(cbs_problem *p, double *E_scat, double *H_scat,
+                                  double h_re, double h_im);
+
+/* Many right-hand sides of one problem (illumination angles x polarisations,
+ * one `solve` per source): every sweep of the iteration is one launch over all
+ * of them.  S[i], E[i] are padded fields; H may be NULL. */
+cbs_status cbs_solve_multi(cbs_problem *p, int32_t nrhs, const double *const *S, double **E,
+                           double **H, const cbs_solve_opts *opts, cbs_solve_stats *stats);
+
+/* Many problems of possibly different grids, each with one right-hand side:
+ * every sweep is one launch over all of them (uneven-size work table). */
+cbs_status cbs_solve_batch(cbs_ctx *ctx, int32_t n, cbs_problem *const *probs,
+                           const double *const *S, double **E, double **H,
+                           const cbs_solve_opts *opts, cbs_solve_stats *stats);
+
+/* ---- serving: continuous batching with HBM-footprint admission ---------- */
+typedef struct {
+  cbs_problem_desc desc;  /* grid and Green parameters                       */
+  const double *V;        /* potential, copied at submit                     */
+  const double *S;        /* source term, copied at submit                   */
+  cbs_solve_opts opts;    /* per-request tolerance / method / precision      */
+} cbs_request;
+
+/* Bytes of HBM a request of this description holds while resident. */
+cbs_status cbs_request_footprint(const cbs_problem_desc *desc, const cbs_solve_opts *opts,
+                                 uint64_t *bytes);
+cbs_status cbs_server_start(cbs_ctx *ctx, int32_t max_active);
+cbs_status cbs_server_stop(cbs_ctx *ctx);
+cbs_status cbs_submit(cbs_ctx *ctx, const cbs_request *req, uint64_t *ticket);
+/* done = 1 when the result is ready; E/H (padded fields) may be NULL. */
+cbs_status cbs_poll(cbs_ctx *ctx, uint64_t ticket, int32_t *done, cbs_solve_stats *stats);
+cbs_status cbs_wait(cbs_ctx *ctx, uint64_t ticket, double *E_scat, double *H_scat,
+                    double h_re, double h_im, cbs_solve_stats *stats);
+cbs_status cbs_cancel(cbs_ctx *ctx, uint64_t ticket);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CBS_MI355X_H */

@@ -1,0 +1,103 @@
+"""GPU parity of the operator layer: Green apply, CBS operator, preconditioner, curl.
+
+Each test builds one problem on the oracle and sends the same padded
+potential and Green parameters through the C ABI.  Bars: FP64 <= 1e-12
+relative (the FFT roundoff is ~1e-15); FP32 <= 2e-6 relative.
+"""
+
+import numpy as np
+import pytest
+
+from cases import RES, oracle_setup, random_field, rel, spec_from
+
+pytestmark = pytest.mark.gpu
+
+PERIODIC_GRIDS = [(16, 16, 8), (32, 16, 8), (6, 5, 4), (64, 64, 32), (24, 20, 12), (96, 32, 8)]
+ABSORB_GRIDS = [(16, 16, 8), (12, 10, 6)]
+FP64_TOL = 1e-12
+FP32_TOL = 2e-6
+
+
+def _absorbing(o):
+    return o.AbsorbingBC(thickness=400e-9, attenuation_thickness=300e-9)
+
+
+@pytest.mark.parametrize("grid", PERIODIC_GRIDS)
+def test_green_fp64_periodic(fm, oracle, grid):
+    print(f"case green fp64 periodic {grid}", flush=True)
+    _, _, init = oracle_setup(oracle, grid, oracle.PeriodicBC(), seed=1)
+    prob = fm.CBSProblem(init.potential, spec_from(fm, init), precision="fp64")
+    u = random_field(init.potential.shape, 2)
+    got = prob.green_apply(u)
+    want = oracle.green_conv(u, init.green)
+    assert rel(got, want) <= FP64_TOL
+
+
+@pytest.mark.parametrize("grid", ABSORB_GRIDS)
+def test_green_fp64_absorbing(fm, oracle, grid):
+    print(f"case green fp64 absorbing {grid}", flush=True)
+    _, _, init = oracle_setup(oracle, grid, _absorbing(oracle), seed=3)
+    assert init.green.use_averaging
+    prob = fm.CBSProblem(init.potential, spec_from(fm, init), precision="fp64")
+    u = random_field(init.potential.shape, 4)
+    got = prob.green_apply(u)
+    want = oracle.green_conv(u, init.green)
+    assert rel(got, want) <= FP64_TOL
+
+
+def test_green_mixed_axes(fm, oracle):
+    print("case green fp64 mixed periodic/absorbing", flush=True)
+    bc = (oracle.PeriodicBC(), oracle.PeriodicBC(), _absorbing(oracle))
+    _, _, init = oracle_setup(oracle, (16, 8, 8), bc, seed=5)
+    prob = fm.CBSProblem(init.potential, spec_from(fm, init), precision="fp64")
+    u = random_field(init.potential.shape, 6)
+    assert rel(prob.green_apply(u), oracle.green_conv(u, init.green)) <= FP64_TOL
+
+
+@pytest.mark.parametrize("grid,absorb", [((32, 32, 16), False), ((16, 16, 8), True)])
+def test_green_fp32(fm, oracle, grid, absorb):
+    print(f"case green fp32 {grid} absorb={absorb}", flush=True)
+    bc = _absorbing(oracle) if absorb else oracle.PeriodicBC()
+    _, _, init = oracle_setup(oracle, grid, bc, seed=7)
+    prob = fm.CBSProblem(init.potential, spec_from(fm, init), precision="fp32")
+    u = random_field(init.potential.shape, 8)
+    assert rel(prob.green_apply(u), oracle.green_conv(u, init.green)) <= FP32_TOL
+
+
+@pytest.mark.parametrize("absorb", [False, True])
+def test_operator_fp64(fm, oracle, absorb):
+    print(f"case operator fp64 absorb={absorb}", flush=True)
+    bc = _absorbing(oracle) if absorb else oracle.PeriodicBC()
+    _, _, init = oracle_setup(oracle, (16, 12, 8), bc, seed=9)
+    prob = fm.CBSProblem(init.potential, spec_from(fm, init), precision="fp64")
+    x = random_field(init.potential.shape, 10)
+    y = random_field(init.potential.shape, 11)
+    a, b = 0.7 - 0.2j, -1.3 + 0.4j
+    got = prob.operator_apply(x, y, a, b)
+    want = oracle.cbs_operator(x, init.potential, init.green, alpha=a, beta=b, y=y)
+    assert rel(got, want) <= FP64_TOL
+    got0 = prob.operator_apply(x)
+    assert rel(got0, oracle.cbs_operator(x, init.potential, init.green)) <= FP64_TOL
+
+
+def test_precond(fm, oracle):
+    print("case precond fp64/fp32", flush=True)
+    _, _, init = oracle_setup(oracle, (8, 8, 4), oracle.PeriodicBC(), seed=12)
+    x = random_field(init.potential.shape, 13)
+    want = oracle.cbs_precond(x, init.potential, init.eps_imag)
+    p64 = fm.CBSProblem(init.potential, spec_from(fm, init), precision="fp64")
+    assert rel(p64.precond_apply(x), want) <= 1e-15
+    p32 = fm.CBSProblem(init.potential, spec_from(fm, init), precision="fp32")
+    assert rel(p32.precond_apply(x), want) <= FP32_TOL
+
+
+@pytest.mark.parametrize("grid", [(16, 16, 8), (7, 5, 6), (32, 8, 16)])
+def test_curl_fp64(fm, oracle, grid):
+    print(f"case curl fp64 {grid}", flush=True)
+    _, _, init = oracle_setup(oracle, grid, oracle.PeriodicBC(), seed=14)
+    prob = fm.CBSProblem(init.potential, spec_from(fm, init), precision="fp64")
+    e = random_field(init.potential.shape, 15)
+    lam = 500e-9
+    got = prob.curl_apply(e, oracle.h_scale(lam))
+    want = oracle.magnetic_field(e, RES, lam)
+    assert rel(got, want) <= FP64_TOL
