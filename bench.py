@@ -1,0 +1,193 @@
+#!/usr/bin/env python3
+"""Benchmark of the CBS solve on one MI355X; prints one JSON line.
+
+Workload (N=1): the 128x128x64 grid, 50 nm voxels, water background n=1.33,
+one centred sphere n=1.5 of radius 16 px, x-polarised plane wave at 500 nm
+along +z, periodic boundaries, tolerance 1e-6 on the preconditioned residual
+||gamma (b - A x)|| / ||gamma b|| (FP64-verified).  A step is one full solve:
+FP32 Born sweeps inside an FP64 iterative-refinement loop, the iteration
+captured as a HIP graph, inputs resident in HBM.
+
+value        voxel-iterations/s of those steps (operator applications x padded voxels)
+fp64_plain   the same solve on the plain FP64 path (no graph, host check every sweep)
+roofline     dominant sweep kernel, algorithmic bytes / average launch time
+             (HIP events on the library stream)
+cpu_baseline the FP64 NumPy/SciPy restatement (oracle/) on the host, bounded sample
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+for _p in (str(ROOT), str(ROOT / "frequencymaxwell.jl_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+import numpy as np  # noqa: E402
+
+METRIC = "voxel-iters/s @1e-6 resid: batched FP32+refine vs plain FP64; p50/p99 latency"
+HBM_PEAK_GBS = 8000.0
+KERNELS = ["xfwd_Vx", "y_fwd", "z_green", "y_inv", "xinv_born_update"]
+
+
+def build_workload(fm, grid=(128, 128, 64), radius=16.0, tol=1e-6, precision="fp32_refine"):
+    from frequencymaxwell import solver as S
+
+    res = (50e-9, 50e-9, 50e-9)
+    med = fm.phantom_bead(grid, [1.5 ** 2], radius, permittivity_bg=1.33 ** 2)
+    sol = fm.ConvergentBornSolver(resolution=res, grid_size=grid,
+                                  boundary_conditions=fm.PeriodicBoundaryCondition(),
+                                  tolerance=tol, linear_solver="born", stop_rule="rhs",
+                                  precision=precision)
+    src = fm.PlaneWaveSource(wavelength=500e-9, polarization=[1, 0, 0], k_vector=[0, 0, 1])
+    lam, pot, eps_imag, bornmax, prob = S._prepare(sol, src, med)
+    inc = fm.generate_incident_field(src, sol, permittivity_bg=1.33 ** 2)
+    Sterm = np.asfortranarray(pot[..., None] * inc.E + complex(0, eps_imag) * inc.E)
+    return med, sol, pot, eps_imag, prob, Sterm
+
+
+def cpu_baseline(med, seconds: float):
+    """FP64 Born sweeps of the restatement for about `seconds` seconds."""
+    from oracle import cbs_oracle as o
+
+    s = o.Solver(resolution=(50e-9,) * 3, grid_size=med.permittivity.shape,
+                 boundary_conditions=o.PeriodicBC(), tolerance=1e-6)
+    init = o.initialize_solver(s, med.permittivity, 1.33 ** 2, 500e-9)
+    E_inc, _ = o.incident_field(s, o.PlaneWave(wavelength=500e-9, polarization=[1, 0, 0],
+                                               k_vector=[0, 0, 1]), 1.33 ** 2)
+    b = o.green_conv(o.source_term(init.potential, E_inc, init.eps_imag), init.green)
+    x = b.copy(order="F")
+    # one untimed sweep (plans, caches)
+    x = x + o.cbs_precond(b - o.cbs_operator(x, init.potential, init.green), init.potential,
+                          init.eps_imag)
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        x = x + o.cbs_precond(b - o.cbs_operator(x, init.potential, init.green),
+                              init.potential, init.eps_imag)
+        n += 1
+        dt = time.perf_counter() - t0
+        if dt >= seconds:
+            break
+    nvox = int(np.prod(med.permittivity.shape))
+    return {"value": n * nvox / dt, "unit": "voxel-iters/s", "cores": o.WORKERS, "kind": "port",
+            "sample": f"{n} FP64 Born sweeps of the 128x128x64 grid in {dt:.1f} s "
+                      f"(NumPy + scipy.fft restatement, {o.WORKERS} FFT threads)"}
+
+
+def read_traffic(name: str):
+    p = ROOT / "profiles" / name
+    if p.exists():
+        try:
+            return json.loads(p.read_text()).get("traffic_bytes_per_launch")
+        except (ValueError, OSError):
+            return None
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--check-every", type=int, default=32)
+    ap.add_argument("--maxiters", type=int, default=20000)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: this build measures the single-GPU path only "
+              f"(--gpus {args.gpus} requested)", file=sys.stderr)
+        sys.exit(2)
+
+    import frequencymaxwell as fm
+
+    med, sol, pot, eps_imag, prob, Sterm = build_workload(fm)
+    nvox = int(np.prod(pot.shape))
+    prob.set_source(Sterm)
+    fast = fm.make_opts(method="born", precision="fp32_refine", tol=1e-6, stop_rule="rhs",
+                        maxiters=args.maxiters, use_graph=True, check_every=args.check_every)
+    print(f"[bench] workload ready: grid {pot.shape}, nvox {nvox}", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        prob.run(fast)
+    lat, vi, iters = [], 0, []
+    last = None
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        t = time.perf_counter()
+        st = prob.run(fast)
+        lat.append((time.perf_counter() - t) * 1e3)
+        vi += st.voxel_iters
+        iters.append(st.iters)
+        last = st
+        print(f"[bench] step {k}: {lat[-1]:.1f} ms, iters {st.iters} (fp64 {st.iters_fp64}), "
+              f"resid {st.resid:.2e}", file=sys.stderr, flush=True)
+    total = time.perf_counter() - t0
+    value = vi / total
+    E32, _ = prob.get_fields()
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "voxel-iters/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": total / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "complex fp32 sweeps + fp64 residual/refinement", "data": "synthetic",
+        "config": {"workload": "128x128x64 periodic grid, bead n=1.5 r=16px in water n=1.33, "
+                               "x-pol 500 nm along +z, 50 nm voxels, tol 1e-6 (FP64-verified)",
+                   "method": "Born (CBS) FP32 inner + FP64 refinement, hipGraph chunks",
+                   "nvox": nvox},
+        "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+        "iters_per_solve": iters[-1], "fp64_iters_per_solve": last.iters_fp64,
+        "resid_fp64": last.resid_true_fp64, "converged": bool(last.converged),
+    }
+
+    if not args.no_fp64:
+        p64 = fm.CBSProblem(pot, prob.spec, precision="fp64")
+        p64.set_source(Sterm)
+        plain = fm.make_opts(method="born", precision="fp64", tol=1e-6, stop_rule="rhs",
+                             maxiters=args.maxiters, use_graph=False, check_every=1)
+        p64.run(fm.make_opts(method="born", precision="fp64", tol=1e-6, maxiters=3))
+        t = time.perf_counter()
+        st64 = p64.run(plain)
+        dt = time.perf_counter() - t
+        E64, _ = p64.get_fields()
+        v64 = st64.voxel_iters / dt
+        out["fp64_plain"] = {"value": v64, "ms_per_solve": dt * 1e3, "iters": st64.iters,
+                             "converged": bool(st64.converged), "resid": st64.resid,
+                             "time_to_tol_speedup": dt / (total / args.steps)}
+        out["speedup_vs_fp64_plain"] = value / v64
+        out["field_err_vs_fp64"] = float(np.linalg.norm(E32 - E64) / np.linalg.norm(E64))
+        out["companion_vi_per_s"] = nvox * st64.iters / (total / args.steps)
+        p64.close()
+
+    ms, by = prob.time_sweeps("fp32", reps=20)
+    k = int(np.argmax(ms))
+    achieved = by[k] / (ms[k] * 1e-3) / 1e9
+    traffic = read_traffic(f"traffic_{KERNELS[k]}_128x128x64_fp32.json")
+    out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                       "kernel": KERNELS[k], "algorithmic_bytes_per_launch": by[k],
+                       "avg_launch_ms": ms[k]}
+    out["sweeps"] = {KERNELS[i]: {"ms": ms[i], "GB/s": by[i] / (ms[i] * 1e-3) / 1e9}
+                     for i in range(5)}
+    sweep_ms = sum(ms)
+    out["sweep_iteration_ms"] = sweep_ms
+    out["sweep_roofline_frac"] = sum(by) / (sweep_ms * 1e-3) / 1e9 / HBM_PEAK_GBS
+
+    if not args.no_cpu:
+        print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(med, args.cpu_seconds)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -7,6 +7,7 @@
 
 #include "cbs_engine.h"
 #include "cbs_serve.h"
+#include "cbs_solve.h"
 
 using namespace cbs;
 
@@ -413,6 +414,22 @@ cbs_status cbs_solve_batch(cbs_ctx* ctx, int32_t n, cbs_problem* const* pps,
       solve_many(ctx->impl, pk, sk.data(), ek.data(), hk.data(), *opts, stk.data());
       if (stats)
         for (size_t j = 0; j < g[k].size(); ++j) stats[g[k][j]] = stk[j];
+    }
+  });
+}
+
+cbs_status cbs_time_sweeps(cbs_problem* pp, int32_t precision, int32_t reps, double* ms,
+                           double* bytes) {
+  return guarded([&] {
+    CBS_REQUIRE(pp && ms && bytes, "NULL argument");
+    Problem& p = pp->impl;
+    set_device(*p.ctx);
+    if (precision == CBS_FP64) {
+      time_sweeps<double>(p, reps, ms, bytes);
+    } else {
+      fill_tables<float>(p);
+      ensure_v32(p, p.ctx->stream);
+      time_sweeps<float>(p, reps, ms, bytes);
     }
   });
 }

@@ -18,6 +18,9 @@ struct PipeLaunch {
 
 template <class T>
 void launch_pipeline(const PipeLaunch<T>& L, hipStream_t s);
+// one of the five sweeps: 0 xfwd, 1 y forward, 2 z (k-space), 3 y inverse, 4 xinv
+template <class T>
+void launch_pass(const PipeLaunch<T>& L, int k, hipStream_t s);
 
 void launch_born_finalize(RState* const* sts, double* const* parts, const int* ntiles, int nrhs,
                           int parity, hipStream_t s);

@@ -612,6 +612,41 @@ __global__ void k_gamma_mul(Cx<T>* __restrict__ dst, const Cx<T>* __restrict__ s
 static inline unsigned nblk(long long n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
 
 template <class T>
+void launch_pass(const PipeLaunch<T>& L, int k, hipStream_t s) {
+  const int nb = L.nprob + 1;
+  switch (k) {
+    case 0:
+      if (L.S == 2)
+        hipLaunchKernelGGL((k_xfwd<T, 2>), dim3(L.blocks[0]), dim3(kThreads), L.lds[0], s,
+                           L.probs, L.bstart, L.nprob);
+      else
+        hipLaunchKernelGGL((k_xfwd<T, 1>), dim3(L.blocks[0]), dim3(kThreads), L.lds[0], s,
+                           L.probs, L.bstart, L.nprob);
+      break;
+    case 1:
+      hipLaunchKernelGGL((k_ypass<T, -1>), dim3(L.blocks[1]), dim3(kThreads), L.lds[1], s,
+                         L.probs, L.bstart + nb, L.nprob);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_zpass<T>), dim3(L.blocks[2]), dim3(kThreads), L.lds[2], s, L.probs,
+                         L.bstart + 2 * nb, L.nprob);
+      break;
+    case 3:
+      hipLaunchKernelGGL((k_ypass<T, 1>), dim3(L.blocks[1]), dim3(kThreads), L.lds[1], s,
+                         L.probs, L.bstart + nb, L.nprob);
+      break;
+    default:
+      if (L.S == 2)
+        hipLaunchKernelGGL((k_xinv<T, 2>), dim3(L.blocks[3]), dim3(kThreads), L.lds[3], s,
+                           L.probs, L.bstart + 3 * nb, L.nprob);
+      else
+        hipLaunchKernelGGL((k_xinv<T, 1>), dim3(L.blocks[3]), dim3(kThreads), L.lds[3], s,
+                           L.probs, L.bstart + 3 * nb, L.nprob);
+      break;
+  }
+}
+
+template <class T>
 void launch_pipeline(const PipeLaunch<T>& L, hipStream_t s) {
   if (L.S == 2) {
     hipLaunchKernelGGL((k_xfwd<T, 2>), dim3(L.blocks[0]), dim3(kThreads), L.lds[0], s, L.probs,
@@ -695,6 +730,7 @@ void launch_gamma_mul(Cx<T>* dst, const Cx<T>* src, const Cx<T>* V, long long nv
 
 #define CBS_INST(T)                                                                         \
   template void launch_pipeline<T>(const PipeLaunch<T>&, hipStream_t);                      \
+  template void launch_pass<T>(const PipeLaunch<T>&, int, hipStream_t);                     \
   template int launch_gnorm_parts<T>(const Cx<T>*, const Cx<T>*, long long, double, double*, \
                                      hipStream_t);                                          \
   template void launch_precond<T>(Cx<T>*, const Cx<T>*, const Cx<T>*, long long, double,    \

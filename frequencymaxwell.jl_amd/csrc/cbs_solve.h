@@ -45,4 +45,8 @@ template <class T>
 void gmres_solve(Ctx* ctx, std::vector<SolveItem<T>>& it, const IterOpts& io, hipStream_t s,
                  std::vector<RState>& states, std::vector<const Cx<T>*>& result);
 
+// per-kernel timing of the Born sweeps on the resident source term
+template <class T>
+void time_sweeps(Problem& p, int reps, double* ms, double* bytes);
+
 }  // namespace cbs

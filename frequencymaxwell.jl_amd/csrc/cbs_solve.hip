@@ -10,6 +10,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 
 #include "cbs_engine.h"
 #include "cbs_solve.h"
@@ -464,6 +465,63 @@ void solve_set(Ctx* ctx, std::vector<RhsIO>& io, const cbs_solve_opts& o,
     stats[i].voxel_iters = (int64_t)stats[i].iters * io[i].p->nvox;
   }
 }
+
+template <class T>
+void time_sweeps(Problem& p, int reps, double* ms, double* bytes) {
+  CBS_REQUIRE(p.S64.p, "no source term set");
+  CBS_REQUIRE(reps >= 1, "reps must be >= 1");
+  hipStream_t s = p.ctx->stream;
+  std::vector<RhsIO> io{RhsIO{&p, p.S64.as<Cx<double>>(), nullptr}};
+  std::vector<SolveItem<T>> it;
+  prepare_items<T>(io, it, false);
+  Work<T>& w = *it[0].w;
+  Cx<T>* X0 = w.X[0].template as<Cx<T>>();
+  Cx<T>* X1 = w.X[1].template as<Cx<T>>();
+  Cx<T>* B = w.B.template as<Cx<T>>();
+  std::vector<const Cx<T>*> src;
+  std::vector<Cx<T>*> dst{B};
+  if (std::is_same<T, double>::value) {
+    src.push_back(reinterpret_cast<const Cx<T>*>(p.S64.p));
+  } else {
+    launch_convert<float, double>(reinterpret_cast<Cx<float>*>(X1), p.S64.as<Cx<double>>(),
+                                  3 * p.nvox, s);
+    src.push_back(X1);
+  }
+  green_set<T>(p.ctx, it, src, dst, s);
+  copy_dev<T>(X0, B, 3 * p.nvox, s);
+  Pipe<T> pipe;
+  pipe.h.push_back(make_desc<T>(p, PRE_V, ZOP_GREEN, POST_BORN, X0, B, X1,
+                                w.W.template as<Cx<T>>(), w.part.template as<double>(),
+                                nullptr));
+  pipe.build(p.ctx, s);
+  hipEvent_t ev[6];
+  for (auto& e : ev) CBS_HIP(hipEventCreate(&e));
+  double acc[5] = {0, 0, 0, 0, 0};
+  for (int r = 0; r < reps + 1; ++r) {  // the first pass warms up
+    CBS_HIP(hipEventRecord(ev[0], s));
+    for (int k = 0; k < 5; ++k) {
+      launch_pass<T>(pipe.L, k, s);
+      CBS_HIP(hipEventRecord(ev[k + 1], s));
+    }
+    CBS_HIP(hipEventSynchronize(ev[5]));
+    CBS_HIP(hipGetLastError());
+    if (r == 0) continue;
+    for (int k = 0; k < 5; ++k) {
+      float t = 0.f;
+      CBS_HIP(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
+      acc[k] += t;
+    }
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  const double sz = (double)sizeof(Cx<T>), S = (double)p.S, nv = (double)p.nvox;
+  const double per[5] = {4 + 3 * S, 6 * S, 6 * S, 6 * S, 3 * S + 10};
+  for (int k = 0; k < 5; ++k) {
+    ms[k] = acc[k] / reps;
+    bytes[k] = per[k] * sz * nv;
+  }
+}
+template void time_sweeps<double>(Problem&, int, double*, double*);
+template void time_sweeps<float>(Problem&, int, double*, double*);
 
 template void born_solve<double>(Ctx*, std::vector<SolveItem<double>>&, const IterOpts&,
                                  hipStream_t, std::vector<RState>&,

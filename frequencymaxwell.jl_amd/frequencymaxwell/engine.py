@@ -175,6 +175,14 @@ class CBSProblem:
                                                  hs.real, hs.imag))
         return E, H
 
+    def time_sweeps(self, precision="fp32", reps: int = 20):
+        """Average duration (ms) and algorithmic bytes of each of the five sweep
+        kernels, timed with HIP events on the library stream."""
+        ms = (C.c_double * 5)()
+        by = (C.c_double * 5)()
+        L.check(self._lib.cbs_time_sweeps(self.handle, precision_code(precision), int(reps), ms, by))
+        return list(ms), list(by)
+
     def solve_multi(self, S_list, opts: L.SolveOpts, want_h: bool = True):
         """Many right-hand sides of this problem in one set of launches per sweep."""
         n = len(S_list)

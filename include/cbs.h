@@ -174,6 +174,16 @@ cbs_status cbs_solve_batch(cbs_ctx *ctx, int32_t n, cbs_problem *const *probs,
                            const double *const *S, double **E, double **H,
                            const cbs_solve_opts *opts, cbs_solve_stats *stats);
 
+/* Measurement aid -- no counterpart in the reference; for benchmarks that
+ * need per-kernel durations on the library's own stream.  Runs `reps` Born
+ * sweeps of the resident source term (cbs_problem_set_source first) in the
+ * given precision and times each of the five sweep kernels with HIP events.
+ * ms[k]: average duration of one launch of kernel k (0 x forward + V x,
+ * 1 y forward, 2 z + Green, 3 y inverse, 4 x inverse + Born update);
+ * bytes[k]: algorithmic HBM bytes of one launch of kernel k. */
+cbs_status cbs_time_sweeps(cbs_problem *p, int32_t precision, int32_t reps, double *ms,
+                           double *bytes);
+
 /* ---- serving: continuous batching with HBM-footprint admission ---------- */
 typedef struct {
   cbs_problem_desc desc;  /* grid and Green parameters                       */

@@ -43,7 +43,8 @@ from typing import Sequence
 import numpy as np
 import scipy.fft as sfft
 
-WORKERS = os.cpu_count() or 1
+# FFT threads: the host's cores, at most 16 (a one-GPU share of a GPU node)
+WORKERS = int(os.environ.get("CBS_ORACLE_WORKERS", min(16, os.cpu_count() or 1)))
 Z0_REF = 377.0  # convergent_born.jl:797, plane_wave.jl:232 (quirk: 377, not 376.73)
 
 
