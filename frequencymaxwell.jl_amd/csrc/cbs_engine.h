@@ -82,9 +82,17 @@ struct Tables {
 template <class T>
 struct Work {
   DevBuf X[2], B, W, R, part, st, D;
+  DevBuf K;      // Krylov vectors, nk fields of 3 nvox, contiguous
+  DevBuf small;  // Krylov small dense data (Hessenberg, rotations, coefficients)
   size_t nv = 0;
   int S = 1;
+  int nk = 0;
   void ensure(Ctx* c, long long nvox, int s, int ntiles, bool need_r);
+  void ensure_k(Ctx* c, int n) {
+    K.ensure(c, (size_t)n * 3 * nv * sizeof(Cx<T>));
+    nk = n;
+  }
+  Cx<T>* kvec(int i) const { return K.template as<Cx<T>>() + (size_t)i * 3 * nv; }
 };
 
 struct Problem {

@@ -14,6 +14,7 @@
 // grids): block b finds its entry by binary search over per-pass block offsets.
 #include "cbs_fft.h"
 #include "cbs_kernels.h"
+#include "cbs_reduce.h"
 
 namespace cbs {
 
@@ -32,35 +33,6 @@ __device__ __forceinline__ int find_prob(const int* __restrict__ bstart, int npr
 template <class T>
 __device__ __forceinline__ Cx<T> ramp_at(const DevProb<T>& pr, int x, int y, int z) {
   return pr.ramp[0][x] * pr.ramp[1][y] * pr.ramp[2][z];
-}
-
-// ---------------------------------------------------------------------------
-// block reduction of kNumPart doubles; lane 0 of the block writes them
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ void block_reduce_store(double (&acc)[kNumPart], double* dst,
-                                                   double* scratch, int tid) {
-#pragma unroll
-  for (int k = 0; k < kNumPart; ++k) {
-    double v = acc[k];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-    acc[k] = v;
-  }
-  const int lane = tid & 63, wave = tid >> 6;
-  __syncthreads();  // the caller's LDS tile may still be read by other waves
-  if (lane == 0) {
-#pragma unroll
-    for (int k = 0; k < kNumPart; ++k) scratch[wave * kNumPart + k] = acc[k];
-  }
-  __syncthreads();
-  if (tid == 0) {
-#pragma unroll
-    for (int k = 0; k < kNumPart; ++k) {
-      double s = 0.0;
-      for (int w = 0; w < kThreads / 64; ++w) s += scratch[w * kNumPart + k];
-      dst[k] = s;
-    }
-  }
 }
 
 // ---------------------------------------------------------------------------
@@ -130,7 +102,7 @@ __global__ void __launch_bounds__(kThreads)
 k_xfwd(const DevProb<T>* __restrict__ probs, const int* __restrict__ bstart, int nprob) {
   const int pi = find_prob(bstart, nprob, blockIdx.x);
   const DevProb<T>& pr = probs[pi];
-  if (pr.st && pr.st->done) return;
+  if (pr.st && (pr.st->done | pr.st->cyc)) return;
   const int tile = blockIdx.x - bstart[pi];
   const int tid = threadIdx.x;
   switch (pr.rt[0] ? 0 : pr.n[0]) {
@@ -190,7 +162,7 @@ __global__ void __launch_bounds__(kThreads)
 k_ypass(const DevProb<T>* __restrict__ probs, const int* __restrict__ bstart, int nprob) {
   const int pi = find_prob(bstart, nprob, blockIdx.x);
   const DevProb<T>& pr = probs[pi];
-  if (pr.st && pr.st->done) return;
+  if (pr.st && (pr.st->done | pr.st->cyc)) return;
   const int tile = blockIdx.x - bstart[pi];
   const int tid = threadIdx.x;
   switch (pr.rt[1] ? 0 : pr.n[1]) {
@@ -299,7 +271,7 @@ __global__ void __launch_bounds__(kThreads)
 k_zpass(const DevProb<T>* __restrict__ probs, const int* __restrict__ bstart, int nprob) {
   const int pi = find_prob(bstart, nprob, blockIdx.x);
   const DevProb<T>& pr = probs[pi];
-  if (pr.st && pr.st->done) return;
+  if (pr.st && (pr.st->done | pr.st->cyc)) return;
   const int tile = blockIdx.x - bstart[pi];
   const int tid = threadIdx.x;
   switch (pr.rt[2] ? 0 : pr.n[2]) {
@@ -435,7 +407,7 @@ __global__ void __launch_bounds__(kThreads)
 k_xinv(const DevProb<T>* __restrict__ probs, const int* __restrict__ bstart, int nprob) {
   const int pi = find_prob(bstart, nprob, blockIdx.x);
   const DevProb<T>& pr = probs[pi];
-  if (pr.st && pr.st->done) return;
+  if (pr.st && (pr.st->done | pr.st->cyc)) return;
   const int tile = blockIdx.x - bstart[pi];
   const int tid = threadIdx.x;
   switch (pr.rt[0] ? 0 : pr.n[0]) {
@@ -456,22 +428,6 @@ k_xinv(const DevProb<T>* __restrict__ probs, const int* __restrict__ bstart, int
 // ---------------------------------------------------------------------------
 // reductions and solver state
 // ---------------------------------------------------------------------------
-// sum kNumPart-strided partials [0, ntile) of column k, fixed order
-__device__ double block_sum_parts(const double* __restrict__ part, int ntile, int k,
-                                  double* red) {
-  const int tid = threadIdx.x;
-  double v = 0.0;
-  for (int i = tid; i < ntile; i += kThreads) v += part[(long long)i * kNumPart + k];
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-  __syncthreads();
-  if ((tid & 63) == 0) red[tid >> 6] = v;
-  __syncthreads();
-  double s = 0.0;
-  for (int w = 0; w < kThreads / 64; ++w) s += red[w];
-  return s;
-}
-
 // Born: one block per RHS.  Sums |r|^2 of this iteration and advances the state.
 __global__ void __launch_bounds__(kThreads)
 k_born_finalize(RState* const* __restrict__ sts, double* const* __restrict__ parts,
@@ -561,6 +517,11 @@ __global__ void k_state_init(RState* st, const double* __restrict__ sums, double
   st->thr2 = rule == 0 ? tol * tol * bn2 : -1.0;
   st->rho_re = st->alpha_re = st->omega_re = 1.0;
   st->rho_im = st->alpha_im = st->omega_im = 0.0;
+  st->beta_re = st->beta_im = 0.0;
+  st->s2 = 0.0;
+  st->nhist = 0;
+  st->cyc = 0;
+  st->pend = 0;
 }
 
 // ---------------------------------------------------------------------------

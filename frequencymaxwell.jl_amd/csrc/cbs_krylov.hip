@@ -1,18 +1,802 @@
-// Krylov solvers (BiCGStab, GMRES) on the preconditioned CBS operator.
+// Krylov solvers on the preconditioned CBS system gamma (I - G_avg V) x = gamma b
+// (LinearSolve's KrylovJL_BICGSTAB / KrylovJL_GMRES routes of
+// src/solvers/convergent_born.jl:737-750 and examples/two_beam_interference.jl:58).
+//
+// Every scalar (rho, alpha, omega, Hessenberg entries, rotations) lives in
+// device memory and is produced by one-block finalize kernels, so an
+// iteration never waits for the host and can be captured in a graph.  Each
+// right-hand side has its own scalars; reductions never mix right-hand sides.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "cbs_reduce.h"
 #include "cbs_solve.h"
 
 namespace cbs {
 
+constexpr long long kChunk = (long long)kThreads * 16;  // voxels per vector-kernel block
+
+// ---------------------------------------------------------------------------
+// BiCGStab (van der Vorst 1992), shadow residual r^ = r0
+// ---------------------------------------------------------------------------
 template <class T>
-void bicgstab_solve(Ctx*, std::vector<SolveItem<T>>&, const IterOpts&, hipStream_t,
-                    std::vector<RState>&, std::vector<const Cx<T>*>&) {
-  throw Error(CBS_EINVAL, "BiCGStab is not available in this build");
+struct BiVec {
+  Cx<T>* x;
+  Cx<T>* r;
+  Cx<T>* rh;
+  Cx<T>* p;
+  Cx<T>* v;
+  Cx<T>* s;
+  Cx<T>* t;
+  const Cx<T>* V;
+  const Cx<T>* b;
+  long long nv;
+  double eps_inv;
+  double* part;
+  RState* st;
+  int use_v;  // BI_INIT: r = gamma b - v (x0 given) instead of r = gamma b
+};
+
+enum { BI_INIT = 0, BI_P = 1, BI_S = 2, BI_XR = 3 };
+
+template <class T, int OP>
+__global__ void __launch_bounds__(kThreads) k_bicg_vec(const BiVec<T>* __restrict__ vs) {
+  __shared__ double scratch[(kThreads / 64) * kNumPart];
+  const BiVec<T>& q = vs[blockIdx.y];
+  if (q.st->done) return;
+  const long long start = (long long)blockIdx.x * kChunk;
+  if (start >= q.nv) return;
+  const long long end = min(start + kChunk, q.nv);
+  const long long nv = q.nv;
+  const RState* st = q.st;
+  double acc[kNumPart] = {0, 0, 0, 0, 0};
+  const Cx<T> alpha = mk<T>((T)st->alpha_re, (T)st->alpha_im);
+  const Cx<T> omega = mk<T>((T)st->omega_re, (T)st->omega_im);
+  const Cx<T> beta = mk<T>((T)st->beta_re, (T)st->beta_im);
+  for (long long i = start + threadIdx.x; i < end; i += kThreads) {
+    Cx<T> gam = mk<T>(T(0), T(0));
+    if (OP == BI_INIT) gam = mul_i((T)q.eps_inv * q.V[i]);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const long long j = c * nv + i;
+      if (OP == BI_INIT) {
+        Cx<T> r = gam * q.b[j];
+        if (q.use_v) r = r - q.v[j];
+        q.r[j] = r;
+        q.rh[j] = r;
+        q.p[j] = mk<T>(T(0), T(0));
+        q.v[j] = mk<T>(T(0), T(0));
+        acc[0] += (double)abs2(r);
+      } else if (OP == BI_P) {
+        q.p[j] = q.r[j] + beta * (q.p[j] - omega * q.v[j]);
+      } else if (OP == BI_S) {
+        const Cx<T> s = q.r[j] - alpha * q.v[j];
+        q.s[j] = s;
+        q.x[j] = q.x[j] + alpha * q.p[j];
+        acc[0] += (double)abs2(s);
+      } else {  // BI_XR
+        const Cx<T> s = q.s[j];
+        const Cx<T> r = s - omega * q.t[j];
+        q.x[j] = q.x[j] + omega * s;
+        q.r[j] = r;
+        acc[0] += (double)abs2(r);
+        const Cx<T> d = cdot(q.rh[j], r);
+        acc[1] += (double)d.re;
+        acc[2] += (double)d.im;
+      }
+    }
+  }
+  if (OP != BI_P)
+    block_reduce_store(acc, q.part + (long long)blockIdx.x * kNumPart, scratch, threadIdx.x);
+}
+
+enum { BF_INIT = 0, BF_ALPHA = 1, BF_S = 2, BF_OMEGA = 3, BF_R = 4 };
+
+__device__ __forceinline__ void push_hist(RState* st, double r2) {
+  if (st->nhist < 512) st->hist[st->nhist] = st->bnorm2 > 0 ? sqrt(r2 / st->bnorm2) : 0.0;
+  st->nhist++;
+}
+
+// one block per RHS; ntiles[i] = partial blocks of the producing kernel
+__global__ void __launch_bounds__(kThreads)
+k_bicg_fin(RState* const* __restrict__ sts, double* const* __restrict__ parts,
+           const int* __restrict__ ntiles, int mode, int init_iters) {
+  __shared__ double red[kThreads / 64];
+  RState* st = sts[blockIdx.y * gridDim.x + blockIdx.x];
+  const int rid = blockIdx.y * gridDim.x + blockIdx.x;
+  if (st->done) return;
+  const double* part = parts[rid];
+  const int nt = ntiles[rid];
+  if (mode == BF_INIT) {
+    const double r2 = block_sum_parts(part, nt, 0, red);
+    if (threadIdx.x) return;
+    st->r2 = r2;
+    st->iters += init_iters;
+    st->nhist = 0;
+    push_hist(st, r2);
+    if (st->thr2 < 0.0) {
+      const double t = st->abstol + st->tol * sqrt(r2);
+      st->thr2 = t * t;
+    }
+    // rho_new = <r^, r> = ||r||^2; beta is irrelevant while p = v = 0
+    st->beta_re = r2;
+    st->beta_im = 0.0;
+    st->rho_re = r2;
+    st->rho_im = 0.0;
+    st->alpha_re = st->omega_re = 1.0;
+    st->alpha_im = st->omega_im = 0.0;
+    if (!(r2 == r2)) st->done = 3;
+    else if (r2 <= st->thr2) st->done = 1;
+    else if (st->iters + 2 > st->maxit) st->done = 2;
+  } else if (mode == BF_ALPHA) {
+    const double sr = block_sum_parts(part, nt, 0, red);
+    const double si = block_sum_parts(part, nt, 1, red);
+    if (threadIdx.x) return;
+    st->iters += 1;
+    const Cx<double> sig = mk<double>(sr, si);
+    if (sr == 0.0 && si == 0.0) {
+      st->done = 4;  // breakdown: <r^, v> = 0
+      return;
+    }
+    const Cx<double> a = cdiv(mk<double>(st->rho_re, st->rho_im), sig);
+    st->alpha_re = a.re;
+    st->alpha_im = a.im;
+  } else if (mode == BF_S) {
+    const double s2 = block_sum_parts(part, nt, 0, red);
+    if (threadIdx.x) return;
+    st->s2 = s2;
+    if (!(s2 == s2)) {
+      st->done = 3;
+    } else if (s2 <= st->thr2) {  // x already holds x + alpha p
+      st->r2 = s2;
+      push_hist(st, s2);
+      st->done = 1;
+    }
+  } else if (mode == BF_OMEGA) {
+    const double ts_r = block_sum_parts(part, nt, 2, red);
+    const double ts_i = block_sum_parts(part, nt, 3, red);
+    const double tt = block_sum_parts(part, nt, 4, red);
+    if (threadIdx.x) return;
+    st->iters += 1;
+    if (tt == 0.0) {  // t = 0: s is the residual of x + alpha p
+      st->r2 = st->s2;
+      st->done = 4;
+      return;
+    }
+    st->omega_re = ts_r / tt;
+    st->omega_im = ts_i / tt;
+  } else {  // BF_R
+    const double r2 = block_sum_parts(part, nt, 0, red);
+    const double pr = block_sum_parts(part, nt, 1, red);
+    const double pi = block_sum_parts(part, nt, 2, red);
+    if (threadIdx.x) return;
+    st->r2 = r2;
+    push_hist(st, r2);
+    const Cx<double> rn = mk<double>(pr, pi);
+    if (!(r2 == r2)) {
+      st->done = 3;
+    } else if (r2 <= st->thr2) {
+      st->done = 1;
+    } else if (st->iters + 2 > st->maxit) {
+      st->done = 2;
+    } else if (pr == 0.0 && pi == 0.0) {
+      st->done = 4;  // breakdown: <r^, r> = 0
+    } else {
+      // beta = (rho_new / rho) (alpha / omega)
+      const Cx<double> b = cdiv(rn, mk<double>(st->rho_re, st->rho_im)) *
+                           cdiv(mk<double>(st->alpha_re, st->alpha_im),
+                                mk<double>(st->omega_re, st->omega_im));
+      st->beta_re = b.re;
+      st->beta_im = b.im;
+      st->rho_re = rn.re;
+      st->rho_im = rn.im;
+    }
+  }
 }
 
 template <class T>
-void gmres_solve(Ctx*, std::vector<SolveItem<T>>&, const IterOpts&, hipStream_t,
-                 std::vector<RState>&, std::vector<const Cx<T>*>&) {
-  throw Error(CBS_EINVAL, "GMRES is not available in this build");
+void bicgstab_solve(Ctx* ctx, std::vector<SolveItem<T>>& it, const IterOpts& io, hipStream_t s,
+                    std::vector<RState>& states, std::vector<const Cx<T>*>& result) {
+  const int n = (int)it.size();
+  StateSet gn, fin_mv, fin_vec;
+  long long maxchunk = 1;
+  std::vector<BiVec<T>> hv(n);
+  for (int i = 0; i < n; ++i) {
+    SolveItem<T>& x = it[i];
+    x.w->ensure_k(ctx, 6);
+    BiVec<T>& q = hv[i];
+    q.x = x.w->X[0].template as<Cx<T>>();
+    q.r = x.w->kvec(0);
+    q.rh = x.w->kvec(1);
+    q.p = x.w->kvec(2);
+    q.v = x.w->kvec(3);
+    q.s = x.w->kvec(4);
+    q.t = x.w->kvec(5);
+    q.V = potential<T>(*x.p);
+    q.b = x.b;
+    q.nv = x.p->nvox;
+    q.eps_inv = x.p->eps_inv;
+    q.part = x.w->part.template as<double>();
+    q.st = x.w->st.template as<RState>();
+    q.use_v = io.x0_zero ? 0 : 1;
+    maxchunk = std::max(maxchunk, (q.nv + kChunk - 1) / kChunk);
+    if (io.x0_zero) launch_fill<T>(q.x, 3 * q.nv, s);
+    else copy_dev<T>(q.x, x.b, 3 * q.nv, s);
+  }
+  init_states<T>(ctx, it, io.tol, io.abstol, io.rule, io.maxit, s, gn);
+  DevBuf dv;
+  dv.alloc(ctx, n * sizeof(BiVec<T>));
+  CBS_HIP(hipMemcpyAsync(dv.p, hv.data(), n * sizeof(BiVec<T>), hipMemcpyHostToDevice, s));
+  const BiVec<T>* d_vs = dv.as<BiVec<T>>();
+
+  // matvec pipelines: v = A^ x0 (init), v = A^ p (with <r^, v>), t = A^ s (with <t,s>, |t|^2)
+  Pipe<T> m0, m1, m2;
+  std::vector<RState*> sts;
+  std::vector<double*> parts;
+  std::vector<int> nt_mv, nt_vec;
+  for (int i = 0; i < n; ++i) {
+    Problem& p = *it[i].p;
+    Cx<T>* W = it[i].w->W.template as<Cx<T>>();
+    double* part = hv[i].part;
+    RState* st = hv[i].st;
+    m0.h.push_back(make_desc<T>(p, PRE_V, ZOP_GREEN, POST_PMATVEC, hv[i].x, nullptr, hv[i].v, W,
+                                part, st));
+    m1.h.push_back(make_desc<T>(p, PRE_V, ZOP_GREEN, POST_PMATVEC, hv[i].p, hv[i].rh, hv[i].v,
+                                W, part, st));
+    m2.h.push_back(make_desc<T>(p, PRE_V, ZOP_GREEN, POST_PMATVEC, hv[i].s, nullptr, hv[i].t, W,
+                                part, st));
+    sts.push_back(st);
+    parts.push_back(part);
+    nt_mv.push_back(m1.h.back().tiles[3]);
+    nt_vec.push_back((int)((hv[i].nv + kChunk - 1) / kChunk));
+  }
+  m0.build(ctx, s);
+  m1.build(ctx, s);
+  m2.build(ctx, s);
+  fin_mv.build(ctx, sts, parts, nt_mv, s);
+  fin_vec.build(ctx, sts, parts, nt_vec, s);
+  const dim3 vgrid((unsigned)maxchunk, (unsigned)n);
+  auto fin = [&](StateSet& f, int mode, int init_iters) {
+    hipLaunchKernelGGL(k_bicg_fin, dim3(n), dim3(kThreads), 0, s, f.sts.as<RState*>(),
+                       f.parts.as<double*>(), f.ntiles.as<int>(), mode, init_iters);
+  };
+  // r0
+  if (!io.x0_zero) m0.launch(s);
+  hipLaunchKernelGGL((k_bicg_vec<T, BI_INIT>), vgrid, dim3(kThreads), 0, s, d_vs);
+  fin(fin_vec, BF_INIT, io.x0_zero ? 0 : 1);
+  CBS_HIP(hipGetLastError());
+
+  auto iteration = [&]() {
+    hipLaunchKernelGGL((k_bicg_vec<T, BI_P>), vgrid, dim3(kThreads), 0, s, d_vs);
+    m1.launch(s);
+    fin(fin_mv, BF_ALPHA, 0);
+    hipLaunchKernelGGL((k_bicg_vec<T, BI_S>), vgrid, dim3(kThreads), 0, s, d_vs);
+    fin(fin_vec, BF_S, 0);
+    m2.launch(s);
+    fin(fin_mv, BF_OMEGA, 0);
+    hipLaunchKernelGGL((k_bicg_vec<T, BI_XR>), vgrid, dim3(kThreads), 0, s, d_vs);
+    fin(fin_vec, BF_R, 0);
+  };
+  const int chunk = std::max(1, io.check_every);
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+  if (io.use_graph) {
+    CBS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < chunk; ++k) iteration();
+    CBS_HIP(hipStreamEndCapture(s, &graph));
+    CBS_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+  }
+  DevBuf gath;
+  gath.alloc(ctx, (size_t)n * 4 * sizeof(double));
+  std::vector<double> hg((size_t)n * 4);
+  const long long cap = (long long)io.maxit / 2 + chunk + 2;
+  long long launched = 0;
+  try {
+    while (true) {
+      if (gexec) CBS_HIP(hipGraphLaunch(gexec, s));
+      else
+        for (int k = 0; k < chunk; ++k) iteration();
+      CBS_HIP(hipGetLastError());
+      launched += chunk;
+      launch_gather_state(fin_vec.sts.as<RState*>(), n, gath.as<double>(), s);
+      CBS_HIP(hipMemcpyAsync(hg.data(), gath.p, hg.size() * sizeof(double),
+                             hipMemcpyDeviceToHost, s));
+      CBS_HIP(hipStreamSynchronize(s));
+      bool all = true;
+      for (int i = 0; i < n; ++i) all = all && hg[4 * i] != 0.0;
+      if (all || launched >= cap) break;
+    }
+  } catch (...) {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    throw;
+  }
+  if (gexec) (void)hipGraphExecDestroy(gexec);
+  if (graph) (void)hipGraphDestroy(graph);
+  read_states<T>(it, states, s);
+  result.resize(n);
+  for (int i = 0; i < n; ++i) result[i] = hv[i].x;
+}
+
+// ---------------------------------------------------------------------------
+// GMRES (restarted when io.restart > 0), classical Gram-Schmidt with one
+// re-orthogonalisation (CGS2), Givens rotations on the device
+// ---------------------------------------------------------------------------
+template <class T>
+struct GmVec {
+  Cx<T>* x;
+  Cx<T>* K;  // basis V_0 .. V_m, each 3 nv
+  const Cx<T>* V;
+  const Cx<T>* b;
+  long long nv;
+  double eps_inv;
+  double* part;   // kNumPart per vector-kernel block
+  double* gpart;  // dot partials [(m+1)][nwp][2], nwp = 4 per block (one per wave)
+  double* sm;     // small dense data, layout GmLayout
+  RState* st;
+  int nchunk;
+};
+
+struct GmLayout {
+  int m;
+  __host__ __device__ int H(int i, int j) const { return 8 + 2 * (i + j * (m + 1)); }
+  __host__ __device__ int cs(int i) const { return 8 + 2 * (m + 1) * m + i; }
+  __host__ __device__ int sn(int i) const { return 8 + 2 * (m + 1) * m + m + 2 * i; }
+  __host__ __device__ int g(int i) const { return 8 + 2 * (m + 1) * m + 3 * m + 2 * i; }
+  __host__ __device__ int y(int i) const { return 8 + 2 * (m + 1) * m + 3 * m + 2 * (m + 1) + 2 * i; }
+  __host__ __device__ int c(int i) const { return 8 + 2 * (m + 1) * m + 5 * m + 2 * (m + 1) + 2 * i; }
+  __host__ __device__ int size() const { return c(m + 1); }
+};
+// header slots of sm
+enum { GM_KDONE = 0, GM_SCALE = 1 };
+
+__device__ __forceinline__ Cx<double> ld(const double* sm, int o) { return mk<double>(sm[o], sm[o + 1]); }
+__device__ __forceinline__ void st2(double* sm, int o, Cx<double> v) {
+  sm[o] = v.re;
+  sm[o + 1] = v.im;
+}
+__device__ __forceinline__ bool halted(const RState* st) { return st->done | st->cyc; }
+
+// r = gamma b - (use_v ? V_0 : 0) into V_0; partial ||r||^2
+template <class T>
+__global__ void __launch_bounds__(kThreads) k_gm_resid(const GmVec<T>* __restrict__ vs, int use_v) {
+  __shared__ double scratch[(kThreads / 64) * kNumPart];
+  const GmVec<T>& q = vs[blockIdx.y];
+  if (q.st->done) return;
+  const long long start = (long long)blockIdx.x * kChunk;
+  if (start >= q.nv) return;
+  const long long end = min(start + kChunk, q.nv), nv = q.nv;
+  double acc[kNumPart] = {0, 0, 0, 0, 0};
+  for (long long i = start + threadIdx.x; i < end; i += kThreads) {
+    const Cx<T> gam = mul_i((T)q.eps_inv * q.V[i]);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      Cx<T> r = gam * q.b[c * nv + i];
+      if (use_v) r = r - q.K[c * nv + i];
+      q.K[c * nv + i] = r;
+      acc[0] += (double)abs2(r);
+    }
+  }
+  block_reduce_store(acc, q.part + (long long)blockIdx.x * kNumPart, scratch, threadIdx.x);
+}
+
+// V_j *= sm[GM_SCALE]
+template <class T>
+__global__ void __launch_bounds__(kThreads) k_gm_scale(const GmVec<T>* __restrict__ vs, int j) {
+  const GmVec<T>& q = vs[blockIdx.y];
+  if (halted(q.st)) return;
+  const long long start = (long long)blockIdx.x * kChunk;
+  if (start >= q.nv) return;
+  const long long end = min(start + kChunk, q.nv), nv = q.nv;
+  const T sc = (T)q.sm[GM_SCALE];
+  Cx<T>* v = q.K + (long long)j * 3 * nv;
+  for (long long i = start + threadIdx.x; i < end; i += kThreads) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) v[c * nv + i] = sc * v[c * nv + i];
+  }
+}
+
+// per-wave partials of <V_j, w> (w = V_{k+1}) for j = 0..k
+template <class T>
+__global__ void __launch_bounds__(kThreads) k_gm_dots(const GmVec<T>* __restrict__ vs, int k) {
+  const GmVec<T>& q = vs[blockIdx.y];
+  if (halted(q.st)) return;
+  const long long start = (long long)blockIdx.x * kChunk;
+  if (start >= q.nv) return;
+  const long long end = min(start + kChunk, q.nv), nv = q.nv;
+  const Cx<T>* w = q.K + (long long)(k + 1) * 3 * nv;
+  const int nwp = q.nchunk * (kThreads / 64);
+  const int pidx = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+  for (int j = 0; j <= k; ++j) {
+    const Cx<T>* vj = q.K + (long long)j * 3 * nv;
+    double ar = 0.0, ai = 0.0;
+    for (long long i = start + threadIdx.x; i < end; i += kThreads) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const Cx<T> d = cdot(vj[c * nv + i], w[c * nv + i]);
+        ar += (double)d.re;
+        ai += (double)d.im;
+      }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      ar += __shfl_down(ar, off, 64);
+      ai += __shfl_down(ai, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+      q.gpart[((long long)j * nwp + pidx) * 2 + 0] = ar;
+      q.gpart[((long long)j * nwp + pidx) * 2 + 1] = ai;
+    }
+  }
+}
+
+// coef_j = sum of the partials; H(j, k) = coef (pass 0) or += coef (pass 1)
+__global__ void __launch_bounds__(kThreads)
+k_gm_fin_dots(RState* const* __restrict__ sts, double* const* __restrict__ gparts,
+              double* const* __restrict__ sms, const int* __restrict__ nchunks, int m, int k,
+              int pass) {
+  __shared__ double red[2][kThreads / 64];
+  const int rid = blockIdx.x;
+  RState* st = sts[rid];
+  if (halted(st)) return;
+  const double* gp = gparts[rid];
+  double* sm = sms[rid];
+  const int nwp = nchunks[rid] * (kThreads / 64);
+  const GmLayout L{m};
+  for (int j = 0; j <= k; ++j) {
+    double ar = 0.0, ai = 0.0;
+    for (int i = threadIdx.x; i < nwp; i += kThreads) {
+      ar += gp[((long long)j * nwp + i) * 2 + 0];
+      ai += gp[((long long)j * nwp + i) * 2 + 1];
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      ar += __shfl_down(ar, off, 64);
+      ai += __shfl_down(ai, off, 64);
+    }
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) {
+      red[0][threadIdx.x >> 6] = ar;
+      red[1][threadIdx.x >> 6] = ai;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double sr = 0.0, si = 0.0;
+      for (int w = 0; w < kThreads / 64; ++w) {
+        sr += red[0][w];
+        si += red[1][w];
+      }
+      const Cx<double> cj = mk<double>(sr, si);
+      st2(sm, L.c(j), cj);
+      st2(sm, L.H(j, k), pass ? ld(sm, L.H(j, k)) + cj : cj);
+    }
+  }
+}
+
+// w -= sum_j coef_j V_j ; partial ||w||^2
+template <class T>
+__global__ void __launch_bounds__(kThreads) k_gm_update(const GmVec<T>* __restrict__ vs, int m,
+                                                        int k) {
+  __shared__ double scratch[(kThreads / 64) * kNumPart];
+  const GmVec<T>& q = vs[blockIdx.y];
+  if (halted(q.st)) return;
+  const long long start = (long long)blockIdx.x * kChunk;
+  if (start >= q.nv) return;
+  const long long end = min(start + kChunk, q.nv), nv = q.nv;
+  const GmLayout L{m};
+  Cx<T>* w = q.K + (long long)(k + 1) * 3 * nv;
+  double acc[kNumPart] = {0, 0, 0, 0, 0};
+  for (long long i = start + threadIdx.x; i < end; i += kThreads) {
+    Cx<T> a[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) a[c] = w[c * nv + i];
+    for (int j = 0; j <= k; ++j) {
+      const Cx<T> cj = mk<T>((T)q.sm[L.c(j)], (T)q.sm[L.c(j) + 1]);
+      const Cx<T>* vj = q.K + (long long)j * 3 * nv;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) a[c] = a[c] - cj * vj[c * nv + i];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      w[c * nv + i] = a[c];
+      acc[0] += (double)abs2(a[c]);
+    }
+  }
+  block_reduce_store(acc, q.part + (long long)blockIdx.x * kNumPart, scratch, threadIdx.x);
+}
+
+// start of a cycle: beta = ||r||, thresholds, g = (beta, 0, ...), scale = 1/beta
+__global__ void __launch_bounds__(kThreads)
+k_gm_fin_start(RState* const* __restrict__ sts, double* const* __restrict__ parts,
+               const int* __restrict__ ntiles, double* const* __restrict__ sms, int m, int count) {
+  __shared__ double red[kThreads / 64];
+  const int rid = blockIdx.x;
+  RState* st = sts[rid];
+  if (st->done) return;
+  const double r2 = block_sum_parts(parts[rid], ntiles[rid], 0, red);
+  if (threadIdx.x) return;
+  double* sm = sms[rid];
+  const GmLayout L{m};
+  st->iters += count;
+  st->r2 = r2;
+  push_hist(st, r2);
+  if (st->thr2 < 0.0) {
+    const double t = st->abstol + st->tol * sqrt(r2);
+    st->thr2 = t * t;
+  }
+  st->cyc = 0;
+  st->pend = 0;
+  sm[GM_KDONE] = 0.0;
+  if (!(r2 == r2)) {
+    st->done = 3;
+  } else if (r2 <= st->thr2) {
+    st->done = 1;
+  } else if (st->iters >= st->maxit) {
+    st->done = 2;
+  } else {
+    const double beta = sqrt(r2);
+    for (int i = 0; i <= m; ++i) st2(sm, L.g(i), mk<double>(0.0, 0.0));
+    st2(sm, L.g(0), mk<double>(beta, 0.0));
+    sm[GM_SCALE] = 1.0 / beta;
+  }
+}
+
+// finish Arnoldi step k: h = ||w||, rotations, residual estimate, stop decision
+__global__ void __launch_bounds__(kThreads)
+k_gm_fin_arnoldi(RState* const* __restrict__ sts, double* const* __restrict__ parts,
+                 const int* __restrict__ ntiles, double* const* __restrict__ sms, int m, int k) {
+  __shared__ double red[kThreads / 64];
+  const int rid = blockIdx.x;
+  RState* st = sts[rid];
+  if (halted(st)) return;
+  const double h2 = block_sum_parts(parts[rid], ntiles[rid], 0, red);
+  if (threadIdx.x) return;
+  double* sm = sms[rid];
+  const GmLayout L{m};
+  const double h = sqrt(h2);
+  st2(sm, L.H(k + 1, k), mk<double>(h, 0.0));
+  for (int i = 0; i < k; ++i) {  // previous rotations on column k
+    const double c = sm[L.cs(i)];
+    const Cx<double> s = ld(sm, L.sn(i));
+    const Cx<double> a = ld(sm, L.H(i, k)), b = ld(sm, L.H(i + 1, k));
+    st2(sm, L.H(i, k), c * a + s * b);
+    st2(sm, L.H(i + 1, k), c * b - conj(s) * a);
+  }
+  const Cx<double> a = ld(sm, L.H(k, k)), b = ld(sm, L.H(k + 1, k));
+  const double na = sqrt(abs2(a)), den = sqrt(abs2(a) + abs2(b));
+  double c;
+  Cx<double> s;
+  if (den == 0.0) {
+    c = 1.0;
+    s = mk<double>(0.0, 0.0);
+  } else if (na == 0.0) {
+    c = 0.0;
+    s = mk<double>(1.0, 0.0);
+  } else {
+    c = na / den;
+    s = (1.0 / (na * den)) * (a * conj(b));
+  }
+  sm[L.cs(k)] = c;
+  st2(sm, L.sn(k), s);
+  st2(sm, L.H(k, k), c * a + s * b);
+  st2(sm, L.H(k + 1, k), mk<double>(0.0, 0.0));
+  const Cx<double> gk = ld(sm, L.g(k));
+  st2(sm, L.g(k + 1), (-1.0) * (conj(s) * gk));
+  st2(sm, L.g(k), c * gk);
+  const double res2 = abs2(ld(sm, L.g(k + 1)));
+  st->iters += 1;
+  sm[GM_KDONE] = (double)(k + 1);
+  st->r2 = res2;
+  push_hist(st, res2);
+  sm[GM_SCALE] = h > 0.0 ? 1.0 / h : 0.0;
+  if (!(res2 == res2) || !(h == h)) {
+    st->cyc = 1;
+    st->pend = 3;
+  } else if (res2 <= st->thr2 || h == 0.0) {
+    st->cyc = 1;
+    st->pend = 1;
+  } else if (st->iters >= st->maxit) {
+    st->cyc = 1;
+    st->pend = 2;
+  } else if (k + 1 == m) {
+    st->cyc = 1;
+    st->pend = 0;
+  }
+}
+
+// back substitution y = R^-1 g over the kdone completed steps
+__global__ void k_gm_solve(RState* const* __restrict__ sts, double* const* __restrict__ sms,
+                           int m) {
+  const int rid = blockIdx.x;
+  RState* st = sts[rid];
+  if (st->done || threadIdx.x) return;
+  double* sm = sms[rid];
+  const GmLayout L{m};
+  const int kd = (int)sm[GM_KDONE];
+  for (int i = kd - 1; i >= 0; --i) {
+    Cx<double> acc = ld(sm, L.g(i));
+    for (int j = i + 1; j < kd; ++j) acc = acc - ld(sm, L.H(i, j)) * ld(sm, L.y(j));
+    st2(sm, L.y(i), cdiv(acc, ld(sm, L.H(i, i))));
+  }
+}
+
+// x += sum_{j < kdone} y_j V_j
+template <class T>
+__global__ void __launch_bounds__(kThreads) k_gm_xupdate(const GmVec<T>* __restrict__ vs, int m) {
+  const GmVec<T>& q = vs[blockIdx.y];
+  if (q.st->done) return;
+  const long long start = (long long)blockIdx.x * kChunk;
+  if (start >= q.nv) return;
+  const long long end = min(start + kChunk, q.nv), nv = q.nv;
+  const GmLayout L{m};
+  const int kd = (int)q.sm[GM_KDONE];
+  for (long long i = start + threadIdx.x; i < end; i += kThreads) {
+    Cx<T> a[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) a[c] = q.x[c * nv + i];
+    for (int j = 0; j < kd; ++j) {
+      const Cx<T> yj = mk<T>((T)q.sm[L.y(j)], (T)q.sm[L.y(j) + 1]);
+      const Cx<T>* vj = q.K + (long long)j * 3 * nv;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) a[c] = a[c] + yj * vj[c * nv + i];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) q.x[c * nv + i] = a[c];
+  }
+}
+
+__global__ void k_gm_fin_cycle(RState* const* __restrict__ sts) {
+  RState* st = sts[blockIdx.x];
+  if (st->done || threadIdx.x) return;
+  if (st->pend == 1) st->done = 1;
+  else if (st->pend == 2) st->done = 2;
+  else if (st->pend == 3) st->done = 3;
+  st->cyc = 0;
+}
+
+template <class T>
+void gmres_solve(Ctx* ctx, std::vector<SolveItem<T>>& it, const IterOpts& io, hipStream_t s,
+                 std::vector<RState>& states, std::vector<const Cx<T>*>& result) {
+  const int n = (int)it.size();
+  const int m = std::max(1, std::min(io.restart > 0 ? io.restart : io.maxit, 400));
+  const GmLayout lay{m};
+  StateSet gn, fv;
+  long long maxchunk = 1;
+  std::vector<GmVec<T>> hv(n);
+  std::vector<double*> gparts, sms;
+  std::vector<int> nchunks;
+  for (int i = 0; i < n; ++i) {
+    SolveItem<T>& x = it[i];
+    x.w->ensure_k(ctx, m + 1);
+    GmVec<T>& q = hv[i];
+    q.x = x.w->X[0].template as<Cx<T>>();
+    q.K = x.w->kvec(0);
+    q.V = potential<T>(*x.p);
+    q.b = x.b;
+    q.nv = x.p->nvox;
+    q.eps_inv = x.p->eps_inv;
+    q.part = x.w->part.template as<double>();
+    q.nchunk = (int)((q.nv + kChunk - 1) / kChunk);
+    x.w->D.ensure(ctx, (size_t)(m + 1) * q.nchunk * (kThreads / 64) * 2 * sizeof(double));
+    x.w->small.ensure(ctx, (size_t)lay.size() * sizeof(double));
+    q.gpart = x.w->D.template as<double>();
+    q.sm = x.w->small.template as<double>();
+    q.st = x.w->st.template as<RState>();
+    maxchunk = std::max(maxchunk, (long long)q.nchunk);
+    gparts.push_back(q.gpart);
+    sms.push_back(q.sm);
+    nchunks.push_back(q.nchunk);
+    if (io.x0_zero) launch_fill<T>(q.x, 3 * q.nv, s);
+    else copy_dev<T>(q.x, x.b, 3 * q.nv, s);
+  }
+  init_states<T>(ctx, it, io.tol, io.abstol, io.rule, io.maxit, s, gn);
+  DevBuf dv, dg, dsm, dnc;
+  dv.alloc(ctx, n * sizeof(GmVec<T>));
+  dg.alloc(ctx, n * sizeof(double*));
+  dsm.alloc(ctx, n * sizeof(double*));
+  dnc.alloc(ctx, n * sizeof(int));
+  CBS_HIP(hipMemcpyAsync(dv.p, hv.data(), n * sizeof(GmVec<T>), hipMemcpyHostToDevice, s));
+  CBS_HIP(hipMemcpyAsync(dg.p, gparts.data(), n * sizeof(double*), hipMemcpyHostToDevice, s));
+  CBS_HIP(hipMemcpyAsync(dsm.p, sms.data(), n * sizeof(double*), hipMemcpyHostToDevice, s));
+  CBS_HIP(hipMemcpyAsync(dnc.p, nchunks.data(), n * sizeof(int), hipMemcpyHostToDevice, s));
+  const GmVec<T>* d_vs = dv.as<GmVec<T>>();
+
+  // pipelines: x -> V_0 (restart residual) and V_k -> V_{k+1}
+  Pipe<T> mx;
+  std::vector<Pipe<T>> mk_(m);
+  std::vector<RState*> sts;
+  std::vector<double*> parts;
+  std::vector<int> nt_vec;
+  for (int i = 0; i < n; ++i) {
+    Problem& p = *it[i].p;
+    Cx<T>* W = it[i].w->W.template as<Cx<T>>();
+    const long long f = 3 * hv[i].nv;
+    mx.h.push_back(make_desc<T>(p, PRE_V, ZOP_GREEN, POST_PMATVEC, hv[i].x, nullptr, hv[i].K, W,
+                                hv[i].part, hv[i].st));
+    for (int k = 0; k < m; ++k)
+      mk_[k].h.push_back(make_desc<T>(p, PRE_V, ZOP_GREEN, POST_PMATVEC, hv[i].K + k * f, nullptr,
+                                      hv[i].K + (k + 1) * f, W, hv[i].part, hv[i].st));
+    sts.push_back(hv[i].st);
+    parts.push_back(hv[i].part);
+    nt_vec.push_back(hv[i].nchunk);
+  }
+  mx.build(ctx, s);
+  for (auto& pk : mk_) pk.build(ctx, s);
+  fv.build(ctx, sts, parts, nt_vec, s);
+  RState* const* d_sts = fv.sts.as<RState*>();
+  double* const* d_parts = fv.parts.as<double*>();
+  const int* d_nt = fv.ntiles.as<int>();
+  double* const* d_sms = dsm.as<double*>();
+  const dim3 vgrid((unsigned)maxchunk, (unsigned)n);
+
+  auto cycle = [&](bool first) {
+    const bool use_v = !(first && io.x0_zero);
+    if (use_v) mx.launch(s);
+    hipLaunchKernelGGL((k_gm_resid<T>), vgrid, dim3(kThreads), 0, s, d_vs, use_v ? 1 : 0);
+    hipLaunchKernelGGL(k_gm_fin_start, dim3(n), dim3(kThreads), 0, s, d_sts, d_parts, d_nt,
+                       d_sms, m, use_v ? 1 : 0);
+    hipLaunchKernelGGL((k_gm_scale<T>), vgrid, dim3(kThreads), 0, s, d_vs, 0);
+    for (int k = 0; k < m; ++k) {
+      mk_[k].launch(s);
+      for (int pass = 0; pass < 2; ++pass) {
+        hipLaunchKernelGGL((k_gm_dots<T>), vgrid, dim3(kThreads), 0, s, d_vs, k);
+        hipLaunchKernelGGL(k_gm_fin_dots, dim3(n), dim3(kThreads), 0, s, d_sts,
+                           dg.as<double*>(), d_sms, dnc.as<int>(), m, k, pass);
+        hipLaunchKernelGGL((k_gm_update<T>), vgrid, dim3(kThreads), 0, s, d_vs, m, k);
+      }
+      hipLaunchKernelGGL(k_gm_fin_arnoldi, dim3(n), dim3(kThreads), 0, s, d_sts, d_parts, d_nt,
+                         d_sms, m, k);
+      if (k + 1 < m)
+        hipLaunchKernelGGL((k_gm_scale<T>), vgrid, dim3(kThreads), 0, s, d_vs, k + 1);
+    }
+    hipLaunchKernelGGL(k_gm_solve, dim3(n), dim3(64), 0, s, d_sts, d_sms, m);
+    hipLaunchKernelGGL((k_gm_xupdate<T>), vgrid, dim3(kThreads), 0, s, d_vs, m);
+    hipLaunchKernelGGL(k_gm_fin_cycle, dim3(n), dim3(64), 0, s, d_sts);
+  };
+
+  DevBuf gath;
+  gath.alloc(ctx, (size_t)n * 4 * sizeof(double));
+  std::vector<double> hg((size_t)n * 4);
+  auto all_done = [&]() {
+    launch_gather_state(d_sts, n, gath.as<double>(), s);
+    CBS_HIP(hipMemcpyAsync(hg.data(), gath.p, hg.size() * sizeof(double), hipMemcpyDeviceToHost,
+                           s));
+    CBS_HIP(hipStreamSynchronize(s));
+    bool all = true;
+    for (int i = 0; i < n; ++i) all = all && hg[4 * i] != 0.0;
+    return all;
+  };
+  cycle(true);
+  CBS_HIP(hipGetLastError());
+  const long long max_cycles = (long long)io.maxit / std::max(1, m) + 3;
+  if (!all_done()) {
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    try {
+      if (io.use_graph) {
+        CBS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        cycle(false);
+        CBS_HIP(hipStreamEndCapture(s, &graph));
+        CBS_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+      }
+      for (long long cyc = 0; cyc < max_cycles; ++cyc) {
+        if (gexec) CBS_HIP(hipGraphLaunch(gexec, s));
+        else cycle(false);
+        CBS_HIP(hipGetLastError());
+        if (all_done()) break;
+      }
+    } catch (...) {
+      if (gexec) (void)hipGraphExecDestroy(gexec);
+      if (graph) (void)hipGraphDestroy(graph);
+      throw;
+    }
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+  read_states<T>(it, states, s);
+  result.resize(n);
+  for (int i = 0; i < n; ++i) result[i] = hv[i].x;
 }
 
 #define CBS_KRYLOV_INST(T)                                                                    \

@@ -1,0 +1,54 @@
+// Block-level FP64 reductions shared by the pass, solver and Krylov kernels.
+// Fixed summation order: results depend only on the data and the tiling of
+// the right-hand side they belong to (never on other right-hand sides).
+#pragma once
+
+#include "cbs_internal.h"
+
+namespace cbs {
+
+// Reduce kNumPart doubles over the block; thread 0 writes them to dst.
+// `scratch` holds (kThreads/64)*kNumPart doubles of LDS.
+__device__ __forceinline__ void block_reduce_store(double (&acc)[kNumPart], double* dst,
+                                                   double* scratch, int tid) {
+#pragma unroll
+  for (int k = 0; k < kNumPart; ++k) {
+    double v = acc[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    acc[k] = v;
+  }
+  const int lane = tid & 63, wave = tid >> 6;
+  __syncthreads();  // the caller's LDS tile may still be read by other waves
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < kNumPart; ++k) scratch[wave * kNumPart + k] = acc[k];
+  }
+  __syncthreads();
+  if (tid == 0) {
+#pragma unroll
+    for (int k = 0; k < kNumPart; ++k) {
+      double s = 0.0;
+      for (int w = 0; w < kThreads / 64; ++w) s += scratch[w * kNumPart + k];
+      dst[k] = s;
+    }
+  }
+}
+
+// Sum column k of kNumPart-strided partials [0, ntile); every thread gets the total.
+__device__ __forceinline__ double block_sum_parts(const double* __restrict__ part, int ntile,
+                                                  int k, double* red) {
+  const int tid = threadIdx.x;
+  double v = 0.0;
+  for (int i = tid; i < ntile; i += kThreads) v += part[(long long)i * kNumPart + k];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  __syncthreads();
+  if ((tid & 63) == 0) red[tid >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int w = 0; w < kThreads / 64; ++w) s += red[w];
+  return s;
+}
+
+}  // namespace cbs
