@@ -87,7 +87,7 @@ struct Work {
   size_t nv = 0;
   int S = 1;
   int nk = 0;
-  void ensure(Ctx* c, long long nvox, int s, int ntiles, bool need_r);
+  void ensure(Ctx* c, long long nvox, int s, int ntiles, bool need_r, long long wcomp);
   void ensure_k(Ctx* c, int n) {
     K.ensure(c, (size_t)n * 3 * nv * sizeof(Cx<T>));
     nk = n;

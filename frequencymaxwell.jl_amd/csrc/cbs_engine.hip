@@ -164,6 +164,12 @@ DevProb<T> make_desc(Problem& p, int pre, int zop, int post, const Cx<T>* in0, c
   d.nvox = p.nvox;
   d.k2c_re = p.d.k2_re;
   d.k2c_im = p.d.k2_im;
+  {
+    const double m2 = p.d.k2_re * p.d.k2_re + p.d.k2_im * p.d.k2_im;
+    d.ik2c_re = p.d.k2_re / m2;
+    d.ik2c_im = -p.d.k2_im / m2;
+  }
+  w_strides(p.n[0], p.n[1], p.n[2], (int)sizeof(Cx<T>), d.ws);
   d.zs_re = 1.0 / (double)p.nvox;
   d.zs_im = 0.0;
   d.eps_inv = p.eps_inv;
@@ -206,6 +212,13 @@ void Pipe<T>::build(Ctx* c, hipStream_t s) {
     // the xinv epilogue reuses the tile LDS for its block reduction
     L.lds[pass] = std::max(lds, (size_t)((kThreads / 64) * kNumPart * sizeof(double)));
   }
+  for (int ax = 0; ax < 3; ++ax) {
+    int u = h[0].rt[ax] ? 0 : h[0].n[ax];
+    for (int i = 1; i < np; ++i)
+      if (h[i].rt[ax] || h[i].n[ax] != u) u = 0;
+    // specialised kernels exist for lengths >= 32; shorter ones use the switch kernel
+    L.uni[ax] = u >= 32 ? u : 0;
+  }
   for (int pass = 0; pass < 4; ++pass)
     if (L.lds[pass] > 160 * 1024)
       throw Error(CBS_EINVAL, "grid axis too long for an LDS tile (" +
@@ -236,12 +249,12 @@ void StateSet::build(Ctx* c, const std::vector<RState*>& st, const std::vector<d
 }
 
 template <class T>
-void Work<T>::ensure(Ctx* c, long long nvox, int s, int ntiles, bool need_r) {
+void Work<T>::ensure(Ctx* c, long long nvox, int s, int ntiles, bool need_r, long long wcomp) {
   const size_t f = (size_t)nvox * 3 * sizeof(Cx<T>);
   X[0].ensure(c, f);
   X[1].ensure(c, f);
   B.ensure(c, f);
-  W.ensure(c, f * s);
+  W.ensure(c, (size_t)s * 3 * (size_t)wcomp * sizeof(Cx<T>));
   if (need_r) R.ensure(c, f);
   const long long chunk = (long long)kThreads * 16;
   const size_t np = std::max<size_t>((size_t)ntiles, (size_t)((nvox + chunk - 1) / chunk));
@@ -272,7 +285,7 @@ void green_apply_dev(Problem& p, Cx<T>* field, hipStream_t s) {
   Work<T>& w = work<T>(p);
   DevProb<T> probe = make_desc<T>(p, PRE_COPY, ZOP_GREEN, POST_STORE, field, nullptr, field,
                                   nullptr, nullptr, nullptr);
-  w.ensure(p.ctx, p.nvox, p.S, probe.tiles[3], false);
+  w.ensure(p.ctx, p.nvox, p.S, probe.tiles[3], false, probe.ws[2]);
   Pipe<T> pipe;
   pipe.h.push_back(make_desc<T>(p, PRE_COPY, ZOP_GREEN, POST_STORE, field, nullptr, field,
                                 w.W.template as<Cx<T>>(), nullptr, nullptr));
@@ -288,7 +301,7 @@ void operator_apply_dev(Problem& p, Cx<T>* y, const Cx<T>* x, double ar, double 
   Work<T>& w = work<T>(p);
   DevProb<T> d = make_desc<T>(p, PRE_V, ZOP_GREEN, POST_OPERATOR, x, nullptr, y, nullptr,
                               nullptr, nullptr);
-  w.ensure(p.ctx, p.nvox, p.S, d.tiles[3], false);
+  w.ensure(p.ctx, p.nvox, p.S, d.tiles[3], false, d.ws[2]);
   d.w = w.W.template as<Cx<T>>();
   d.a_re = ar;
   d.a_im = ai;
@@ -307,7 +320,7 @@ void curl_apply_dev(Problem& p, const Cx<T>* E, Cx<T>* H, double sr, double si, 
   Work<T>& w = work<T>(p);
   DevProb<T> d =
       make_desc<T>(p, PRE_COPY, ZOP_CURL, POST_STORE, E, nullptr, H, nullptr, nullptr, nullptr);
-  w.ensure(p.ctx, p.nvox, p.S, d.tiles[3], false);
+  w.ensure(p.ctx, p.nvox, p.S, d.tiles[3], false, d.ws[2]);
   d.w = w.W.template as<Cx<T>>();
   // H = scale * IFFT(i k x FFT(E)); the 1/N of the inverse transform is folded in
   d.zs_re = sr / (double)p.nvox;

@@ -205,6 +205,8 @@ struct DevProb {
   const double* kk[3][2];  // Green k along each axis for the +/- pipelines
   const double* kc[3];     // curl k along each axis
   double k2c_re, k2c_im;   // k_square of the Green operator
+  double ik2c_re, ik2c_im; // 1 / k_square
+  long long ws[3];         // W strides (elements): row, plane, component
   double zs_re, zs_im;     // scale folded into the z-pass multiply
   double eps_inv;          // 1 / eps  (gamma = i V / eps)
   double a_re, a_im, b_re, b_im;  // POST_OPERATOR coefficients
@@ -217,6 +219,22 @@ struct DevProb {
   const RState* st; // skip the launch work when st->done != 0 (may be null)
   int tiles[4];     // tiles of the xfwd, y, z and xinv passes
 };
+
+// Padded strides of the spectrum workspace W (elements).  Rows stay 16-byte
+// aligned (even element counts); the pads keep the row and plane strides off
+// large powers of two so the rows of a strided tile spread over HBM channels.
+inline void w_strides(int nx, int ny, int nz, int elem_bytes, long long ws[3]) {
+  const int px = (nx >= 32 && (nx & (nx - 1)) == 0) ? (elem_bytes == 8 ? 8 : 4) : (nx & 1);
+  ws[0] = nx + px;
+  long long plane = (long long)ny * ws[0];
+  const long long unit = 256 / elem_bytes;  // 256-byte granule
+  plane = ((plane + unit - 1) / unit) * unit;
+  if ((plane / unit) % 2 == 0) plane += unit;  // odd number of 256-byte granules
+  ws[1] = plane;
+  long long comp = (long long)nz * ws[1];
+  if ((comp / unit) % 2 == 0) comp += unit;
+  ws[2] = comp;
+}
 
 // host-side tile counts (must match the kernels)
 template <class T>

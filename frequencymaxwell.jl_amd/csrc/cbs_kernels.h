@@ -14,7 +14,18 @@ struct PipeLaunch {
   int S;                    // pipelines per RHS (uniform over the table)
   int blocks[4];            // xfwd, y, z, xinv
   size_t lds[4];            // dynamic LDS bytes per pass
+  int uni[3];               // common x / y / z length of the table (0: mixed or runtime plan)
 };
+
+// per-sweep launchers (cbs_pass_x.hip, cbs_pass_yz.hip)
+template <class T>
+void launch_xfwd(const PipeLaunch<T>& L, hipStream_t s);
+template <class T>
+void launch_xinv(const PipeLaunch<T>& L, hipStream_t s);
+template <class T>
+void launch_ypass(const PipeLaunch<T>& L, int dir, hipStream_t s);
+template <class T>
+void launch_zpass(const PipeLaunch<T>& L, hipStream_t s);
 
 template <class T>
 void launch_pipeline(const PipeLaunch<T>& L, hipStream_t s);

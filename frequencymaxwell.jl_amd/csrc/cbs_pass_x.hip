@@ -1,0 +1,118 @@
+// Sweeps A (x forward + V x pre-op) and E (x inverse + fused epilogue).
+// NC > 0: one specialised length for the whole table; NC == 0: per-block
+// switch over the lengths of a mixed-grid table.
+#include "cbs_passes.h"
+
+namespace cbs {
+
+template <class T, int S, int NC>
+__global__ void __launch_bounds__(kThreads)
+k_xfwd(const DevProb<T>* __restrict__ probs, const int* __restrict__ bstart, int nprob) {
+  const int pi = find_prob(bstart, nprob, blockIdx.x);
+  const DevProb<T>& pr = probs[pi];
+  if (skip_prob(pr)) return;
+  const int tile = blockIdx.x - bstart[pi];
+  const int tid = threadIdx.x;
+  if constexpr (NC != 0) {
+    xfwd_tile<T, NC, S>(pr, tile, tid);
+  } else {
+    switch (pr.rt[0] ? 0 : pr.n[0]) {
+      case 8: xfwd_tile<T, 8, S>(pr, tile, tid); break;
+      case 16: xfwd_tile<T, 16, S>(pr, tile, tid); break;
+      case 32: xfwd_tile<T, 32, S>(pr, tile, tid); break;
+      case 64: xfwd_tile<T, 64, S>(pr, tile, tid); break;
+      case 96: xfwd_tile<T, 96, S>(pr, tile, tid); break;
+      case 128: xfwd_tile<T, 128, S>(pr, tile, tid); break;
+      case 160: xfwd_tile<T, 160, S>(pr, tile, tid); break;
+      case 192: xfwd_tile<T, 192, S>(pr, tile, tid); break;
+      case 256: xfwd_tile<T, 256, S>(pr, tile, tid); break;
+      case 288: xfwd_tile<T, 288, S>(pr, tile, tid); break;
+      default: xfwd_tile<T, 0, S>(pr, tile, tid); break;
+    }
+  }
+}
+
+template <class T, int S, int NC>
+__global__ void __launch_bounds__(kThreads)
+k_xinv(const DevProb<T>* __restrict__ probs, const int* __restrict__ bstart, int nprob) {
+  const int pi = find_prob(bstart, nprob, blockIdx.x);
+  const DevProb<T>& pr = probs[pi];
+  if (skip_prob(pr)) return;
+  const int tile = blockIdx.x - bstart[pi];
+  const int tid = threadIdx.x;
+  if constexpr (NC != 0) {
+    xinv_tile<T, NC, S>(pr, tile, tid);
+  } else {
+    switch (pr.rt[0] ? 0 : pr.n[0]) {
+      case 8: xinv_tile<T, 8, S>(pr, tile, tid); break;
+      case 16: xinv_tile<T, 16, S>(pr, tile, tid); break;
+      case 32: xinv_tile<T, 32, S>(pr, tile, tid); break;
+      case 64: xinv_tile<T, 64, S>(pr, tile, tid); break;
+      case 96: xinv_tile<T, 96, S>(pr, tile, tid); break;
+      case 128: xinv_tile<T, 128, S>(pr, tile, tid); break;
+      case 160: xinv_tile<T, 160, S>(pr, tile, tid); break;
+      case 192: xinv_tile<T, 192, S>(pr, tile, tid); break;
+      case 256: xinv_tile<T, 256, S>(pr, tile, tid); break;
+      case 288: xinv_tile<T, 288, S>(pr, tile, tid); break;
+      default: xinv_tile<T, 0, S>(pr, tile, tid); break;
+    }
+  }
+}
+
+template <class T, int S, int NC>
+static void go_xfwd(const PipeLaunch<T>& L, hipStream_t s) {
+  hipLaunchKernelGGL((k_xfwd<T, S, NC>), dim3(L.blocks[0]), dim3(kThreads), L.lds[0], s, L.probs,
+                     L.bstart, L.nprob);
+}
+template <class T, int S, int NC>
+static void go_xinv(const PipeLaunch<T>& L, hipStream_t s) {
+  hipLaunchKernelGGL((k_xinv<T, S, NC>), dim3(L.blocks[3]), dim3(kThreads), L.lds[3], s, L.probs,
+                     L.bstart + 3 * (L.nprob + 1), L.nprob);
+}
+
+template <class T, int S>
+static void xfwd_len(const PipeLaunch<T>& L, hipStream_t s) {
+  switch (L.uni[0]) {
+    case 32: go_xfwd<T, S, 32>(L, s); break;
+    case 64: go_xfwd<T, S, 64>(L, s); break;
+    case 96: go_xfwd<T, S, 96>(L, s); break;
+    case 128: go_xfwd<T, S, 128>(L, s); break;
+    case 160: go_xfwd<T, S, 160>(L, s); break;
+    case 192: go_xfwd<T, S, 192>(L, s); break;
+    case 256: go_xfwd<T, S, 256>(L, s); break;
+    case 288: go_xfwd<T, S, 288>(L, s); break;
+    default: go_xfwd<T, S, 0>(L, s); break;
+  }
+}
+template <class T, int S>
+static void xinv_len(const PipeLaunch<T>& L, hipStream_t s) {
+  switch (L.uni[0]) {
+    case 32: go_xinv<T, S, 32>(L, s); break;
+    case 64: go_xinv<T, S, 64>(L, s); break;
+    case 96: go_xinv<T, S, 96>(L, s); break;
+    case 128: go_xinv<T, S, 128>(L, s); break;
+    case 160: go_xinv<T, S, 160>(L, s); break;
+    case 192: go_xinv<T, S, 192>(L, s); break;
+    case 256: go_xinv<T, S, 256>(L, s); break;
+    case 288: go_xinv<T, S, 288>(L, s); break;
+    default: go_xinv<T, S, 0>(L, s); break;
+  }
+}
+
+template <class T>
+void launch_xfwd(const PipeLaunch<T>& L, hipStream_t s) {
+  if (L.S == 2) xfwd_len<T, 2>(L, s);
+  else xfwd_len<T, 1>(L, s);
+}
+template <class T>
+void launch_xinv(const PipeLaunch<T>& L, hipStream_t s) {
+  if (L.S == 2) xinv_len<T, 2>(L, s);
+  else xinv_len<T, 1>(L, s);
+}
+
+template void launch_xfwd<float>(const PipeLaunch<float>&, hipStream_t);
+template void launch_xfwd<double>(const PipeLaunch<double>&, hipStream_t);
+template void launch_xinv<float>(const PipeLaunch<float>&, hipStream_t);
+template void launch_xinv<double>(const PipeLaunch<double>&, hipStream_t);
+
+}  // namespace cbs

@@ -1,0 +1,114 @@
+// Sweeps B / D (y transforms) and C (z transform, k-space multiply, z inverse).
+// NC > 0: one specialised length for the whole table; NC == 0: per-block
+// switch over the lengths of a mixed-grid table.
+#include "cbs_passes.h"
+
+namespace cbs {
+
+template <class T, int DIR, int NC>
+__global__ void __launch_bounds__(kThreads)
+k_ypass(const DevProb<T>* __restrict__ probs, const int* __restrict__ bstart, int nprob) {
+  const int pi = find_prob(bstart, nprob, blockIdx.x);
+  const DevProb<T>& pr = probs[pi];
+  if (skip_prob(pr)) return;
+  const int tile = blockIdx.x - bstart[pi];
+  const int tid = threadIdx.x;
+  if constexpr (NC != 0) {
+    ypass_tile<T, NC, DIR>(pr, tile, tid);
+  } else {
+    switch (pr.rt[1] ? 0 : pr.n[1]) {
+      case 8: ypass_tile<T, 8, DIR>(pr, tile, tid); break;
+      case 16: ypass_tile<T, 16, DIR>(pr, tile, tid); break;
+      case 32: ypass_tile<T, 32, DIR>(pr, tile, tid); break;
+      case 64: ypass_tile<T, 64, DIR>(pr, tile, tid); break;
+      case 96: ypass_tile<T, 96, DIR>(pr, tile, tid); break;
+      case 128: ypass_tile<T, 128, DIR>(pr, tile, tid); break;
+      case 160: ypass_tile<T, 160, DIR>(pr, tile, tid); break;
+      case 192: ypass_tile<T, 192, DIR>(pr, tile, tid); break;
+      case 256: ypass_tile<T, 256, DIR>(pr, tile, tid); break;
+      case 288: ypass_tile<T, 288, DIR>(pr, tile, tid); break;
+      default: ypass_tile<T, 0, DIR>(pr, tile, tid); break;
+    }
+  }
+}
+
+template <class T, int NC>
+__global__ void __launch_bounds__(kThreads)
+k_zpass(const DevProb<T>* __restrict__ probs, const int* __restrict__ bstart, int nprob) {
+  const int pi = find_prob(bstart, nprob, blockIdx.x);
+  const DevProb<T>& pr = probs[pi];
+  if (skip_prob(pr)) return;
+  const int tile = blockIdx.x - bstart[pi];
+  const int tid = threadIdx.x;
+  if constexpr (NC != 0) {
+    zpass_tile<T, NC>(pr, tile, tid);
+  } else {
+    switch (pr.rt[2] ? 0 : pr.n[2]) {
+      case 8: zpass_tile<T, 8>(pr, tile, tid); break;
+      case 16: zpass_tile<T, 16>(pr, tile, tid); break;
+      case 32: zpass_tile<T, 32>(pr, tile, tid); break;
+      case 64: zpass_tile<T, 64>(pr, tile, tid); break;
+      case 96: zpass_tile<T, 96>(pr, tile, tid); break;
+      case 128: zpass_tile<T, 128>(pr, tile, tid); break;
+      case 160: zpass_tile<T, 160>(pr, tile, tid); break;
+      case 192: zpass_tile<T, 192>(pr, tile, tid); break;
+      case 256: zpass_tile<T, 256>(pr, tile, tid); break;
+      case 288: zpass_tile<T, 288>(pr, tile, tid); break;
+      default: zpass_tile<T, 0>(pr, tile, tid); break;
+    }
+  }
+}
+
+template <class T, int DIR, int NC>
+static void go_y(const PipeLaunch<T>& L, hipStream_t s) {
+  hipLaunchKernelGGL((k_ypass<T, DIR, NC>), dim3(L.blocks[1]), dim3(kThreads), L.lds[1], s,
+                     L.probs, L.bstart + (L.nprob + 1), L.nprob);
+}
+template <class T, int NC>
+static void go_z(const PipeLaunch<T>& L, hipStream_t s) {
+  hipLaunchKernelGGL((k_zpass<T, NC>), dim3(L.blocks[2]), dim3(kThreads), L.lds[2], s, L.probs,
+                     L.bstart + 2 * (L.nprob + 1), L.nprob);
+}
+
+template <class T, int DIR>
+static void y_len(const PipeLaunch<T>& L, hipStream_t s) {
+  switch (L.uni[1]) {
+    case 32: go_y<T, DIR, 32>(L, s); break;
+    case 64: go_y<T, DIR, 64>(L, s); break;
+    case 96: go_y<T, DIR, 96>(L, s); break;
+    case 128: go_y<T, DIR, 128>(L, s); break;
+    case 160: go_y<T, DIR, 160>(L, s); break;
+    case 192: go_y<T, DIR, 192>(L, s); break;
+    case 256: go_y<T, DIR, 256>(L, s); break;
+    case 288: go_y<T, DIR, 288>(L, s); break;
+    default: go_y<T, DIR, 0>(L, s); break;
+  }
+}
+
+template <class T>
+void launch_ypass(const PipeLaunch<T>& L, int dir, hipStream_t s) {
+  if (dir < 0) y_len<T, -1>(L, s);
+  else y_len<T, 1>(L, s);
+}
+
+template <class T>
+void launch_zpass(const PipeLaunch<T>& L, hipStream_t s) {
+  switch (L.uni[2]) {
+    case 32: go_z<T, 32>(L, s); break;
+    case 64: go_z<T, 64>(L, s); break;
+    case 96: go_z<T, 96>(L, s); break;
+    case 128: go_z<T, 128>(L, s); break;
+    case 160: go_z<T, 160>(L, s); break;
+    case 192: go_z<T, 192>(L, s); break;
+    case 256: go_z<T, 256>(L, s); break;
+    case 288: go_z<T, 288>(L, s); break;
+    default: go_z<T, 0>(L, s); break;
+  }
+}
+
+template void launch_ypass<float>(const PipeLaunch<float>&, int, hipStream_t);
+template void launch_ypass<double>(const PipeLaunch<double>&, int, hipStream_t);
+template void launch_zpass<float>(const PipeLaunch<float>&, hipStream_t);
+template void launch_zpass<double>(const PipeLaunch<double>&, hipStream_t);
+
+}  // namespace cbs

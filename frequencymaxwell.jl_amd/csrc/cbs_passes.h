@@ -1,0 +1,395 @@
+// Device tile code of the five sweeps (included by cbs_pass_x.hip and
+// cbs_pass_yz.hip).
+//
+// Field vectors (x, b, r, ...) are dense [component][z][y][x].  The spectrum
+// workspace W uses padded strides ws[0] (row), ws[1] (plane), ws[2]
+// (component): power-of-two strides would put every row of a strided tile on
+// the same HBM channels.
+#pragma once
+
+#include "cbs_fft.h"
+#include "cbs_kernels.h"
+#include "cbs_reduce.h"
+
+namespace cbs {
+
+extern __shared__ __align__(16) unsigned char g_smem[];
+
+__device__ __forceinline__ int find_prob(const int* __restrict__ bstart, int nprob, int blk) {
+  int lo = 0, hi = nprob - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (bstart[mid] <= blk) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <class T>
+__device__ __forceinline__ Cx<T> ramp_at(const DevProb<T>& pr, int x, int y, int z) {
+  return pr.ramp[0][x] * pr.ramp[1][y] * pr.ramp[2][z];
+}
+
+template <class T>
+__device__ __forceinline__ bool skip_prob(const DevProb<T>& pr) {
+  return pr.st && (pr.st->done | pr.st->cyc);
+}
+
+// two adjacent complex FP32 values as one 16-byte access
+struct alignas(16) C2f {
+  Cx<float> a, b;
+};
+
+// ---------------------------------------------------------------------------
+// sweep A: x forward transform, pre-op u = V x (or x), shift ramps
+// ---------------------------------------------------------------------------
+template <class T, int NC, int S>
+__device__ __forceinline__ void xfwd_tile(const DevProb<T>& pr, int tile, int tid) {
+  constexpr bool CT = NC > 0;
+  constexpr int eb = (int)sizeof(Cx<T>);
+  const int n = CT ? NC : pr.n[0];
+  const int P = xlines(n, eb, S, !CT);
+  const int Q = 3 * S * P;
+  const int QS = xpad_stride(Q);
+  Cx<T>* lds = reinterpret_cast<Cx<T>*>(g_smem);
+  const int ny = pr.n[1];
+  const int nlines = ny * pr.n[2];
+  const int l0 = tile * P;
+  const int np = min(P, nlines - l0);
+  const long long nv = pr.nvox;
+  const bool useV = pr.pre == PRE_V;
+  for (int e = tid; e < P * n; e += kThreads) {
+    const int m = e % n, p = e / n;
+    Cx<T> u[3];
+    Cx<T> rp = mk<T>(T(1), T(0));
+    if (p < np) {
+      const long long g = (long long)(l0 + p) * n + m;
+      const Cx<T> vv = useV ? pr.V[g] : mk<T>(T(1), T(0));
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const Cx<T> xv = pr.in0[c * nv + g];
+        u[c] = useV ? vv * xv : xv;
+      }
+      if (pr.has_ramp) {
+        const int l = l0 + p;
+        rp = ramp_at(pr, m, l % ny, l / ny);
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) u[c] = mk<T>(T(0), T(0));
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      lds[m * QS + c * P + p] = pr.has_ramp ? rp * u[c] : u[c];
+      if (S == 2) lds[m * QS + (3 + c) * P + p] = pr.has_ramp ? conj(rp) * u[c] : u[c];
+    }
+  }
+  __syncthreads();
+  Cx<T>* res = lds;
+  if constexpr (CT) {
+    constexpr int Pc = xlines(CT ? NC : 8, eb, S, false);
+    constexpr int Qc = 3 * S * Pc;
+    fft_ct<T, NC, Qc, xpad_stride(Qc), -1>(lds, pr.tw[0], tid);
+  } else {
+    res = fft_rt<T, -1>(lds, lds + n * QS, n, Q, QS, pr.plan[0], pr.tw[0], tid);
+  }
+  for (int e = tid; e < P * n; e += kThreads) {
+    const int m = e % n, p = e / n;
+    if (p < np) {
+      const int l = l0 + p;
+      const long long wo = (long long)(l / ny) * pr.ws[1] + (long long)(l % ny) * pr.ws[0] + m;
+#pragma unroll
+      for (int q = 0; q < 3 * S; ++q) pr.w[q * pr.ws[2] + wo] = res[m * QS + q * P + p];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// strided tile load/store: rows of P consecutive x columns, `rows` rows at
+// row stride `rs` (elements), LDS row stride `ls` with the P columns at
+// offset `lo`.  FP32 moves two complex values per lane (16 bytes).
+// ---------------------------------------------------------------------------
+template <class T>
+__device__ __forceinline__ void tile_load(Cx<T>* __restrict__ lds, const Cx<T>* __restrict__ src,
+                                          int P, int npx, int rows, long long rs, int ls, int lo,
+                                          int tid) {
+  if constexpr (sizeof(Cx<T>) == 8) {
+    if ((P & 1) == 0) {
+      const int P2 = P >> 1;
+      for (int e = tid; e < P2 * rows; e += kThreads) {
+        const int p2 = e % P2, m = e / P2;
+        const int p = 2 * p2;
+        C2f v;
+        if (p + 1 < npx) {
+          v = *reinterpret_cast<const C2f*>(src + m * rs + p);
+        } else {
+          v.a = p < npx ? src[m * rs + p] : mk<float>(0.f, 0.f);
+          v.b = mk<float>(0.f, 0.f);
+        }
+        *reinterpret_cast<C2f*>(lds + m * ls + lo + p) = v;
+      }
+      return;
+    }
+  }
+  for (int e = tid; e < P * rows; e += kThreads) {
+    const int p = e % P, m = e / P;
+    lds[m * ls + lo + p] = p < npx ? src[m * rs + p] : mk<T>(T(0), T(0));
+  }
+}
+
+template <class T>
+__device__ __forceinline__ void tile_store(const Cx<T>* __restrict__ lds, Cx<T>* __restrict__ dst,
+                                           int P, int npx, int rows, long long rs, int ls, int lo,
+                                           int tid) {
+  if constexpr (sizeof(Cx<T>) == 8) {
+    if ((P & 1) == 0) {
+      const int P2 = P >> 1;
+      for (int e = tid; e < P2 * rows; e += kThreads) {
+        const int p2 = e % P2, m = e / P2;
+        const int p = 2 * p2;
+        const C2f v = *reinterpret_cast<const C2f*>(lds + m * ls + lo + p);
+        if (p + 1 < npx) {
+          *reinterpret_cast<C2f*>(dst + m * rs + p) = v;
+        } else if (p < npx) {
+          dst[m * rs + p] = v.a;
+        }
+      }
+      return;
+    }
+  }
+  for (int e = tid; e < P * rows; e += kThreads) {
+    const int p = e % P, m = e / P;
+    if (p < npx) dst[m * rs + p] = lds[m * ls + lo + p];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// sweeps B / D: y transform in place, P consecutive x columns per tile
+// ---------------------------------------------------------------------------
+template <class T, int NC, int DIR>
+__device__ __forceinline__ void ypass_tile(const DevProb<T>& pr, int tile, int tid) {
+  constexpr bool CT = NC > 0;
+  constexpr int eb = (int)sizeof(Cx<T>);
+  const int n = CT ? NC : pr.n[1];
+  const int P = ycols(n, eb, !CT);
+  Cx<T>* lds = reinterpret_cast<Cx<T>*>(g_smem);
+  const int nx = pr.n[0];
+  const int nxb = (nx + P - 1) / P;
+  const int xb = tile % nxb;
+  const int rest = tile / nxb;
+  const int z = rest % pr.n[2];
+  const int sc = rest / pr.n[2];  // stream * 3 + component
+  const int x0 = xb * P;
+  const int npx = min(P, nx - x0);
+  Cx<T>* base = pr.w + sc * pr.ws[2] + (long long)z * pr.ws[1] + x0;
+  tile_load(lds, base, P, npx, n, pr.ws[0], P, 0, tid);
+  __syncthreads();
+  Cx<T>* res = lds;
+  if constexpr (CT) {
+    constexpr int Pc = ycols(CT ? NC : 8, eb, false);
+    fft_ct<T, NC, Pc, Pc, DIR>(lds, pr.tw[1], tid);
+  } else {
+    res = fft_rt<T, DIR>(lds, lds + n * P, n, P, P, pr.plan[1], pr.tw[1], tid);
+  }
+  tile_store(res, base, P, npx, n, pr.ws[0], P, 0, tid);
+}
+
+// ---------------------------------------------------------------------------
+// sweep C: z forward transform, k-space multiply, z inverse transform
+// ---------------------------------------------------------------------------
+template <class T>
+__device__ __forceinline__ void kspace_op(const DevProb<T>& pr, int s, int x, int y, int z,
+                                          Cx<T>& e0, Cx<T>& e1, Cx<T>& e2) {
+  const Cx<double> E0 = cvt<double>(e0), E1 = cvt<double>(e1), E2 = cvt<double>(e2);
+  const Cx<double> zs = mk<double>(pr.zs_re, pr.zs_im);
+  if (pr.zop == ZOP_GREEN) {
+    // G(k) = (I - k k^T / k_c^2) / (|k|^2 - k_c^2)   (dyadic_green.jl:203-219)
+    const double kx = pr.kk[0][s][x], ky = pr.kk[1][s][y], kz = pr.kk[2][s][z];
+    const Cx<double> ik2c = mk<double>(pr.ik2c_re, pr.ik2c_im);
+    const Cx<double> kdote = ik2c * (kx * E0 + ky * E1 + kz * E2);
+    const double dre = kx * kx + ky * ky + kz * kz - pr.k2c_re, dim = -pr.k2c_im;
+    const double inv = 1.0 / (dre * dre + dim * dim);
+    // a = zs / den
+    const Cx<double> a = zs * mk<double>(dre * inv, -dim * inv);
+    e0 = cvt<T>(a * (E0 - kx * kdote));
+    e1 = cvt<T>(a * (E1 - ky * kdote));
+    e2 = cvt<T>(a * (E2 - kz * kdote));
+  } else {
+    // i k x E   (curl.jl:88-98)
+    const double kx = pr.kc[0][x], ky = pr.kc[1][y], kz = pr.kc[2][z];
+    const Cx<double> d0 = mul_i(ky * E2 - kz * E1);
+    const Cx<double> d1 = mul_i(kz * E0 - kx * E2);
+    const Cx<double> d2 = mul_i(kx * E1 - ky * E0);
+    e0 = cvt<T>(zs * d0);
+    e1 = cvt<T>(zs * d1);
+    e2 = cvt<T>(zs * d2);
+  }
+}
+
+template <class T, int NC>
+__device__ __forceinline__ void zpass_tile(const DevProb<T>& pr, int tile, int tid) {
+  constexpr bool CT = NC > 0;
+  constexpr int eb = (int)sizeof(Cx<T>);
+  const int n = CT ? NC : pr.n[2];
+  const int P = zcols(n, eb, !CT);
+  const int Q = 3 * P;
+  Cx<T>* lds = reinterpret_cast<Cx<T>*>(g_smem);
+  const int nx = pr.n[0], ny = pr.n[1];
+  const int nxb = (nx + P - 1) / P;
+  const int xb = tile % nxb;
+  const int rest = tile / nxb;
+  const int y = rest % ny;
+  const int s = rest / ny;
+  const int x0 = xb * P;
+  const int npx = min(P, nx - x0);
+  Cx<T>* base = pr.w + 3 * s * pr.ws[2] + (long long)y * pr.ws[0] + x0;
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+    tile_load(lds, base + c * pr.ws[2], P, npx, n, pr.ws[1], Q, c * P, tid);
+  __syncthreads();
+  Cx<T>* res = lds;
+  if constexpr (CT) {
+    constexpr int Pc = zcols(CT ? NC : 8, eb, false);
+    fft_ct<T, NC, 3 * Pc, 3 * Pc, -1>(lds, pr.tw[2], tid);
+  } else {
+    res = fft_rt<T, -1>(lds, lds + n * Q, n, Q, Q, pr.plan[2], pr.tw[2], tid);
+  }
+  for (int e = tid; e < P * n; e += kThreads) {
+    const int p = e % P, m = e / P;
+    if (p < npx) {
+      Cx<T>* r = res + m * Q + p;
+      Cx<T> e0 = r[0], e1 = r[P], e2 = r[2 * P];
+      kspace_op(pr, s, x0 + p, y, m, e0, e1, e2);
+      r[0] = e0;
+      r[P] = e1;
+      r[2 * P] = e2;
+    }
+  }
+  __syncthreads();
+  if constexpr (CT) {
+    constexpr int Pc = zcols(CT ? NC : 8, eb, false);
+    fft_ct<T, NC, 3 * Pc, 3 * Pc, 1>(lds, pr.tw[2], tid);
+  } else {
+    Cx<T>* other = res == lds ? lds + n * Q : lds;
+    res = fft_rt<T, 1>(res, other, n, Q, Q, pr.plan[2], pr.tw[2], tid);
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+    tile_store(res, base + c * pr.ws[2], P, npx, n, pr.ws[1], Q, c * P, tid);
+}
+
+// ---------------------------------------------------------------------------
+// sweep E: x inverse transform, inverse ramps, averaging, fused epilogue
+// ---------------------------------------------------------------------------
+template <class T, int NC, int S>
+__device__ __forceinline__ void xinv_tile(const DevProb<T>& pr, int tile, int tid) {
+  constexpr bool CT = NC > 0;
+  constexpr int eb = (int)sizeof(Cx<T>);
+  const int n = CT ? NC : pr.n[0];
+  const int P = xlines(n, eb, S, !CT);
+  const int Q = 3 * S * P;
+  const int QS = xpad_stride(Q);
+  Cx<T>* lds = reinterpret_cast<Cx<T>*>(g_smem);
+  const int ny = pr.n[1];
+  const int nlines = ny * pr.n[2];
+  const int l0 = tile * P;
+  const int np = min(P, nlines - l0);
+  const long long nv = pr.nvox;
+  for (int e = tid; e < P * n; e += kThreads) {
+    const int m = e % n, p = e / n;
+    const int l = l0 + p;
+    const long long wo = (long long)(l / ny) * pr.ws[1] + (long long)(l % ny) * pr.ws[0] + m;
+#pragma unroll
+    for (int q = 0; q < 3 * S; ++q)
+      lds[m * QS + q * P + p] = p < np ? pr.w[q * pr.ws[2] + wo] : mk<T>(T(0), T(0));
+  }
+  __syncthreads();
+  Cx<T>* res = lds;
+  if constexpr (CT) {
+    constexpr int Pc = xlines(CT ? NC : 8, eb, S, false);
+    constexpr int Qc = 3 * S * Pc;
+    fft_ct<T, NC, Qc, xpad_stride(Qc), 1>(lds, pr.tw[0], tid);
+  } else {
+    res = fft_rt<T, 1>(lds, lds + n * QS, n, Q, QS, pr.plan[0], pr.tw[0], tid);
+  }
+  double acc[kNumPart] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  const int post = pr.post;
+  for (int e = tid; e < P * n; e += kThreads) {
+    const int m = e % n, p = e / n;
+    if (p >= np) continue;
+    const long long g = (long long)(l0 + p) * n + m;
+    Cx<T> w[3];
+    if (pr.has_ramp) {
+      const int l = l0 + p;
+      const Cx<T> rp = ramp_at(pr, m, l % ny, l / ny);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        // + pipeline: exp(+2 pi i phi) = conj(ramp);  - pipeline: exp(-2 pi i phi) = ramp
+        Cx<T> v = conj(rp) * res[m * QS + c * P + p];
+        if (S == 2) v = T(0.5) * (v + rp * res[m * QS + (3 + c) * P + p]);
+        w[c] = v;
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        Cx<T> v = res[m * QS + c * P + p];
+        if (S == 2) v = T(0.5) * (v + res[m * QS + (3 + c) * P + p]);
+        w[c] = v;
+      }
+    }
+    if (post == POST_STORE) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) pr.out0[c * nv + g] = w[c];
+    } else if (post == POST_OPERATOR) {
+      const Cx<T> a = mk<T>((T)pr.a_re, (T)pr.a_im), b = mk<T>((T)pr.b_re, (T)pr.b_im);
+      const bool use_b = pr.b_re != 0.0 || pr.b_im != 0.0;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        Cx<T> y = a * (pr.in0[c * nv + g] - w[c]);
+        if (use_b) y = y + b * pr.out0[c * nv + g];
+        pr.out0[c * nv + g] = y;
+      }
+    } else {
+      // gamma = i V / eps
+      const Cx<T> gam = mul_i((T)pr.eps_inv * pr.V[g]);
+      if (post == POST_BORN) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const Cx<T> xv = pr.in0[c * nv + g];
+          const Cx<T> r = gam * (pr.in1[c * nv + g] - xv + w[c]);
+          pr.out0[c * nv + g] = xv + r;
+          acc[0] += (double)abs2(r);
+        }
+      } else if (post == POST_RESID) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const Cx<T> r = pr.in1[c * nv + g] - pr.in0[c * nv + g] + w[c];
+          pr.out0[c * nv + g] = r;
+          acc[0] += (double)abs2(gam * r);
+        }
+      } else {  // POST_PMATVEC
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const Cx<T> xv = pr.in0[c * nv + g];
+          const Cx<T> o = gam * (xv - w[c]);
+          pr.out0[c * nv + g] = o;
+          if (pr.in1) {
+            const Cx<T> d = cdot(pr.in1[c * nv + g], o);
+            acc[0] += (double)d.re;
+            acc[1] += (double)d.im;
+          }
+          const Cx<T> d2 = cdot(o, xv);
+          acc[2] += (double)d2.re;
+          acc[3] += (double)d2.im;
+          acc[4] += (double)abs2(o);
+        }
+      }
+    }
+  }
+  if (post >= POST_BORN) {
+    double* scratch = reinterpret_cast<double*>(g_smem);
+    block_reduce_store(acc, pr.part + (long long)tile * kNumPart, scratch, tid);
+  }
+}
+
+}  // namespace cbs

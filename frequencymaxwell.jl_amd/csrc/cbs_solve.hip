@@ -262,7 +262,7 @@ static void prepare_items(std::vector<RhsIO>& io, std::vector<SolveItem<T>>& it,
     Work<T>& w = work_slot<T>(*r.p, k);
     DevProb<T> d = make_desc<T>(*r.p, PRE_V, ZOP_GREEN, POST_BORN, nullptr, nullptr, nullptr,
                                 nullptr, nullptr, nullptr);
-    w.ensure(r.p->ctx, r.p->nvox, r.p->S, d.tiles[3], need_r);
+    w.ensure(r.p->ctx, r.p->nvox, r.p->S, d.tiles[3], need_r, d.ws[2]);
     SolveItem<T> x;
     x.p = r.p;
     x.w = &w;
