@@ -11,6 +11,10 @@ constexpr int kThreads = 256;     // threads per block of every pass kernel
 constexpr int kMaxStages = 12;    // radix stages of one axis transform
 constexpr int kLdsBudget = 65536; // LDS bytes a pass tile may use (two blocks per CU)
 constexpr int kNumPart = 5;       // doubles of partial sums written per xinv tile
+#ifndef CBS_XLINES
+#define CBS_XLINES 4  // measured best of 16 / 8 / 4 on the 160x160x96 and 256x256x128 grids
+#endif
+constexpr int kXLines = CBS_XLINES;  // most x lines per x-pass tile
 
 // ---------------------------------------------------------------------------
 // complex arithmetic (interleaved re, im; 8 or 16 bytes, naturally aligned)
@@ -143,9 +147,10 @@ __host__ __device__ constexpr int tile_p(int n, int elem_bytes, int pencils_per_
   while (p > 1 && bufs * (n * (pencils_per_p * p + pad)) * elem_bytes > kLdsBudget) p /= 2;
   return p;
 }
-// x-axis passes: S streams * 3 components * P lines per tile
+// x-axis passes: 3 components * P lines per tile; specialised lengths run the
+// S pipelines one after the other through the tile, runtime plans hold all S.
 __host__ __device__ constexpr int xlines(int nx, int eb, int S, bool rt) {
-  return tile_p(nx, eb, 3 * S, 16, rt, 1);
+  return tile_p(nx, eb, 3 * (rt ? S : 1), kXLines, rt, 1);
 }
 // y pass: P columns of one component
 __host__ __device__ constexpr int ycols(int ny, int eb, bool rt) {
@@ -261,7 +266,7 @@ inline int tile_lds_bytes(const DevProb<T>& p, int pass) {
     case 3: {
       const bool rt = p.rt[0] != 0;
       const int P = xlines(nx, eb, p.S, rt);
-      return (rt ? 2 : 1) * nx * xpad_stride(3 * p.S * P) * eb;
+      return (rt ? 2 : 1) * nx * xpad_stride(3 * (rt ? p.S : 1) * P) * eb;
     }
     case 1: {
       const bool rt = p.rt[1] != 0;

@@ -41,7 +41,10 @@ struct alignas(16) C2f {
 };
 
 // ---------------------------------------------------------------------------
-// sweep A: x forward transform, pre-op u = V x (or x), shift ramps
+// sweep A: x forward transform, pre-op u = V x (or x), shift ramps.
+// Specialised lengths run the S pipelines one after the other through one
+// 3P-pencil LDS tile (u and the ramp stay in registers); the runtime-plan path
+// keeps all 3 S P pencils in LDS.
 // ---------------------------------------------------------------------------
 template <class T, int NC, int S>
 __device__ __forceinline__ void xfwd_tile(const DevProb<T>& pr, int tile, int tid) {
@@ -49,8 +52,6 @@ __device__ __forceinline__ void xfwd_tile(const DevProb<T>& pr, int tile, int ti
   constexpr int eb = (int)sizeof(Cx<T>);
   const int n = CT ? NC : pr.n[0];
   const int P = xlines(n, eb, S, !CT);
-  const int Q = 3 * S * P;
-  const int QS = xpad_stride(Q);
   Cx<T>* lds = reinterpret_cast<Cx<T>*>(g_smem);
   const int ny = pr.n[1];
   const int nlines = ny * pr.n[2];
@@ -58,48 +59,101 @@ __device__ __forceinline__ void xfwd_tile(const DevProb<T>& pr, int tile, int ti
   const int np = min(P, nlines - l0);
   const long long nv = pr.nvox;
   const bool useV = pr.pre == PRE_V;
-  for (int e = tid; e < P * n; e += kThreads) {
-    const int m = e % n, p = e / n;
-    Cx<T> u[3];
-    Cx<T> rp = mk<T>(T(1), T(0));
-    if (p < np) {
-      const long long g = (long long)(l0 + p) * n + m;
-      const Cx<T> vv = useV ? pr.V[g] : mk<T>(T(1), T(0));
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const Cx<T> xv = pr.in0[c * nv + g];
-        u[c] = useV ? vv * xv : xv;
-      }
-      if (pr.has_ramp) {
-        const int l = l0 + p;
-        rp = ramp_at(pr, m, l % ny, l / ny);
-      }
-    } else {
-#pragma unroll
-      for (int c = 0; c < 3; ++c) u[c] = mk<T>(T(0), T(0));
-    }
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      lds[m * QS + c * P + p] = pr.has_ramp ? rp * u[c] : u[c];
-      if (S == 2) lds[m * QS + (3 + c) * P + p] = pr.has_ramp ? conj(rp) * u[c] : u[c];
-    }
-  }
-  __syncthreads();
-  Cx<T>* res = lds;
   if constexpr (CT) {
     constexpr int Pc = xlines(CT ? NC : 8, eb, S, false);
-    constexpr int Qc = 3 * S * Pc;
-    fft_ct<T, NC, Qc, xpad_stride(Qc), -1>(lds, pr.tw[0], tid);
-  } else {
-    res = fft_rt<T, -1>(lds, lds + n * QS, n, Q, QS, pr.plan[0], pr.tw[0], tid);
-  }
-  for (int e = tid; e < P * n; e += kThreads) {
-    const int m = e % n, p = e / n;
-    if (p < np) {
-      const int l = l0 + p;
-      const long long wo = (long long)(l / ny) * pr.ws[1] + (long long)(l % ny) * pr.ws[0] + m;
+    constexpr int Qc = 3 * Pc;
+    constexpr int QSc = xpad_stride(Qc);
+    constexpr int E = (Pc * NC + kThreads - 1) / kThreads;
+    Cx<T> u[E][3];
+    Cx<T> rp[E];
 #pragma unroll
-      for (int q = 0; q < 3 * S; ++q) pr.w[q * pr.ws[2] + wo] = res[m * QS + q * P + p];
+    for (int k = 0; k < E; ++k) {
+      const int e = tid + k * kThreads;
+      const int m = e % NC, p = e / NC;
+      rp[k] = mk<T>(T(1), T(0));
+      if (e < Pc * NC && p < np) {
+        const long long g = (long long)(l0 + p) * NC + m;
+        const Cx<T> vv = useV ? pr.V[g] : mk<T>(T(1), T(0));
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const Cx<T> xv = pr.in0[c * nv + g];
+          u[k][c] = useV ? vv * xv : xv;
+        }
+        if (pr.has_ramp) {
+          const int l = l0 + p;
+          rp[k] = ramp_at(pr, m, l % ny, l / ny);
+        }
+      } else {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) u[k][c] = mk<T>(T(0), T(0));
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+#pragma unroll
+      for (int k = 0; k < E; ++k) {
+        const int e = tid + k * kThreads;
+        if (e < Pc * NC) {
+          const int m = e % NC, p = e / NC;
+          const Cx<T> r = s == 0 ? rp[k] : conj(rp[k]);
+#pragma unroll
+          for (int c = 0; c < 3; ++c) lds[m * QSc + c * Pc + p] = pr.has_ramp ? r * u[k][c] : u[k][c];
+        }
+      }
+      __syncthreads();
+      fft_ct<T, NC, Qc, QSc, -1>(lds, pr.tw[0], tid);
+#pragma unroll
+      for (int k = 0; k < E; ++k) {
+        const int e = tid + k * kThreads;
+        const int m = e % NC, p = e / NC;
+        if (e < Pc * NC && p < np) {
+          const int l = l0 + p;
+          const long long wo = (long long)(l / ny) * pr.ws[1] + (long long)(l % ny) * pr.ws[0] + m;
+#pragma unroll
+          for (int c = 0; c < 3; ++c) pr.w[(s * 3 + c) * pr.ws[2] + wo] = lds[m * QSc + c * Pc + p];
+        }
+      }
+      if (S == 2 && s == 0) __syncthreads();  // the next pipeline reuses the tile
+    }
+  } else {
+    const int Q = 3 * S * P;
+    const int QS = xpad_stride(Q);
+    for (int e = tid; e < P * n; e += kThreads) {
+      const int m = e % n, p = e / n;
+      Cx<T> u[3];
+      Cx<T> rp = mk<T>(T(1), T(0));
+      if (p < np) {
+        const long long g = (long long)(l0 + p) * n + m;
+        const Cx<T> vv = useV ? pr.V[g] : mk<T>(T(1), T(0));
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const Cx<T> xv = pr.in0[c * nv + g];
+          u[c] = useV ? vv * xv : xv;
+        }
+        if (pr.has_ramp) {
+          const int l = l0 + p;
+          rp = ramp_at(pr, m, l % ny, l / ny);
+        }
+      } else {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) u[c] = mk<T>(T(0), T(0));
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        lds[m * QS + c * P + p] = pr.has_ramp ? rp * u[c] : u[c];
+        if (S == 2) lds[m * QS + (3 + c) * P + p] = pr.has_ramp ? conj(rp) * u[c] : u[c];
+      }
+    }
+    __syncthreads();
+    Cx<T>* res = fft_rt<T, -1>(lds, lds + n * QS, n, Q, QS, pr.plan[0], pr.tw[0], tid);
+    for (int e = tid; e < P * n; e += kThreads) {
+      const int m = e % n, p = e / n;
+      if (p < np) {
+        const int l = l0 + p;
+        const long long wo = (long long)(l / ny) * pr.ws[1] + (long long)(l % ny) * pr.ws[0] + m;
+#pragma unroll
+        for (int q = 0; q < 3 * S; ++q) pr.w[q * pr.ws[2] + wo] = res[m * QS + q * P + p];
+      }
     }
   }
 }
@@ -279,117 +333,174 @@ __device__ __forceinline__ void zpass_tile(const DevProb<T>& pr, int tile, int t
 }
 
 // ---------------------------------------------------------------------------
-// sweep E: x inverse transform, inverse ramps, averaging, fused epilogue
+// sweep E: x inverse transform, inverse ramps, averaging, fused epilogue.
+// Specialised lengths transform the S pipelines one after the other through
+// one 3P-pencil LDS tile and average them in registers.
 // ---------------------------------------------------------------------------
+template <class T>
+__device__ __forceinline__ void xinv_epilogue(const DevProb<T>& pr, long long g, const Cx<T> (&w)[3],
+                                              double (&acc)[kNumPart]) {
+  const long long nv = pr.nvox;
+  const int post = pr.post;
+  if (post == POST_STORE) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) pr.out0[c * nv + g] = w[c];
+  } else if (post == POST_OPERATOR) {
+    const Cx<T> a = mk<T>((T)pr.a_re, (T)pr.a_im), b = mk<T>((T)pr.b_re, (T)pr.b_im);
+    const bool use_b = pr.b_re != 0.0 || pr.b_im != 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      Cx<T> y = a * (pr.in0[c * nv + g] - w[c]);
+      if (use_b) y = y + b * pr.out0[c * nv + g];
+      pr.out0[c * nv + g] = y;
+    }
+  } else {
+    // gamma = i V / eps
+    const Cx<T> gam = mul_i((T)pr.eps_inv * pr.V[g]);
+    if (post == POST_BORN) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const Cx<T> xv = pr.in0[c * nv + g];
+        const Cx<T> r = gam * (pr.in1[c * nv + g] - xv + w[c]);
+        pr.out0[c * nv + g] = xv + r;
+        acc[0] += (double)abs2(r);
+      }
+    } else if (post == POST_RESID) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const Cx<T> r = pr.in1[c * nv + g] - pr.in0[c * nv + g] + w[c];
+        pr.out0[c * nv + g] = r;
+        acc[0] += (double)abs2(gam * r);
+      }
+    } else {  // POST_PMATVEC
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const Cx<T> xv = pr.in0[c * nv + g];
+        const Cx<T> o = gam * (xv - w[c]);
+        pr.out0[c * nv + g] = o;
+        if (pr.in1) {
+          const Cx<T> d = cdot(pr.in1[c * nv + g], o);
+          acc[0] += (double)d.re;
+          acc[1] += (double)d.im;
+        }
+        const Cx<T> d2 = cdot(o, xv);
+        acc[2] += (double)d2.re;
+        acc[3] += (double)d2.im;
+        acc[4] += (double)abs2(o);
+      }
+    }
+  }
+}
+
 template <class T, int NC, int S>
 __device__ __forceinline__ void xinv_tile(const DevProb<T>& pr, int tile, int tid) {
   constexpr bool CT = NC > 0;
   constexpr int eb = (int)sizeof(Cx<T>);
   const int n = CT ? NC : pr.n[0];
   const int P = xlines(n, eb, S, !CT);
-  const int Q = 3 * S * P;
-  const int QS = xpad_stride(Q);
   Cx<T>* lds = reinterpret_cast<Cx<T>*>(g_smem);
   const int ny = pr.n[1];
   const int nlines = ny * pr.n[2];
   const int l0 = tile * P;
   const int np = min(P, nlines - l0);
-  const long long nv = pr.nvox;
-  for (int e = tid; e < P * n; e += kThreads) {
-    const int m = e % n, p = e / n;
-    const int l = l0 + p;
-    const long long wo = (long long)(l / ny) * pr.ws[1] + (long long)(l % ny) * pr.ws[0] + m;
-#pragma unroll
-    for (int q = 0; q < 3 * S; ++q)
-      lds[m * QS + q * P + p] = p < np ? pr.w[q * pr.ws[2] + wo] : mk<T>(T(0), T(0));
-  }
-  __syncthreads();
-  Cx<T>* res = lds;
+  double acc[kNumPart] = {0.0, 0.0, 0.0, 0.0, 0.0};
   if constexpr (CT) {
     constexpr int Pc = xlines(CT ? NC : 8, eb, S, false);
-    constexpr int Qc = 3 * S * Pc;
-    fft_ct<T, NC, Qc, xpad_stride(Qc), 1>(lds, pr.tw[0], tid);
-  } else {
-    res = fft_rt<T, 1>(lds, lds + n * QS, n, Q, QS, pr.plan[0], pr.tw[0], tid);
-  }
-  double acc[kNumPart] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  const int post = pr.post;
-  for (int e = tid; e < P * n; e += kThreads) {
-    const int m = e % n, p = e / n;
-    if (p >= np) continue;
-    const long long g = (long long)(l0 + p) * n + m;
-    Cx<T> w[3];
-    if (pr.has_ramp) {
-      const int l = l0 + p;
-      const Cx<T> rp = ramp_at(pr, m, l % ny, l / ny);
+    constexpr int Qc = 3 * Pc;
+    constexpr int QSc = xpad_stride(Qc);
+    constexpr int E = (Pc * NC + kThreads - 1) / kThreads;
+    Cx<T> w[E][3];
+    Cx<T> rp[E];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        // + pipeline: exp(+2 pi i phi) = conj(ramp);  - pipeline: exp(-2 pi i phi) = ramp
-        Cx<T> v = conj(rp) * res[m * QS + c * P + p];
-        if (S == 2) v = T(0.5) * (v + rp * res[m * QS + (3 + c) * P + p]);
-        w[c] = v;
-      }
-    } else {
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        Cx<T> v = res[m * QS + c * P + p];
-        if (S == 2) v = T(0.5) * (v + res[m * QS + (3 + c) * P + p]);
-        w[c] = v;
+    for (int k = 0; k < E; ++k) {
+      const int e = tid + k * kThreads;
+      const int m = e % NC, p = e / NC;
+      rp[k] = mk<T>(T(1), T(0));
+      if (pr.has_ramp && e < Pc * NC && p < np) {
+        const int l = l0 + p;
+        rp[k] = ramp_at(pr, m, l % ny, l / ny);
       }
     }
-    if (post == POST_STORE) {
 #pragma unroll
-      for (int c = 0; c < 3; ++c) pr.out0[c * nv + g] = w[c];
-    } else if (post == POST_OPERATOR) {
-      const Cx<T> a = mk<T>((T)pr.a_re, (T)pr.a_im), b = mk<T>((T)pr.b_re, (T)pr.b_im);
-      const bool use_b = pr.b_re != 0.0 || pr.b_im != 0.0;
+    for (int s = 0; s < S; ++s) {
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        Cx<T> y = a * (pr.in0[c * nv + g] - w[c]);
-        if (use_b) y = y + b * pr.out0[c * nv + g];
-        pr.out0[c * nv + g] = y;
+      for (int k = 0; k < E; ++k) {
+        const int e = tid + k * kThreads;
+        if (e < Pc * NC) {
+          const int m = e % NC, p = e / NC;
+          const int l = l0 + p;
+          const long long wo = (long long)(l / ny) * pr.ws[1] + (long long)(l % ny) * pr.ws[0] + m;
+#pragma unroll
+          for (int c = 0; c < 3; ++c)
+            lds[m * QSc + c * Pc + p] =
+                p < np ? pr.w[(s * 3 + c) * pr.ws[2] + wo] : mk<T>(T(0), T(0));
+        }
       }
-    } else {
-      // gamma = i V / eps
-      const Cx<T> gam = mul_i((T)pr.eps_inv * pr.V[g]);
-      if (post == POST_BORN) {
+      __syncthreads();
+      fft_ct<T, NC, Qc, QSc, 1>(lds, pr.tw[0], tid);
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const Cx<T> xv = pr.in0[c * nv + g];
-          const Cx<T> r = gam * (pr.in1[c * nv + g] - xv + w[c]);
-          pr.out0[c * nv + g] = xv + r;
-          acc[0] += (double)abs2(r);
-        }
-      } else if (post == POST_RESID) {
+      for (int k = 0; k < E; ++k) {
+        const int e = tid + k * kThreads;
+        if (e < Pc * NC) {
+          const int m = e % NC, p = e / NC;
+          // + pipeline: exp(+2 pi i phi) = conj(ramp);  - pipeline: exp(-2 pi i phi) = ramp
+          const Cx<T> r = s == 0 ? conj(rp[k]) : rp[k];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const Cx<T> r = pr.in1[c * nv + g] - pr.in0[c * nv + g] + w[c];
-          pr.out0[c * nv + g] = r;
-          acc[0] += (double)abs2(gam * r);
-        }
-      } else {  // POST_PMATVEC
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const Cx<T> xv = pr.in0[c * nv + g];
-          const Cx<T> o = gam * (xv - w[c]);
-          pr.out0[c * nv + g] = o;
-          if (pr.in1) {
-            const Cx<T> d = cdot(pr.in1[c * nv + g], o);
-            acc[0] += (double)d.re;
-            acc[1] += (double)d.im;
+          for (int c = 0; c < 3; ++c) {
+            const Cx<T> v = lds[m * QSc + c * Pc + p];
+            const Cx<T> t = pr.has_ramp ? r * v : v;
+            w[k][c] = s == 0 ? t : w[k][c] + t;
           }
-          const Cx<T> d2 = cdot(o, xv);
-          acc[2] += (double)d2.re;
-          acc[3] += (double)d2.im;
-          acc[4] += (double)abs2(o);
         }
       }
+      if (S == 2 && s == 0) __syncthreads();  // the next pipeline reuses the tile
+    }
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      const int e = tid + k * kThreads;
+      const int m = e % NC, p = e / NC;
+      if (e < Pc * NC && p < np) {
+        if (S == 2) {
+#pragma unroll
+          for (int c = 0; c < 3; ++c) w[k][c] = T(0.5) * w[k][c];
+        }
+        xinv_epilogue(pr, (long long)(l0 + p) * NC + m, w[k], acc);
+      }
+    }
+  } else {
+    const int Q = 3 * S * P;
+    const int QS = xpad_stride(Q);
+    for (int e = tid; e < P * n; e += kThreads) {
+      const int m = e % n, p = e / n;
+      const int l = l0 + p;
+      const long long wo = (long long)(l / ny) * pr.ws[1] + (long long)(l % ny) * pr.ws[0] + m;
+#pragma unroll
+      for (int q = 0; q < 3 * S; ++q)
+        lds[m * QS + q * P + p] = p < np ? pr.w[q * pr.ws[2] + wo] : mk<T>(T(0), T(0));
+    }
+    __syncthreads();
+    Cx<T>* res = fft_rt<T, 1>(lds, lds + n * QS, n, Q, QS, pr.plan[0], pr.tw[0], tid);
+    for (int e = tid; e < P * n; e += kThreads) {
+      const int m = e % n, p = e / n;
+      if (p >= np) continue;
+      Cx<T> w[3];
+      const int l = l0 + p;
+      const Cx<T> rp = pr.has_ramp ? ramp_at(pr, m, l % ny, l / ny) : mk<T>(T(1), T(0));
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        Cx<T> v = pr.has_ramp ? conj(rp) * res[m * QS + c * P + p] : res[m * QS + c * P + p];
+        if (S == 2) {
+          const Cx<T> v2 = res[m * QS + (3 + c) * P + p];
+          v = T(0.5) * (v + (pr.has_ramp ? rp * v2 : v2));
+        }
+        w[c] = v;
+      }
+      xinv_epilogue(pr, (long long)(l0 + p) * n + m, w, acc);
     }
   }
-  if (post >= POST_BORN) {
+  if (pr.post >= POST_BORN) {
     double* scratch = reinterpret_cast<double*>(g_smem);
     block_reduce_store(acc, pr.part + (long long)tile * kNumPart, scratch, tid);
   }
 }
-
 }  // namespace cbs

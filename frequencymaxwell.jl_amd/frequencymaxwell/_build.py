@@ -45,9 +45,9 @@ def _hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
-def _compile(src: Path, verbose: bool) -> Path:
-    obj = BUILD / (src.stem + ".o")
-    cmd = [_hipcc(), *HIPCC_FLAGS, "-c", str(src), "-o", str(obj)]
+def _compile(src: Path, verbose: bool, defines=(), objdir: Path = BUILD) -> Path:
+    obj = objdir / (src.stem + ".o")
+    cmd = [_hipcc(), *HIPCC_FLAGS, *[f"-D{d}" for d in defines], "-c", str(src), "-o", str(obj)]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
@@ -56,17 +56,23 @@ def _compile(src: Path, verbose: bool) -> Path:
     return obj
 
 
-def build(verbose: bool = False, jobs: int | None = None) -> Path:
-    """Compile all HIP units and link the shared library; returns its path."""
-    BUILD.mkdir(parents=True, exist_ok=True)
+def build(verbose: bool = False, jobs: int | None = None, defines=(), out: Path | None = None) -> Path:
+    """Compile all HIP units and link the shared library; returns its path.
+
+    ``defines`` (``NAME=VALUE`` strings) and ``out`` build a tuning variant
+    elsewhere; the default builds the library the package loads.
+    """
+    objdir = BUILD if not defines else BUILD / ("v_" + "_".join(d.replace("=", "") for d in defines))
+    objdir.mkdir(parents=True, exist_ok=True)
     LIBDIR.mkdir(parents=True, exist_ok=True)
     srcs = sorted(CSRC.glob("*.hip"))
     if not srcs:
         raise RuntimeError(f"no HIP sources under {CSRC}")
     jobs = jobs or min(len(srcs), int(os.environ.get("MAX_JOBS", os.cpu_count() or 4)), 16)
     with ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(lambda s: _compile(s, verbose), srcs))
-    out = lib_path()
+        objs = list(ex.map(lambda s: _compile(s, verbose, defines, objdir), srcs))
+    out = Path(out) if out else lib_path()
+    out.parent.mkdir(parents=True, exist_ok=True)
     tmp = out.with_suffix(".so.tmp")
     cmd = [_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp)]
     if verbose:
@@ -79,5 +85,11 @@ def build(verbose: bool = False, jobs: int | None = None) -> Path:
 
 
 if __name__ == "__main__":
-    p = build(verbose="-v" in sys.argv)
-    print(p)
+    import argparse
+
+    ap = argparse.ArgumentParser()
+    ap.add_argument("-v", action="store_true")
+    ap.add_argument("-D", dest="defines", action="append", default=[])
+    ap.add_argument("--out")
+    a = ap.parse_args()
+    print(build(verbose=a.v, defines=a.defines, out=a.out))

@@ -154,7 +154,8 @@ def load(path: os.PathLike | None = None):
     global _lib
     with _lock:
         if _lib is None:
-            p = Path(path) if path else LIB_PATH
+            env = os.environ.get("CBS_LIBRARY")
+            p = Path(path) if path else (Path(env) if env else LIB_PATH)
             if not p.exists():
                 raise CBSError(CBS_EDEVICE, f"HIP library not built: {p} (run __graft_entry__.build())")
             lib = C.CDLL(str(p))
