@@ -92,13 +92,16 @@ def cpu_baseline(med, seconds: float):
                       f"(NumPy + scipy.fft restatement, {o.WORKERS} FFT threads)"}
 
 
-def read_traffic(name: str):
-    p = ROOT / "profiles" / name
-    if p.exists():
-        try:
-            return json.loads(p.read_text()).get("traffic_bytes_per_launch")
-        except (ValueError, OSError):
-            return None
+def read_traffic(summary: str, kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE, calibrated on a plain copy of known size)."""
+    p = ROOT / "profiles" / summary
+    if not p.exists():
+        return None
+    for line in p.read_text().splitlines():
+        f = line.split()
+        if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel:
+            return float(f[2])
     return None
 
 
@@ -118,6 +121,9 @@ def main():
     ap.add_argument("--restart", type=int, default=0)
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--profile", action="store_true",
+                    help="profiler run: only the timed FP32 sweeps and the 1 GiB calibration copy")
+    ap.add_argument("--profile-reps", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build measures the single-GPU path only "
@@ -125,10 +131,17 @@ def main():
         sys.exit(2)
 
     import frequencymaxwell as fm
+    from frequencymaxwell.engine import stream_copy_ms
 
     med, sol, pot, eps_imag, prob, Sterm = build_workload(fm)
     nvox = int(np.prod(pot.shape))
     prob.set_source(Sterm)
+    if args.profile:
+        ms, by = prob.time_sweeps("fp32", reps=args.profile_reps)
+        cms = stream_copy_ms(1 << 30, reps=5)
+        print(json.dumps({"sweeps_ms": dict(zip(KERNELS, ms)), "sweeps_bytes": dict(zip(KERNELS, by)),
+                          "copy_bytes": 1 << 30, "copy_ms": cms}), flush=True)
+        return
     fast = fm.make_opts(method=args.method, precision="fp32_refine", tol=1e-6, stop_rule="rhs",
                         maxiters=args.maxiters, use_graph=not args.no_graph,
                         check_every=args.check_every, gmres_restart=args.restart)
@@ -206,7 +219,7 @@ def main():
     ms, by = prob.time_sweeps("fp32", reps=20)
     k = int(np.argmax(ms))
     achieved = by[k] / (ms[k] * 1e-3) / 1e9
-    traffic = read_traffic(f"traffic_{KERNELS[k]}_160x160x96_fp32.json")
+    traffic = read_traffic("sweeps_160x160x96_fp32.md", KERNELS[k])
     out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                        "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                        "kernel": KERNELS[k], "algorithmic_bytes_per_launch": by[k],
@@ -216,6 +229,8 @@ def main():
     sweep_ms = sum(ms)
     out["sweep_iteration_ms"] = sweep_ms
     out["sweep_roofline_frac"] = sum(by) / (sweep_ms * 1e-3) / 1e9 / HBM_PEAK_GBS
+    cms = stream_copy_ms(1 << 30, reps=10)
+    out["stream_copy_GBs"] = 2 * (1 << 30) / (cms * 1e-3) / 1e9  # read + write
 
     if not args.no_cpu:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)

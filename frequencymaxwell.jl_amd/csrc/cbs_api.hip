@@ -434,6 +434,34 @@ cbs_status cbs_time_sweeps(cbs_problem* pp, int32_t precision, int32_t reps, dou
   });
 }
 
+cbs_status cbs_stream_copy(cbs_ctx* ctx, uint64_t bytes, int32_t reps, double* ms) {
+  return guarded([&] {
+    CBS_REQUIRE(ctx && ms, "NULL argument");
+    CBS_REQUIRE(bytes >= 16 && bytes % 16 == 0, "bytes must be a positive multiple of 16");
+    CBS_REQUIRE(reps >= 1, "reps must be >= 1");
+    set_device(ctx->impl);
+    hipStream_t s = ctx->impl.stream;
+    DevBuf a, b;
+    a.alloc(&ctx->impl, bytes);
+    b.alloc(&ctx->impl, bytes);
+    launch_fill<float>(a.as<Cx<float>>(), (long long)(bytes / 8), s);
+    launch_stream_copy(a.p, b.p, (long long)bytes, s);  // warm-up
+    hipEvent_t e0, e1;
+    CBS_HIP(hipEventCreate(&e0));
+    CBS_HIP(hipEventCreate(&e1));
+    CBS_HIP(hipEventRecord(e0, s));
+    for (int r = 0; r < reps; ++r) launch_stream_copy(a.p, b.p, (long long)bytes, s);
+    CBS_HIP(hipEventRecord(e1, s));
+    CBS_HIP(hipEventSynchronize(e1));
+    CBS_HIP(hipGetLastError());
+    float t = 0.f;
+    CBS_HIP(hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms = t / reps;
+  });
+}
+
 cbs_status cbs_request_footprint(const cbs_problem_desc* desc, const cbs_solve_opts* opts,
                                  uint64_t* bytes) {
   return guarded([&] {

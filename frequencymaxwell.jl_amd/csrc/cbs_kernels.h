@@ -35,6 +35,8 @@ void launch_pass(const PipeLaunch<T>& L, int k, hipStream_t s);
 
 void launch_born_finalize(RState* const* sts, double* const* parts, const int* ntiles, int nrhs,
                           int parity, hipStream_t s);
+// plain device copy of `bytes` (multiple of 16) bytes
+void launch_stream_copy(const void* src, void* dst, long long bytes, hipStream_t s);
 // compact copy of (done, iters, r2) of n states into out[4*i ..]
 void launch_gather_state(RState* const* sts, int n, double* out, hipStream_t s);
 void launch_sum_parts(double* const* parts, const int* ntiles, int nrhs, int ncol, double* out,

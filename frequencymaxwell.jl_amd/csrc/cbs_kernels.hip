@@ -4,6 +4,8 @@
 //   A  x forward + V x pre-op      B  y forward      C  z + k-space multiply + z inverse
 //   D  y inverse                   E  x inverse + fused epilogue with FP64 partial sums
 // Every launch covers a table of right-hand sides (possibly of different grids).
+#include <algorithm>
+
 #include "cbs_kernels.h"
 #include "cbs_reduce.h"
 
@@ -149,6 +151,21 @@ __global__ void k_gamma_mul(Cx<T>* __restrict__ dst, const Cx<T>* __restrict__ s
   const Cx<T> g = mul_i((T)eps_inv * V[i]);
 #pragma unroll
   for (int c = 0; c < 3; ++c) dst[c * nv + i] = g * src[c * nv + i];
+}
+
+// plain copy (bandwidth ceiling and counter calibration)
+__global__ void __launch_bounds__(256) k_stream_copy(const float4* __restrict__ src,
+                                                     float4* __restrict__ dst, long long n) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    dst[i] = src[i];
+}
+
+void launch_stream_copy(const void* src, void* dst, long long bytes, hipStream_t s) {
+  const long long n = bytes / 16;
+  const unsigned blocks = (unsigned)std::min<long long>((n + 255) / 256, 256 * 64);
+  hipLaunchKernelGGL(k_stream_copy, dim3(blocks), dim3(256), 0, s,
+                     reinterpret_cast<const float4*>(src), reinterpret_cast<float4*>(dst), n);
 }
 
 // ---------------------------------------------------------------------------

@@ -32,7 +32,7 @@ EXPORTED = (
     "cbs_precond_apply", "cbs_curl_apply", "cbs_solve", "cbs_problem_set_source",
     "cbs_problem_run", "cbs_problem_get_fields", "cbs_solve_multi", "cbs_solve_batch",
     "cbs_request_footprint", "cbs_server_start", "cbs_server_stop", "cbs_submit", "cbs_poll",
-    "cbs_wait", "cbs_cancel", "cbs_time_sweeps",
+    "cbs_wait", "cbs_cancel", "cbs_time_sweeps", "cbs_stream_copy",
 )
 
 
@@ -142,6 +142,7 @@ def _declare(lib):
                          C.POINTER(SolveStats)]),
         "cbs_cancel": (s, [_P, C.c_uint64]),
         "cbs_time_sweeps": (s, [_P, C.c_int32, C.c_int32, _D, _D]),
+        "cbs_stream_copy": (s, [_P, C.c_uint64, C.c_int32, _D]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

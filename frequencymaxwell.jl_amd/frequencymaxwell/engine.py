@@ -197,6 +197,14 @@ class CBSProblem:
         return Es, Hs, list(sts)
 
 
+def stream_copy_ms(nbytes: int, reps: int = 20, ctx=None) -> float:
+    """Average duration (ms) of a plain device copy of ``nbytes`` bytes."""
+    ctx = ctx or L.default_context()
+    ms = C.c_double()
+    L.check(L.load().cbs_stream_copy(ctx.handle, int(nbytes), int(reps), C.byref(ms)))
+    return ms.value
+
+
 def solve_batch(problems, S_list, opts: L.SolveOpts, want_h: bool = True, ctx=None):
     """Problems of possibly different grids, one RHS each, swept together."""
     ctx = ctx or problems[0].ctx
