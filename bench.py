@@ -61,6 +61,96 @@ def build_workload(fm, grid=(128, 128, 64), radius=16.0, tol=1e-6, precision="fp
     return med, sol, pot, eps_imag, prob, Sterm
 
 
+SERVE_GRIDS = [(64, 64, 64), (96, 96, 64), (128, 128, 64), (192, 192, 96), (256, 256, 128)]
+
+
+def random_spheres(fm, grid, rng):
+    """1-10 spheres, n uniform in 1.36-1.60, radii 4-16 px, centres in the interior."""
+    eps_bg = 1.33 ** 2
+    ph = np.full(grid, complex(eps_bg), dtype=np.complex128, order="F")
+    axes = [np.arange(1, g + 1, dtype=np.float64) for g in grid]
+    for _ in range(int(rng.integers(1, 11))):
+        r = float(rng.uniform(4.0, min(16.0, min(grid) / 4)))
+        c = [float(rng.uniform(r + 1, g - r)) for g in grid]
+        n = float(rng.uniform(1.36, 1.60))
+        d2 = ((axes[0][:, None, None] - c[0]) ** 2 + (axes[1][None, :, None] - c[1]) ** 2
+              + (axes[2][None, None, :] - c[2]) ** 2)
+        ph[d2 <= r * r] = n * n
+    return fm.Medium(ph, eps_bg)
+
+
+def serve_request(fm, grid, rng, tol):
+    """Padded potential, Green parameters and source term of one served solve:
+    random-sphere phantom, plane wave at a random angle within NA 1.0, s-polarised."""
+    from frequencymaxwell import solver as S
+
+    med = random_spheres(fm, grid, rng)
+    sol = fm.ConvergentBornSolver(resolution=(50e-9,) * 3, grid_size=grid,
+                                  boundary_conditions=boundary(fm), tolerance=tol,
+                                  linear_solver="born", precision="fp32_refine")
+    sin_t = 1.0 / 1.33 * math.sqrt(rng.uniform(0.0, 1.0))
+    phi = rng.uniform(0.0, 2 * math.pi)
+    k = [sin_t * math.cos(phi), sin_t * math.sin(phi), math.sqrt(1 - sin_t ** 2)]
+    pol = [-math.sin(phi), math.cos(phi), 0.0]
+    src = fm.PlaneWaveSource(wavelength=500e-9, polarization=pol, k_vector=k)
+    pot, eps_imag, bornmax, k_bg, spec = S._initialize_solver(sol, med, 500e-9)
+    inc = fm.generate_incident_field(src, sol, permittivity_bg=1.33 ** 2)
+    Sterm = np.asfortranarray(pot[..., None] * inc.E + complex(0, eps_imag) * inc.E)
+    return pot, spec, Sterm
+
+
+def serve_stream(fm, n_req, rate, max_active, seed, maxiters, pool=None):
+    """Poisson arrivals (rate per second) of solves drawn from SERVE_GRIDS with
+    tolerance 1e-4 or 1e-6; returns latency percentiles and throughput."""
+    import threading
+
+    rng = np.random.default_rng(seed)
+    if pool is None:  # two phantoms / angles per grid, prepared before the clock starts
+        pool = {g: [serve_request(fm, g, rng, 1e-6) for _ in range(2)] for g in SERVE_GRIDS}
+    picks = [(SERVE_GRIDS[int(rng.integers(len(SERVE_GRIDS)))], int(rng.integers(2)),
+              1e-4 if rng.random() < 0.5 else 1e-6) for _ in range(n_req)]
+    gaps = rng.exponential(1.0 / rate, n_req)
+    srv = fm.CBSServer(max_active=max_active)
+    tickets = [None] * n_req
+    t0 = time.perf_counter()
+
+    def feed():
+        t = t0
+        for i, (g, j, tol) in enumerate(picks):
+            t += gaps[i]
+            dt = t - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            pot, spec, Sterm = pool[g][j]
+            o = fm.make_opts(method="born", precision="fp32_refine", tol=tol, maxiters=maxiters,
+                             compute_h=False)
+            tickets[i] = srv.submit(pot, spec, Sterm, o)
+
+    th = threading.Thread(target=feed)
+    th.start()
+    lat, vi, conv, worst = [], 0, 0, 0.0
+    try:
+        for i in range(n_req):
+            while tickets[i] is None:
+                time.sleep(0.001)
+            _, _, st = srv.wait(tickets[i])
+            lat.append(st.time_ms)
+            vi += st.voxel_iters
+            conv += int(st.converged)
+            worst = max(worst, st.resid_true_fp64 / picks[i][2])
+    finally:
+        th.join()
+        srv.stop()
+    wall = time.perf_counter() - t0
+    return pool, {
+        "requests": n_req, "max_active": max_active, "arrival_rate_per_s": rate,
+        "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+        "mean_ms": float(np.mean(lat)), "throughput_voxel_iters_per_s": vi / wall,
+        "requests_per_s": n_req / wall, "wall_s": wall, "converged": conv,
+        "max_resid_over_tol": worst,
+    }
+
+
 def cpu_baseline(med, seconds: float):
     """FP64 Born sweeps of the restatement for about `seconds` seconds."""
     from oracle import cbs_oracle as o
@@ -105,6 +195,22 @@ def read_traffic(summary: str, kernel: str):
     return None
 
 
+def serving_leg(fm, args):
+    """The mixed-size request stream, continuously batched and one at a time."""
+    print("[bench] serving stream ...", file=sys.stderr, flush=True)
+    pool, cb = serve_stream(fm, args.serve_requests, args.serve_rate, args.serve_active, 7,
+                            args.maxiters)
+    print(f"[bench] continuous batching: p50 {cb['p50_ms']:.0f} ms p99 {cb['p99_ms']:.0f} ms",
+          file=sys.stderr, flush=True)
+    _, seq = serve_stream(fm, args.serve_requests, args.serve_rate, 1, 7, args.maxiters, pool)
+    print(f"[bench] one at a time: p50 {seq['p50_ms']:.0f} ms p99 {seq['p99_ms']:.0f} ms",
+          file=sys.stderr, flush=True)
+    return {"workload": "Poisson arrivals, grids " + ", ".join("x".join(map(str, g)) for g in SERVE_GRIDS)
+                        + " (absorbing 16 px padding each side), 1-10 spheres n 1.36-1.60, "
+                          "plane wave within NA 1.0, tol 1e-4 or 1e-6, Born FP32+refine",
+            "continuous_batching": cb, "one_at_a_time": seq}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -124,6 +230,11 @@ def main():
     ap.add_argument("--profile", action="store_true",
                     help="profiler run: only the timed FP32 sweeps and the 1 GiB calibration copy")
     ap.add_argument("--profile-reps", type=int, default=20)
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving-stream leg")
+    ap.add_argument("--serve-requests", type=int, default=24)
+    ap.add_argument("--serve-rate", type=float, default=1.5, help="arrivals per second")
+    ap.add_argument("--serve-active", type=int, default=8)
+    ap.add_argument("--serve-only", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build measures the single-GPU path only "
@@ -132,6 +243,10 @@ def main():
 
     import frequencymaxwell as fm
     from frequencymaxwell.engine import stream_copy_ms
+
+    if args.serve_only:
+        print(json.dumps({"serving": serving_leg(fm, args)}), flush=True)
+        return
 
     med, sol, pot, eps_imag, prob, Sterm = build_workload(fm)
     nvox = int(np.prod(pot.shape))
@@ -212,6 +327,9 @@ def main():
                                        "fp64_iters": stk.iters_fp64, "resid_fp64": stk.resid_true_fp64,
                                        "converged": bool(stk.converged),
                                        "voxel_iters_per_s": stk.voxel_iters / dtk}
+
+    if not args.no_serve:
+        out["serving"] = serving_leg(fm, args)
 
     if args.no_roofline:
         print(json.dumps(out), flush=True)

@@ -127,7 +127,7 @@ cbs_status cbs_ctx_create(cbs_ctx** out, int32_t device, uint64_t hbm_budget_byt
 cbs_status cbs_ctx_destroy(cbs_ctx* ctx) {
   return guarded([&] {
     if (!ctx) return;
-    server_stop(ctx->impl);
+    server_destroy(ctx->impl);
     set_device(ctx->impl);
     CBS_HIP(hipStreamSynchronize(ctx->impl.stream));
     delete ctx;
@@ -142,23 +142,8 @@ cbs_status cbs_problem_create(cbs_ctx* ctx, const cbs_problem_desc* desc, const 
     validate_desc(*desc);
     set_device(ctx->impl);
     std::unique_ptr<cbs_problem> pp(new cbs_problem());
-    Problem& p = pp->impl;
-    p.ctx = &ctx->impl;
-    p.d = *desc;
-    for (int k = 0; k < 3; ++k) p.n[k] = (int)desc->n[k];
-    p.nvox = (long long)desc->n[0] * desc->n[1] * desc->n[2];
-    p.S = desc->averaging ? 2 : 1;
-    p.has_ramp = desc->shift[0] != 0.0 || desc->shift[1] != 0.0 || desc->shift[2] != 0.0;
-    p.eps_inv = 1.0 / desc->eps_imag;
-    check_finite(V, (size_t)p.nvox * 2, "potential");
-    p.V64.alloc(p.ctx, (size_t)p.nvox * sizeof(Cx<double>));
-    CBS_HIP(hipMemcpy(p.V64.p, V, (size_t)p.nvox * sizeof(Cx<double>), hipMemcpyHostToDevice));
-    fill_tables<double>(p);
-    if (use32(p)) {
-      fill_tables<float>(p);
-      ensure_v32(p, p.ctx->stream);
-      CBS_HIP(hipStreamSynchronize(p.ctx->stream));
-    }
+    problem_init(pp->impl, &ctx->impl, *desc, V, desc->precision != CBS_FP64,
+                 ctx->impl.stream);
     *out = pp.release();
   });
 }
@@ -500,11 +485,10 @@ cbs_status cbs_poll(cbs_ctx* ctx, uint64_t ticket, int32_t* done, cbs_solve_stat
   });
 }
 
-cbs_status cbs_wait(cbs_ctx* ctx, uint64_t ticket, double* E, double* H, double h_re, double h_im,
-                    cbs_solve_stats* stats) {
+cbs_status cbs_wait(cbs_ctx* ctx, uint64_t ticket, double* E, double* H, cbs_solve_stats* stats) {
   return guarded([&] {
     CBS_REQUIRE(ctx, "NULL argument");
-    server_wait(ctx->impl, ticket, E, H, h_re, h_im, stats);
+    server_wait(ctx->impl, ticket, E, H, stats);
   });
 }
 

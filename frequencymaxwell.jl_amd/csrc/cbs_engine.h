@@ -168,6 +168,10 @@ template <class T>
 void curl_apply_dev(Problem& p, const Cx<T>* E, Cx<T>* H, double sr, double si, hipStream_t s);
 
 void validate_desc(const cbs_problem_desc& d);
+// set up problem p of context c from a host potential V (Nx*Ny*Nz complex):
+// FP64 tables always, FP32 tables and V32 when want32
+void problem_init(Problem& p, Ctx* c, const cbs_problem_desc& d, const double* V, bool want32,
+                  hipStream_t s);
 uint64_t footprint_bytes(const cbs_problem_desc& d, const cbs_solve_opts& o);
 
 }  // namespace cbs

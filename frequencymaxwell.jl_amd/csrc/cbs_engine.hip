@@ -195,42 +195,74 @@ template <class T>
 void Pipe<T>::build(Ctx* c, hipStream_t s) {
   const int np = (int)h.size();
   CBS_REQUIRE(np > 0, "empty pipeline");
-  std::vector<int> bs(4 * (np + 1), 0);
   L = PipeLaunch<T>{};
   L.S = h[0].S;
+  for (int i = 0; i < np; ++i)
+    CBS_REQUIRE(h[i].S == L.S, "pipeline mixes averaging and non-averaging problems");
+  struct Grp {
+    int pass, off_p, off_b, n, blocks, len;
+    size_t lds;
+  };
+  std::vector<Grp> grps;
+  std::vector<DevProb<T>> hp;
+  std::vector<int> hb;
+  hp.reserve(4 * (size_t)np);
+  const int axis_of[4] = {0, 1, 2, 0};
   for (int pass = 0; pass < 4; ++pass) {
-    int acc = 0;
-    size_t lds = 0;
+    const int ax = axis_of[pass];
+    // group key: the axis length where a specialised kernel exists (>= 32,
+    // compile-time plan), else 0 (per-block switch kernel)
+    std::vector<int> key(np), keys;
     for (int i = 0; i < np; ++i) {
-      CBS_REQUIRE(h[i].S == L.S, "pipeline mixes averaging and non-averaging problems");
-      bs[pass * (np + 1) + i] = acc;
-      acc += h[i].tiles[pass];
-      lds = std::max(lds, (size_t)tile_lds_bytes(h[i], pass));
+      const int u = h[i].rt[ax] ? 0 : h[i].n[ax];
+      key[i] = u >= 32 ? u : 0;
     }
-    bs[pass * (np + 1) + np] = acc;
-    L.blocks[pass] = acc;
-    // the xinv epilogue reuses the tile LDS for its block reduction
-    L.lds[pass] = std::max(lds, (size_t)((kThreads / 64) * kNumPart * sizeof(double)));
+    for (int i = 0; i < np; ++i) {
+      if (std::find(keys.begin(), keys.end(), key[i]) != keys.end()) continue;
+      if (key[i] != 0 && (int)keys.size() >= kMaxGroups - 1) {
+        key[i] = 0;  // too many lengths: the rest share the switch kernel
+        if (std::find(keys.begin(), keys.end(), 0) != keys.end()) continue;
+      }
+      keys.push_back(key[i]);
+    }
+    for (int i = 0; i < np; ++i)
+      if (std::find(keys.begin(), keys.end(), key[i]) == keys.end()) key[i] = 0;
+    for (int k : keys) {
+      Grp g{pass, (int)hp.size(), (int)hb.size(), 0, 0, k, 0};
+      for (int i = 0; i < np; ++i) {
+        if (key[i] != k) continue;
+        hp.push_back(h[i]);
+        hb.push_back(g.blocks);
+        g.blocks += h[i].tiles[pass];
+        g.lds = std::max(g.lds, (size_t)tile_lds_bytes(h[i], pass));
+        g.n++;
+      }
+      if (g.n == 0) continue;
+      hb.push_back(g.blocks);
+      // the xinv epilogue reuses the tile LDS for its block reduction
+      g.lds = std::max(g.lds, (size_t)((kThreads / 64) * kNumPart * sizeof(double)));
+      if (g.lds > 160 * 1024)
+        throw Error(CBS_EINVAL, "grid axis too long for an LDS tile (" + std::to_string(g.lds) +
+                                    " bytes)");
+      grps.push_back(g);
+    }
   }
-  for (int ax = 0; ax < 3; ++ax) {
-    int u = h[0].rt[ax] ? 0 : h[0].n[ax];
-    for (int i = 1; i < np; ++i)
-      if (h[i].rt[ax] || h[i].n[ax] != u) u = 0;
-    // specialised kernels exist for lengths >= 32; shorter ones use the switch kernel
-    L.uni[ax] = u >= 32 ? u : 0;
-  }
-  for (int pass = 0; pass < 4; ++pass)
-    if (L.lds[pass] > 160 * 1024)
-      throw Error(CBS_EINVAL, "grid axis too long for an LDS tile (" +
-                                  std::to_string(L.lds[pass]) + " bytes)");
-  d_probs.ensure(c, np * sizeof(DevProb<T>));
-  d_bstart.ensure(c, bs.size() * sizeof(int));
-  CBS_HIP(hipMemcpyAsync(d_probs.p, h.data(), np * sizeof(DevProb<T>), hipMemcpyHostToDevice, s));
-  CBS_HIP(hipMemcpyAsync(d_bstart.p, bs.data(), bs.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  d_probs.ensure(c, hp.size() * sizeof(DevProb<T>));
+  d_bstart.ensure(c, hb.size() * sizeof(int));
+  CBS_HIP(hipMemcpyAsync(d_probs.p, hp.data(), hp.size() * sizeof(DevProb<T>),
+                         hipMemcpyHostToDevice, s));
+  CBS_HIP(hipMemcpyAsync(d_bstart.p, hb.data(), hb.size() * sizeof(int), hipMemcpyHostToDevice,
+                         s));
   CBS_HIP(hipStreamSynchronize(s));
-  L.probs = d_probs.as<DevProb<T>>();
-  L.bstart = d_bstart.as<int>();
-  L.nprob = np;
+  for (const Grp& g : grps) {
+    PassLaunch<T>& G = L.g[g.pass][L.ng[g.pass]++];
+    G.probs = d_probs.as<DevProb<T>>() + g.off_p;
+    G.bstart = d_bstart.as<int>() + g.off_b;
+    G.nprob = g.n;
+    G.blocks = g.blocks;
+    G.lds = g.lds;
+    G.len = g.len;
+  }
 }
 template struct Pipe<double>;
 template struct Pipe<float>;
@@ -358,23 +390,53 @@ void validate_desc(const cbs_problem_desc& d) {
   CBS_REQUIRE(d.averaging == 0 || d.averaging == 1, "averaging must be 0 or 1");
 }
 
+void problem_init(Problem& p, Ctx* c, const cbs_problem_desc& d, const double* V, bool want32,
+                  hipStream_t s) {
+  p.ctx = c;
+  p.d = d;
+  for (int k = 0; k < 3; ++k) p.n[k] = (int)d.n[k];
+  p.nvox = (long long)d.n[0] * d.n[1] * d.n[2];
+  p.S = d.averaging ? 2 : 1;
+  p.has_ramp = d.shift[0] != 0.0 || d.shift[1] != 0.0 || d.shift[2] != 0.0;
+  p.eps_inv = 1.0 / d.eps_imag;
+  for (long long i = 0; i < 2 * p.nvox; ++i)
+    if (!std::isfinite(V[i])) throw Error(CBS_ENUMERIC, "potential holds NaN or Inf");
+  p.V64.alloc(c, (size_t)p.nvox * sizeof(Cx<double>));
+  CBS_HIP(hipMemcpy(p.V64.p, V, (size_t)p.nvox * sizeof(Cx<double>), hipMemcpyHostToDevice));
+  fill_tables<double>(p);
+  if (want32) {
+    fill_tables<float>(p);
+    if (!p.V32.p) {
+      p.V32.alloc(c, (size_t)p.nvox * sizeof(Cx<float>));
+      launch_convert<float, double>(p.V32.as<Cx<float>>(), p.V64.as<Cx<double>>(), p.nvox, s);
+    }
+    CBS_HIP(hipStreamSynchronize(s));
+  }
+}
+
 uint64_t footprint_bytes(const cbs_problem_desc& d, const cbs_solve_opts& o) {
   const uint64_t nv = (uint64_t)d.n[0] * d.n[1] * d.n[2];
   const uint64_t S = d.averaging ? 2 : 1;
+  long long ws64[3], ws32[3];
+  w_strides((int)d.n[0], (int)d.n[1], (int)d.n[2], 16, ws64);
+  w_strides((int)d.n[0], (int)d.n[1], (int)d.n[2], 8, ws32);
   const uint64_t f64 = nv * 3 * 16, f32 = nv * 3 * 8;
+  const uint64_t W64 = S * 3 * (uint64_t)ws64[2] * 16, W32 = S * 3 * (uint64_t)ws32[2] * 8;
   uint64_t b = nv * 16 /*V64*/ + f64 /*S*/ + f64 /*E*/;
   if (o.precision == CBS_FP64) {
-    b += f64 * (3 + S);
+    b += 3 * f64 + W64;  // X[2], B, W
   } else {
-    b += nv * 8 /*V32*/ + f32 * (3 + S) + f32;
-    if (o.precision == CBS_FP32_REFINE) b += f64 * (3 + S) + f64;
+    b += nv * 8 /*V32*/ + 3 * f32 + W32;
+    if (o.precision == CBS_FP32_REFINE) b += 4 * f64 + W64;  // + R
   }
-  if (o.method == CBS_BICGSTAB) b += (o.precision == CBS_FP64 ? f64 : f32) * 6;
+  const uint64_t fk = o.precision == CBS_FP64 ? f64 : f32;
+  if (o.method == CBS_BICGSTAB) b += fk * 6;
   if (o.method == CBS_GMRES) {
     const uint64_t m = o.gmres_restart > 0 ? (uint64_t)o.gmres_restart : (uint64_t)o.maxiters;
-    b += (o.precision == CBS_FP64 ? f64 : f32) * (m + 3);
+    b += fk * (std::min<uint64_t>(m, 400) + 3);
   }
-  return b + (4u << 20);
+  if (o.compute_h) b += f64;  // curl output
+  return b + (4u << 20);      // tables, partial sums, solver state
 }
 
 }  // namespace cbs

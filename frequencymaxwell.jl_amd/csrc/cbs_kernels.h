@@ -5,16 +5,28 @@
 
 namespace cbs {
 
-// One five-sweep pipeline launch over a table of right-hand sides.
+// One launch of one sweep over a group of right-hand sides that share the
+// sweep's axis length (specialised kernel), or a mixed group (per-block switch).
+template <class T>
+struct PassLaunch {
+  const DevProb<T>* probs;  // device table of the group
+  const int* bstart;        // device [nprob + 1] block offsets
+  int nprob;
+  int blocks;
+  size_t lds;               // dynamic LDS bytes
+  int len;                  // common axis length (0: mixed lengths, short or runtime plans)
+};
+
+constexpr int kMaxGroups = 8;
+
+// One five-sweep pipeline over a table of right-hand sides: each sweep is one
+// launch per group of equal axis length, so a mix of grids runs the same
+// specialised kernels (registers, LDS) as each grid alone.
 template <class T>
 struct PipeLaunch {
-  const DevProb<T>* probs;  // device table [nprob]
-  const int* bstart;        // device [4][nprob + 1] block offsets per pass
-  int nprob;
-  int S;                    // pipelines per RHS (uniform over the table)
-  int blocks[4];            // xfwd, y, z, xinv
-  size_t lds[4];            // dynamic LDS bytes per pass
-  int uni[3];               // common x / y / z length of the table (0: mixed or runtime plan)
+  int S;                            // pipelines per RHS (uniform over the table)
+  int ng[4];                        // groups of the xfwd, y, z and xinv passes
+  PassLaunch<T> g[4][kMaxGroups];
 };
 
 // per-sweep launchers (cbs_pass_x.hip, cbs_pass_yz.hip)

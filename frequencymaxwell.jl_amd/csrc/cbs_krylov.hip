@@ -196,127 +196,181 @@ k_bicg_fin(RState* const* __restrict__ sts, double* const* __restrict__ parts,
 }
 
 template <class T>
-void bicgstab_solve(Ctx* ctx, std::vector<SolveItem<T>>& it, const IterOpts& io, hipStream_t s,
-                    std::vector<RState>& states, std::vector<const Cx<T>*>& result) {
-  const int n = (int)it.size();
-  StateSet gn, fin_mv, fin_vec;
-  long long maxchunk = 1;
+static BiVec<T> make_bivec(SolveItem<T>& x, int use_v) {
+  BiVec<T> q;
+  q.x = x.w->X[0].template as<Cx<T>>();
+  q.r = x.w->kvec(0);
+  q.rh = x.w->kvec(1);
+  q.p = x.w->kvec(2);
+  q.v = x.w->kvec(3);
+  q.s = x.w->kvec(4);
+  q.t = x.w->kvec(5);
+  q.V = potential<T>(*x.p);
+  q.b = x.b;
+  q.nv = x.p->nvox;
+  q.eps_inv = x.p->eps_inv;
+  q.part = x.w->part.template as<double>();
+  q.st = x.w->st.template as<RState>();
+  q.use_v = use_v;
+  return q;
+}
+
+static void bicg_fin(StateSet& f, int n, int mode, int init_iters, hipStream_t s) {
+  hipLaunchKernelGGL(k_bicg_fin, dim3(n), dim3(kThreads), 0, s, f.sts.as<RState*>(),
+                     f.parts.as<double*>(), f.ntiles.as<int>(), mode, init_iters);
+}
+
+// x0, r0 = gamma (b - A x0), r^ = r0 and the first state of every new member
+template <class T>
+void BicgRunner<T>::add(const std::vector<long long>& keys, std::vector<SolveItem<T>>& xs,
+                        const IterOpts& io) {
+  const int n = (int)xs.size();
+  if (n == 0) return;
+  hipStream_t s = this->s;
+  Ctx* ctx = this->ctx;
+  StateSet gn, fv;
+  long long mc = 1;
   std::vector<BiVec<T>> hv(n);
+  std::vector<RState*> sts;
+  std::vector<double*> parts;
+  std::vector<int> nt;
+  Pipe<T> m0;
   for (int i = 0; i < n; ++i) {
-    SolveItem<T>& x = it[i];
+    SolveItem<T>& x = xs[i];
     x.w->ensure_k(ctx, 6);
-    BiVec<T>& q = hv[i];
-    q.x = x.w->X[0].template as<Cx<T>>();
-    q.r = x.w->kvec(0);
-    q.rh = x.w->kvec(1);
-    q.p = x.w->kvec(2);
-    q.v = x.w->kvec(3);
-    q.s = x.w->kvec(4);
-    q.t = x.w->kvec(5);
-    q.V = potential<T>(*x.p);
-    q.b = x.b;
-    q.nv = x.p->nvox;
-    q.eps_inv = x.p->eps_inv;
-    q.part = x.w->part.template as<double>();
-    q.st = x.w->st.template as<RState>();
-    q.use_v = io.x0_zero ? 0 : 1;
-    maxchunk = std::max(maxchunk, (q.nv + kChunk - 1) / kChunk);
+    BiVec<T>& q = hv[i] = make_bivec<T>(x, io.x0_zero ? 0 : 1);
+    mc = std::max(mc, (q.nv + kChunk - 1) / kChunk);
     if (io.x0_zero) launch_fill<T>(q.x, 3 * q.nv, s);
     else copy_dev<T>(q.x, x.b, 3 * q.nv, s);
+    // v = A^ x0 (consumed by the r0 kernel)
+    m0.h.push_back(make_desc<T>(*x.p, PRE_V, ZOP_GREEN, POST_PMATVEC, q.x, nullptr, q.v,
+                                x.w->W.template as<Cx<T>>(), q.part, q.st));
+    sts.push_back(q.st);
+    parts.push_back(q.part);
+    nt.push_back((int)((q.nv + kChunk - 1) / kChunk));
   }
-  init_states<T>(ctx, it, io.tol, io.abstol, io.rule, io.maxit, s, gn);
-  DevBuf dv;
-  dv.alloc(ctx, n * sizeof(BiVec<T>));
-  CBS_HIP(hipMemcpyAsync(dv.p, hv.data(), n * sizeof(BiVec<T>), hipMemcpyHostToDevice, s));
-  const BiVec<T>* d_vs = dv.as<BiVec<T>>();
+  init_states<T>(ctx, xs, io.tol, io.abstol, io.rule, io.maxit, s, gn);
+  DevBuf dvi;
+  dvi.alloc(ctx, n * sizeof(BiVec<T>));
+  CBS_HIP(hipMemcpyAsync(dvi.p, hv.data(), n * sizeof(BiVec<T>), hipMemcpyHostToDevice, s));
+  fv.build(ctx, sts, parts, nt, s);
+  if (!io.x0_zero) {
+    m0.build(ctx, s);
+    m0.launch(s);
+  }
+  hipLaunchKernelGGL((k_bicg_vec<T, BI_INIT>), dim3((unsigned)mc, (unsigned)n), dim3(kThreads), 0,
+                     s, dvi.as<BiVec<T>>());
+  bicg_fin(fv, n, BF_INIT, io.x0_zero ? 0 : 1, s);
+  CBS_HIP(hipGetLastError());
+  CBS_HIP(hipStreamSynchronize(s));  // dvi, fv and m0 go out of scope
+  for (int i = 0; i < n; ++i)
+    this->items.push_back({keys[i], xs[i], 0, (long long)io.maxit / 2 + this->k + 2});
+  this->dirty = true;
+  this->hg.clear();
+}
 
-  // matvec pipelines: v = A^ x0 (init), v = A^ p (with <r^, v>), t = A^ s (with <t,s>, |t|^2)
-  Pipe<T> m0, m1, m2;
+template <class T>
+void BicgRunner<T>::rebuild() {
+  this->drop_graph();
+  const int n = (int)this->items.size();
+  hipStream_t s = this->s;
+  Ctx* ctx = this->ctx;
+  std::vector<BiVec<T>> hv(n);
   std::vector<RState*> sts;
   std::vector<double*> parts;
   std::vector<int> nt_mv, nt_vec;
+  m1.h.clear();
+  m2.h.clear();
+  maxchunk = 1;
   for (int i = 0; i < n; ++i) {
-    Problem& p = *it[i].p;
-    Cx<T>* W = it[i].w->W.template as<Cx<T>>();
-    double* part = hv[i].part;
-    RState* st = hv[i].st;
-    m0.h.push_back(make_desc<T>(p, PRE_V, ZOP_GREEN, POST_PMATVEC, hv[i].x, nullptr, hv[i].v, W,
-                                part, st));
-    m1.h.push_back(make_desc<T>(p, PRE_V, ZOP_GREEN, POST_PMATVEC, hv[i].p, hv[i].rh, hv[i].v,
-                                W, part, st));
-    m2.h.push_back(make_desc<T>(p, PRE_V, ZOP_GREEN, POST_PMATVEC, hv[i].s, nullptr, hv[i].t, W,
-                                part, st));
-    sts.push_back(st);
-    parts.push_back(part);
+    SolveItem<T>& x = this->items[i].it;
+    BiVec<T>& q = hv[i] = make_bivec<T>(x, 1);
+    Cx<T>* W = x.w->W.template as<Cx<T>>();
+    // v = A^ p (with <r^, v>), t = A^ s (with <t, s>, |t|^2)
+    m1.h.push_back(make_desc<T>(*x.p, PRE_V, ZOP_GREEN, POST_PMATVEC, q.p, q.rh, q.v, W, q.part,
+                                q.st));
+    m2.h.push_back(make_desc<T>(*x.p, PRE_V, ZOP_GREEN, POST_PMATVEC, q.s, nullptr, q.t, W,
+                                q.part, q.st));
+    sts.push_back(q.st);
+    parts.push_back(q.part);
     nt_mv.push_back(m1.h.back().tiles[3]);
-    nt_vec.push_back((int)((hv[i].nv + kChunk - 1) / kChunk));
+    nt_vec.push_back((int)((q.nv + kChunk - 1) / kChunk));
+    maxchunk = std::max(maxchunk, (q.nv + kChunk - 1) / kChunk);
   }
-  m0.build(ctx, s);
+  dv.ensure(ctx, n * sizeof(BiVec<T>));
+  CBS_HIP(hipMemcpyAsync(dv.p, hv.data(), n * sizeof(BiVec<T>), hipMemcpyHostToDevice, s));
   m1.build(ctx, s);
   m2.build(ctx, s);
   fin_mv.build(ctx, sts, parts, nt_mv, s);
   fin_vec.build(ctx, sts, parts, nt_vec, s);
-  const dim3 vgrid((unsigned)maxchunk, (unsigned)n);
-  auto fin = [&](StateSet& f, int mode, int init_iters) {
-    hipLaunchKernelGGL(k_bicg_fin, dim3(n), dim3(kThreads), 0, s, f.sts.as<RState*>(),
-                       f.parts.as<double*>(), f.ntiles.as<int>(), mode, init_iters);
-  };
-  // r0
-  if (!io.x0_zero) m0.launch(s);
-  hipLaunchKernelGGL((k_bicg_vec<T, BI_INIT>), vgrid, dim3(kThreads), 0, s, d_vs);
-  fin(fin_vec, BF_INIT, io.x0_zero ? 0 : 1);
-  CBS_HIP(hipGetLastError());
-
-  auto iteration = [&]() {
-    hipLaunchKernelGGL((k_bicg_vec<T, BI_P>), vgrid, dim3(kThreads), 0, s, d_vs);
-    m1.launch(s);
-    fin(fin_mv, BF_ALPHA, 0);
-    hipLaunchKernelGGL((k_bicg_vec<T, BI_S>), vgrid, dim3(kThreads), 0, s, d_vs);
-    fin(fin_vec, BF_S, 0);
-    m2.launch(s);
-    fin(fin_mv, BF_OMEGA, 0);
-    hipLaunchKernelGGL((k_bicg_vec<T, BI_XR>), vgrid, dim3(kThreads), 0, s, d_vs);
-    fin(fin_vec, BF_R, 0);
-  };
-  const int chunk = std::max(1, io.check_every);
-  hipGraphExec_t gexec = nullptr;
-  hipGraph_t graph = nullptr;
-  if (io.use_graph) {
+  this->gath.ensure(ctx, (size_t)n * 4 * sizeof(double));
+  if (this->use_graph) {
     CBS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    for (int k = 0; k < chunk; ++k) iteration();
-    CBS_HIP(hipStreamEndCapture(s, &graph));
-    CBS_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+    for (int j = 0; j < this->k; ++j) iteration();
+    CBS_HIP(hipStreamEndCapture(s, &this->graph));
+    CBS_HIP(hipGraphInstantiate(&this->gexec, this->graph, nullptr, nullptr, 0));
   }
-  DevBuf gath;
-  gath.alloc(ctx, (size_t)n * 4 * sizeof(double));
-  std::vector<double> hg((size_t)n * 4);
-  const long long cap = (long long)io.maxit / 2 + chunk + 2;
-  long long launched = 0;
-  try {
-    while (true) {
-      if (gexec) CBS_HIP(hipGraphLaunch(gexec, s));
-      else
-        for (int k = 0; k < chunk; ++k) iteration();
-      CBS_HIP(hipGetLastError());
-      launched += chunk;
-      launch_gather_state(fin_vec.sts.as<RState*>(), n, gath.as<double>(), s);
-      CBS_HIP(hipMemcpyAsync(hg.data(), gath.p, hg.size() * sizeof(double),
-                             hipMemcpyDeviceToHost, s));
-      CBS_HIP(hipStreamSynchronize(s));
-      bool all = true;
-      for (int i = 0; i < n; ++i) all = all && hg[4 * i] != 0.0;
-      if (all || launched >= cap) break;
-    }
-  } catch (...) {
-    if (gexec) (void)hipGraphExecDestroy(gexec);
-    if (graph) (void)hipGraphDestroy(graph);
-    throw;
+  this->dirty = false;
+}
+
+template <class T>
+void BicgRunner<T>::iteration() {
+  const int n = (int)this->items.size();
+  hipStream_t s = this->s;
+  const dim3 vgrid((unsigned)maxchunk, (unsigned)n);
+  const BiVec<T>* d_vs = dv.as<BiVec<T>>();
+  hipLaunchKernelGGL((k_bicg_vec<T, BI_P>), vgrid, dim3(kThreads), 0, s, d_vs);
+  m1.launch(s);
+  bicg_fin(fin_mv, n, BF_ALPHA, 0, s);
+  hipLaunchKernelGGL((k_bicg_vec<T, BI_S>), vgrid, dim3(kThreads), 0, s, d_vs);
+  bicg_fin(fin_vec, n, BF_S, 0, s);
+  m2.launch(s);
+  bicg_fin(fin_mv, n, BF_OMEGA, 0, s);
+  hipLaunchKernelGGL((k_bicg_vec<T, BI_XR>), vgrid, dim3(kThreads), 0, s, d_vs);
+  bicg_fin(fin_vec, n, BF_R, 0, s);
+}
+
+template <class T>
+void BicgRunner<T>::chunk() {
+  const int n = (int)this->items.size();
+  if (n == 0) return;
+  if (this->dirty) rebuild();
+  hipStream_t s = this->s;
+  if (this->gexec) CBS_HIP(hipGraphLaunch(this->gexec, s));
+  else
+    for (int j = 0; j < this->k; ++j) iteration();
+  CBS_HIP(hipGetLastError());
+  for (auto& m : this->items) m.launched += this->k;
+  this->hg.resize((size_t)n * 4);
+  launch_gather_state(fin_vec.sts.as<RState*>(), n, this->gath.template as<double>(), s);
+  CBS_HIP(hipMemcpyAsync(this->hg.data(), this->gath.p, this->hg.size() * sizeof(double),
+                         hipMemcpyDeviceToHost, s));
+  CBS_HIP(hipStreamSynchronize(s));
+}
+
+template <class T>
+void bicgstab_solve(Ctx* ctx, std::vector<SolveItem<T>>& it, const IterOpts& io, hipStream_t s,
+                    std::vector<RState>& states, std::vector<const Cx<T>*>& result) {
+  const int n = (int)it.size();
+  BicgRunner<T> R;
+  R.ctx = ctx;
+  R.s = s;
+  R.k = runner_chunk(CBS_BICGSTAB, io.check_every);
+  R.use_graph = io.use_graph != 0;
+  std::vector<long long> keys(n);
+  for (int i = 0; i < n; ++i) keys[i] = i;
+  R.add(keys, it, io);
+  std::vector<Finished> done;
+  while (!R.empty()) {
+    R.chunk();
+    R.collect(done);
   }
-  if (gexec) (void)hipGraphExecDestroy(gexec);
-  if (graph) (void)hipGraphDestroy(graph);
-  read_states<T>(it, states, s);
+  states.resize(n);
   result.resize(n);
-  for (int i = 0; i < n; ++i) result[i] = hv[i].x;
+  for (auto& f : done) {
+    states[f.key] = f.st;
+    result[f.key] = reinterpret_cast<const Cx<T>*>(f.x);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -807,5 +861,7 @@ void gmres_solve(Ctx* ctx, std::vector<SolveItem<T>>& it, const IterOpts& io, hi
                                std::vector<RState>&, std::vector<const Cx<T>*>&);
 CBS_KRYLOV_INST(double)
 CBS_KRYLOV_INST(float)
+template struct BicgRunner<double>;
+template struct BicgRunner<float>;
 
 }  // namespace cbs

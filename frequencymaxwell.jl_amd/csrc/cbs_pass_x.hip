@@ -60,54 +60,58 @@ k_xinv(const DevProb<T>* __restrict__ probs, const int* __restrict__ bstart, int
 }
 
 template <class T, int S, int NC>
-static void go_xfwd(const PipeLaunch<T>& L, hipStream_t s) {
-  hipLaunchKernelGGL((k_xfwd<T, S, NC>), dim3(L.blocks[0]), dim3(kThreads), L.lds[0], s, L.probs,
-                     L.bstart, L.nprob);
+static void go_xfwd(const PassLaunch<T>& G, hipStream_t s) {
+  hipLaunchKernelGGL((k_xfwd<T, S, NC>), dim3(G.blocks), dim3(kThreads), G.lds, s, G.probs,
+                     G.bstart, G.nprob);
 }
 template <class T, int S, int NC>
-static void go_xinv(const PipeLaunch<T>& L, hipStream_t s) {
-  hipLaunchKernelGGL((k_xinv<T, S, NC>), dim3(L.blocks[3]), dim3(kThreads), L.lds[3], s, L.probs,
-                     L.bstart + 3 * (L.nprob + 1), L.nprob);
+static void go_xinv(const PassLaunch<T>& G, hipStream_t s) {
+  hipLaunchKernelGGL((k_xinv<T, S, NC>), dim3(G.blocks), dim3(kThreads), G.lds, s, G.probs,
+                     G.bstart, G.nprob);
 }
 
 template <class T, int S>
-static void xfwd_len(const PipeLaunch<T>& L, hipStream_t s) {
-  switch (L.uni[0]) {
-    case 32: go_xfwd<T, S, 32>(L, s); break;
-    case 64: go_xfwd<T, S, 64>(L, s); break;
-    case 96: go_xfwd<T, S, 96>(L, s); break;
-    case 128: go_xfwd<T, S, 128>(L, s); break;
-    case 160: go_xfwd<T, S, 160>(L, s); break;
-    case 192: go_xfwd<T, S, 192>(L, s); break;
-    case 256: go_xfwd<T, S, 256>(L, s); break;
-    case 288: go_xfwd<T, S, 288>(L, s); break;
-    default: go_xfwd<T, S, 0>(L, s); break;
+static void xfwd_len(const PassLaunch<T>& G, hipStream_t s) {
+  switch (G.len) {
+    case 32: go_xfwd<T, S, 32>(G, s); break;
+    case 64: go_xfwd<T, S, 64>(G, s); break;
+    case 96: go_xfwd<T, S, 96>(G, s); break;
+    case 128: go_xfwd<T, S, 128>(G, s); break;
+    case 160: go_xfwd<T, S, 160>(G, s); break;
+    case 192: go_xfwd<T, S, 192>(G, s); break;
+    case 256: go_xfwd<T, S, 256>(G, s); break;
+    case 288: go_xfwd<T, S, 288>(G, s); break;
+    default: go_xfwd<T, S, 0>(G, s); break;
   }
 }
 template <class T, int S>
-static void xinv_len(const PipeLaunch<T>& L, hipStream_t s) {
-  switch (L.uni[0]) {
-    case 32: go_xinv<T, S, 32>(L, s); break;
-    case 64: go_xinv<T, S, 64>(L, s); break;
-    case 96: go_xinv<T, S, 96>(L, s); break;
-    case 128: go_xinv<T, S, 128>(L, s); break;
-    case 160: go_xinv<T, S, 160>(L, s); break;
-    case 192: go_xinv<T, S, 192>(L, s); break;
-    case 256: go_xinv<T, S, 256>(L, s); break;
-    case 288: go_xinv<T, S, 288>(L, s); break;
-    default: go_xinv<T, S, 0>(L, s); break;
+static void xinv_len(const PassLaunch<T>& G, hipStream_t s) {
+  switch (G.len) {
+    case 32: go_xinv<T, S, 32>(G, s); break;
+    case 64: go_xinv<T, S, 64>(G, s); break;
+    case 96: go_xinv<T, S, 96>(G, s); break;
+    case 128: go_xinv<T, S, 128>(G, s); break;
+    case 160: go_xinv<T, S, 160>(G, s); break;
+    case 192: go_xinv<T, S, 192>(G, s); break;
+    case 256: go_xinv<T, S, 256>(G, s); break;
+    case 288: go_xinv<T, S, 288>(G, s); break;
+    default: go_xinv<T, S, 0>(G, s); break;
   }
 }
 
 template <class T>
 void launch_xfwd(const PipeLaunch<T>& L, hipStream_t s) {
-  if (L.S == 2) xfwd_len<T, 2>(L, s);
-  else xfwd_len<T, 1>(L, s);
+  for (int i = 0; i < L.ng[0]; ++i) {
+    if (L.S == 2) xfwd_len<T, 2>(L.g[0][i], s);
+    else xfwd_len<T, 1>(L.g[0][i], s);
+  }
 }
 template <class T>
 void launch_xinv(const PipeLaunch<T>& L, hipStream_t s) {
-  if (L.S == 2) xinv_len<T, 2>(L, s);
-  else xinv_len<T, 1>(L, s);
+  for (int i = 0; i < L.ng[3]; ++i) {
+    if (L.S == 2) xinv_len<T, 2>(L.g[3][i], s);
+    else xinv_len<T, 1>(L.g[3][i], s);
+  }
 }
 
 template void launch_xfwd<float>(const PipeLaunch<float>&, hipStream_t);

@@ -60,49 +60,54 @@ k_zpass(const DevProb<T>* __restrict__ probs, const int* __restrict__ bstart, in
 }
 
 template <class T, int DIR, int NC>
-static void go_y(const PipeLaunch<T>& L, hipStream_t s) {
-  hipLaunchKernelGGL((k_ypass<T, DIR, NC>), dim3(L.blocks[1]), dim3(kThreads), L.lds[1], s,
-                     L.probs, L.bstart + (L.nprob + 1), L.nprob);
+static void go_y(const PassLaunch<T>& G, hipStream_t s) {
+  hipLaunchKernelGGL((k_ypass<T, DIR, NC>), dim3(G.blocks), dim3(kThreads), G.lds, s, G.probs,
+                     G.bstart, G.nprob);
 }
 template <class T, int NC>
-static void go_z(const PipeLaunch<T>& L, hipStream_t s) {
-  hipLaunchKernelGGL((k_zpass<T, NC>), dim3(L.blocks[2]), dim3(kThreads), L.lds[2], s, L.probs,
-                     L.bstart + 2 * (L.nprob + 1), L.nprob);
+static void go_z(const PassLaunch<T>& G, hipStream_t s) {
+  hipLaunchKernelGGL((k_zpass<T, NC>), dim3(G.blocks), dim3(kThreads), G.lds, s, G.probs,
+                     G.bstart, G.nprob);
 }
 
 template <class T, int DIR>
-static void y_len(const PipeLaunch<T>& L, hipStream_t s) {
-  switch (L.uni[1]) {
-    case 32: go_y<T, DIR, 32>(L, s); break;
-    case 64: go_y<T, DIR, 64>(L, s); break;
-    case 96: go_y<T, DIR, 96>(L, s); break;
-    case 128: go_y<T, DIR, 128>(L, s); break;
-    case 160: go_y<T, DIR, 160>(L, s); break;
-    case 192: go_y<T, DIR, 192>(L, s); break;
-    case 256: go_y<T, DIR, 256>(L, s); break;
-    case 288: go_y<T, DIR, 288>(L, s); break;
-    default: go_y<T, DIR, 0>(L, s); break;
+static void y_len(const PassLaunch<T>& G, hipStream_t s) {
+  switch (G.len) {
+    case 32: go_y<T, DIR, 32>(G, s); break;
+    case 64: go_y<T, DIR, 64>(G, s); break;
+    case 96: go_y<T, DIR, 96>(G, s); break;
+    case 128: go_y<T, DIR, 128>(G, s); break;
+    case 160: go_y<T, DIR, 160>(G, s); break;
+    case 192: go_y<T, DIR, 192>(G, s); break;
+    case 256: go_y<T, DIR, 256>(G, s); break;
+    case 288: go_y<T, DIR, 288>(G, s); break;
+    default: go_y<T, DIR, 0>(G, s); break;
   }
 }
 
 template <class T>
 void launch_ypass(const PipeLaunch<T>& L, int dir, hipStream_t s) {
-  if (dir < 0) y_len<T, -1>(L, s);
-  else y_len<T, 1>(L, s);
+  for (int i = 0; i < L.ng[1]; ++i) {
+    if (dir < 0) y_len<T, -1>(L.g[1][i], s);
+    else y_len<T, 1>(L.g[1][i], s);
+  }
 }
 
 template <class T>
 void launch_zpass(const PipeLaunch<T>& L, hipStream_t s) {
-  switch (L.uni[2]) {
-    case 32: go_z<T, 32>(L, s); break;
-    case 64: go_z<T, 64>(L, s); break;
-    case 96: go_z<T, 96>(L, s); break;
-    case 128: go_z<T, 128>(L, s); break;
-    case 160: go_z<T, 160>(L, s); break;
-    case 192: go_z<T, 192>(L, s); break;
-    case 256: go_z<T, 256>(L, s); break;
-    case 288: go_z<T, 288>(L, s); break;
-    default: go_z<T, 0>(L, s); break;
+  for (int i = 0; i < L.ng[2]; ++i) {
+    const PassLaunch<T>& G = L.g[2][i];
+    switch (G.len) {
+      case 32: go_z<T, 32>(G, s); break;
+      case 64: go_z<T, 64>(G, s); break;
+      case 96: go_z<T, 96>(G, s); break;
+      case 128: go_z<T, 128>(G, s); break;
+      case 160: go_z<T, 160>(G, s); break;
+      case 192: go_z<T, 192>(G, s); break;
+      case 256: go_z<T, 256>(G, s); break;
+      case 288: go_z<T, 288>(G, s); break;
+      default: go_z<T, 0>(G, s); break;
+    }
   }
 }
 

@@ -7,10 +7,11 @@ namespace cbs {
 
 void server_start(Ctx& c, int max_active);
 void server_stop(Ctx& c);
+// stop and free the server of a context that is being destroyed
+void server_destroy(Ctx& c);
 uint64_t server_submit(Ctx& c, const cbs_request& r);
 bool server_poll(Ctx& c, uint64_t ticket, cbs_solve_stats* stats);
-void server_wait(Ctx& c, uint64_t ticket, double* E, double* H, double hr, double hi,
-                 cbs_solve_stats* stats);
+void server_wait(Ctx& c, uint64_t ticket, double* E, double* H, cbs_solve_stats* stats);
 void server_cancel(Ctx& c, uint64_t ticket);
 
 }  // namespace cbs

@@ -49,4 +49,96 @@ void gmres_solve(Ctx* ctx, std::vector<SolveItem<T>>& it, const IterOpts& io, hi
 template <class T>
 void time_sweeps(Problem& p, int reps, double* ms, double* bytes);
 
+// rhs_i = G_avg src_i for every item (one pipeline launch)
+template <class T>
+void green_set(Ctx* ctx, std::vector<SolveItem<T>>& it, std::vector<const Cx<T>*>& src,
+               std::vector<Cx<T>*>& dst, hipStream_t s);
+// FP64 residual r = b - A x per item; returns ||gamma r||^2
+void resid64(Ctx* ctx, std::vector<SolveItem<double>>& it, std::vector<const Cx<double>*>& x,
+             std::vector<Cx<double>*>& r, std::vector<double>& g2, hipStream_t s);
+void gnorm64(Ctx* ctx, std::vector<SolveItem<double>>& it, std::vector<double>& g2,
+             hipStream_t s);
+// inner tolerance of the FP32 correction solve of iterative refinement
+double refine_inner_tol(double need, double user_inner_tol);
+// iterations per runner chunk for a method (Born: even)
+int runner_chunk(int method, int check_every);
+// algorithmic HBM bytes of one operator application (five sweeps)
+double algo_bytes_per_iter(const Problem& p, int prec);
+// iteration options of a solve from the C-ABI option block
+IterOpts iter_opts(const cbs_solve_opts& o, int maxit);
+// add the outcome of one solver run (state rs, precision prec) to st
+void fill_stats(cbs_solve_stats& st, const RState& rs, const Problem& p, int prec);
+// allocate a work slot of problem p for one right-hand side
+template <class T>
+Work<T>& prepare_work(Problem& p, int slot, bool need_r);
+
+// ---------------------------------------------------------------------------
+// Stepwise solver sets for the serving loop: items join (add) and leave
+// (remove) between chunks; every chunk runs k iterations of all items in
+// shared launches.  An item's arithmetic does not depend on the other items.
+// ---------------------------------------------------------------------------
+struct Finished {
+  long long key;
+  RState st;
+  const void* x;  // device result (3 nvox, precision of the runner)
+};
+
+// state shared by both runners: membership, chunk graph, state gather
+template <class T>
+struct RunnerBase {
+  struct Item {
+    long long key;
+    SolveItem<T> it;
+    long long launched;  // iterations launched for this item
+    long long cap;       // safety bound on launched iterations
+  };
+  Ctx* ctx = nullptr;
+  hipStream_t s = nullptr;
+  int k = 2;              // iterations per chunk
+  bool use_graph = false;
+  std::vector<Item> items;
+  bool dirty = true;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  DevBuf gath;
+  std::vector<double> hg;  // (done, iters, r2, ans) per item after the last chunk
+  RunnerBase() = default;
+  RunnerBase(const RunnerBase&) = delete;
+  RunnerBase& operator=(const RunnerBase&) = delete;
+  ~RunnerBase() { drop_graph(); }
+  void drop_graph();
+  bool empty() const { return items.empty(); }
+  // finished items (device done flag set, or cap reached) leave the set
+  void collect(std::vector<Finished>& out, bool born);
+  void remove(long long key);
+};
+
+// Born: x <- x + gamma (b - A x), buffers alternate X[0] -> X[1] -> X[0];
+// chunks are even so every member holds its iterate in X[0] between chunks
+template <class T>
+struct BornRunner : RunnerBase<T> {
+  Pipe<T> pipe[2];
+  StateSet fin;
+  void add(const std::vector<long long>& keys, std::vector<SolveItem<T>>& xs,
+           const IterOpts& io);
+  void chunk();
+  void collect(std::vector<Finished>& out) { RunnerBase<T>::collect(out, true); }
+  void rebuild();
+};
+
+// BiCGStab on the preconditioned system; x is X[0] of each member
+template <class T>
+struct BicgRunner : RunnerBase<T> {
+  Pipe<T> m1, m2;
+  StateSet fin_mv, fin_vec;
+  DevBuf dv;
+  long long maxchunk = 1;
+  void add(const std::vector<long long>& keys, std::vector<SolveItem<T>>& xs,
+           const IterOpts& io);
+  void chunk();
+  void collect(std::vector<Finished>& out) { RunnerBase<T>::collect(out, false); }
+  void rebuild();
+  void iteration();
+};
+
 }  // namespace cbs

@@ -91,97 +91,168 @@ void read_states(std::vector<SolveItem<T>>& it, std::vector<RState>& out, hipStr
 }
 
 // ---------------------------------------------------------------------------
-// Born iteration over a set of right-hand sides
+// runner membership, chunk graph and state gather (shared by Born and BiCGStab)
 // ---------------------------------------------------------------------------
 template <class T>
-void born_solve(Ctx* ctx, std::vector<SolveItem<T>>& it, const IterOpts& io, hipStream_t s,
-                std::vector<RState>& states, std::vector<const Cx<T>*>& result) {
-  const int n = (int)it.size();
-  StateSet gn, fin;
+void RunnerBase<T>::drop_graph() {
+  if (gexec) (void)hipGraphExecDestroy(gexec);
+  if (graph) (void)hipGraphDestroy(graph);
+  gexec = nullptr;
+  graph = nullptr;
+}
+
+template <class T>
+void RunnerBase<T>::collect(std::vector<Finished>& out, bool born) {
+  if (hg.size() != 4 * items.size()) return;  // no gather since the last change
+  std::vector<Item> keep;
+  for (size_t i = 0; i < items.size(); ++i) {
+    Item& m = items[i];
+    if (hg[4 * i] != 0.0 || m.launched >= m.cap) {
+      Finished f;
+      f.key = m.key;
+      CBS_HIP(hipMemcpyAsync(&f.st, m.it.w->st.p, sizeof(RState), hipMemcpyDeviceToHost, s));
+      CBS_HIP(hipStreamSynchronize(s));
+      // Born: the flagged operand X[ans]; between chunks a running iterate is in X[0]
+      const int b = born && f.st.done ? (f.st.ans & 1) : 0;
+      f.x = m.it.w->X[b].p;
+      out.push_back(f);
+    } else {
+      keep.push_back(m);
+    }
+  }
+  if (keep.size() != items.size()) {
+    items.swap(keep);
+    dirty = true;
+  }
+  hg.clear();
+}
+
+template <class T>
+void RunnerBase<T>::remove(long long key) {
+  for (size_t i = 0; i < items.size(); ++i)
+    if (items[i].key == key) {
+      items.erase(items.begin() + i);
+      dirty = true;
+      hg.clear();
+      return;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Born iteration
+// ---------------------------------------------------------------------------
+template <class T>
+void BornRunner<T>::add(const std::vector<long long>& keys, std::vector<SolveItem<T>>& xs,
+                        const IterOpts& io) {
+  if (xs.empty()) return;
+  StateSet gn;
   // x0 = b (reference, convergent_born.jl:727-728) or x0 = 0 -> x1 = gamma b
-  for (auto& x : it) {
+  for (auto& x : xs) {
     Cx<T>* X0 = x.w->X[0].template as<Cx<T>>();
     if (io.x0_zero)
-      launch_gamma_mul<T>(X0, x.b, potential<T>(*x.p), x.p->nvox, x.p->eps_inv, s);
+      launch_gamma_mul<T>(X0, x.b, potential<T>(*x.p), x.p->nvox, x.p->eps_inv, this->s);
     else
-      copy_dev<T>(X0, x.b, 3 * x.p->nvox, s);
+      copy_dev<T>(X0, x.b, 3 * x.p->nvox, this->s);
   }
-  init_states<T>(ctx, it, io.tol, io.abstol, io.rule, io.maxit, s, gn);
+  init_states<T>(this->ctx, xs, io.tol, io.abstol, io.rule, io.maxit, this->s, gn);
+  for (size_t i = 0; i < xs.size(); ++i)
+    this->items.push_back({keys[i], xs[i], 0, (long long)io.maxit + 2 * this->k + 2});
+  this->dirty = true;
+  this->hg.clear();
+}
 
-  Pipe<T> pipe[2];
+template <class T>
+void BornRunner<T>::rebuild() {
+  this->drop_graph();
+  const int n = (int)this->items.size();
   std::vector<RState*> sts;
   std::vector<double*> parts;
   std::vector<int> nt;
   for (int par = 0; par < 2; ++par) {
-    for (auto& x : it) {
-      Cx<T>* Xi = x.w->X[par].template as<Cx<T>>();
-      Cx<T>* Xo = x.w->X[1 - par].template as<Cx<T>>();
-      pipe[par].h.push_back(make_desc<T>(*x.p, PRE_V, ZOP_GREEN, POST_BORN, Xi, x.b, Xo,
+    pipe[par].h.clear();
+    for (auto& m : this->items) {
+      SolveItem<T>& x = m.it;
+      pipe[par].h.push_back(make_desc<T>(*x.p, PRE_V, ZOP_GREEN, POST_BORN,
+                                         x.w->X[par].template as<Cx<T>>(), x.b,
+                                         x.w->X[1 - par].template as<Cx<T>>(),
                                          x.w->W.template as<Cx<T>>(),
                                          x.w->part.template as<double>(),
                                          x.w->st.template as<RState>()));
     }
-    pipe[par].build(ctx, s);
+    pipe[par].build(this->ctx, this->s);
   }
   for (int i = 0; i < n; ++i) {
-    sts.push_back(it[i].w->st.template as<RState>());
-    parts.push_back(it[i].w->part.template as<double>());
+    sts.push_back(this->items[i].it.w->st.template as<RState>());
+    parts.push_back(this->items[i].it.w->part.template as<double>());
     nt.push_back(pipe[0].h[i].tiles[3]);
   }
-  fin.build(ctx, sts, parts, nt, s);
-  DevBuf gath;
-  gath.alloc(ctx, (size_t)n * 4 * sizeof(double));
-  std::vector<double> hg((size_t)n * 4);
-
-  auto step = [&](int par) {
-    pipe[par].launch(s);
-    launch_born_finalize(fin.sts.as<RState*>(), fin.parts.as<double*>(), fin.ntiles.as<int>(), n,
-                         par, s);
-  };
-  const int chunk = std::max(1, io.check_every);
-  const int chunk2 = io.use_graph ? 2 * ((chunk + 1) / 2) : chunk;
-  hipGraphExec_t gexec = nullptr;
-  hipGraph_t graph = nullptr;
-  if (io.use_graph) {
-    CBS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    for (int k = 0; k < chunk2; ++k) step(k & 1);
-    CBS_HIP(hipStreamEndCapture(s, &graph));
-    CBS_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
-  }
-  int par = 0;
-  long long launched = 0;
-  const long long cap = (long long)io.maxit + 2 * chunk2 + 2;
-  try {
-    while (true) {
-      if (gexec) {
-        CBS_HIP(hipGraphLaunch(gexec, s));
-      } else {
-        for (int k = 0; k < chunk2; ++k) {
-          step(par);
-          par ^= 1;
-        }
-      }
-      CBS_HIP(hipGetLastError());
-      launched += chunk2;
-      launch_gather_state(fin.sts.as<RState*>(), n, gath.as<double>(), s);
-      CBS_HIP(hipMemcpyAsync(hg.data(), gath.p, hg.size() * sizeof(double),
-                             hipMemcpyDeviceToHost, s));
-      CBS_HIP(hipStreamSynchronize(s));
-      bool all = true;
-      for (int i = 0; i < n; ++i) all = all && hg[4 * i] != 0.0;
-      if (all || launched >= cap) break;
+  fin.build(this->ctx, sts, parts, nt, this->s);
+  this->gath.ensure(this->ctx, (size_t)n * 4 * sizeof(double));
+  if (this->use_graph) {
+    CBS_HIP(hipStreamBeginCapture(this->s, hipStreamCaptureModeThreadLocal));
+    for (int j = 0; j < this->k; ++j) {
+      pipe[j & 1].launch(this->s);
+      launch_born_finalize(fin.sts.as<RState*>(), fin.parts.as<double*>(), fin.ntiles.as<int>(),
+                           n, j & 1, this->s);
     }
-  } catch (...) {
-    if (gexec) (void)hipGraphExecDestroy(gexec);
-    if (graph) (void)hipGraphDestroy(graph);
-    throw;
+    CBS_HIP(hipStreamEndCapture(this->s, &this->graph));
+    CBS_HIP(hipGraphInstantiate(&this->gexec, this->graph, nullptr, nullptr, 0));
   }
-  if (gexec) (void)hipGraphExecDestroy(gexec);
-  if (graph) (void)hipGraphDestroy(graph);
-  read_states<T>(it, states, s);
+  this->dirty = false;
+}
+
+template <class T>
+void BornRunner<T>::chunk() {
+  const int n = (int)this->items.size();
+  if (n == 0) return;
+  if (this->dirty) rebuild();
+  if (this->gexec) {
+    CBS_HIP(hipGraphLaunch(this->gexec, this->s));
+  } else {
+    for (int j = 0; j < this->k; ++j) {
+      pipe[j & 1].launch(this->s);
+      launch_born_finalize(fin.sts.as<RState*>(), fin.parts.as<double*>(), fin.ntiles.as<int>(),
+                           n, j & 1, this->s);
+    }
+  }
+  CBS_HIP(hipGetLastError());
+  for (auto& m : this->items) m.launched += this->k;
+  this->hg.resize((size_t)n * 4);
+  launch_gather_state(fin.sts.as<RState*>(), n, this->gath.template as<double>(), this->s);
+  CBS_HIP(hipMemcpyAsync(this->hg.data(), this->gath.p, this->hg.size() * sizeof(double),
+                         hipMemcpyDeviceToHost, this->s));
+  CBS_HIP(hipStreamSynchronize(this->s));
+}
+
+// iterations per runner chunk: Born needs an even count (>= check_every)
+int runner_chunk(int method, int check_every) {
+  const int c = std::max(1, check_every);
+  return method == CBS_BORN ? 2 * ((c + 1) / 2) : c;
+}
+
+template <class T>
+void born_solve(Ctx* ctx, std::vector<SolveItem<T>>& it, const IterOpts& io, hipStream_t s,
+                std::vector<RState>& states, std::vector<const Cx<T>*>& result) {
+  const int n = (int)it.size();
+  BornRunner<T> R;
+  R.ctx = ctx;
+  R.s = s;
+  R.k = runner_chunk(CBS_BORN, io.check_every);
+  R.use_graph = io.use_graph != 0;
+  std::vector<long long> keys(n);
+  for (int i = 0; i < n; ++i) keys[i] = i;
+  R.add(keys, it, io);
+  std::vector<Finished> done;
+  while (!R.empty()) {
+    R.chunk();
+    R.collect(done);
+  }
+  states.resize(n);
   result.resize(n);
-  for (int i = 0; i < n; ++i)
-    result[i] = it[i].w->X[states[i].ans & 1].template as<Cx<T>>();
+  for (auto& f : done) {
+    states[f.key] = f.st;
+    result[f.key] = reinterpret_cast<const Cx<T>*>(f.x);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -240,7 +311,7 @@ void gnorm64(Ctx* ctx, std::vector<SolveItem<double>>& it, std::vector<double>& 
 // ---------------------------------------------------------------------------
 // solve_set: the C-ABI solves route here
 // ---------------------------------------------------------------------------
-static double algo_bytes_per_iter(const Problem& p, int prec) {
+double algo_bytes_per_iter(const Problem& p, int prec) {
   const double s = prec == CBS_FP64 ? 16.0 : 8.0;
   return (p.S == 2 ? 62.0 : 38.0) * s * (double)p.nvox;
 }
@@ -254,15 +325,22 @@ static void ensure_v32(Problem& p, hipStream_t s) {
 }
 
 template <class T>
+Work<T>& prepare_work(Problem& p, int slot, bool need_r) {
+  Work<T>& w = work_slot<T>(p, slot);
+  DevProb<T> d = make_desc<T>(p, PRE_V, ZOP_GREEN, POST_BORN, nullptr, nullptr, nullptr, nullptr,
+                              nullptr, nullptr);
+  w.ensure(p.ctx, p.nvox, p.S, d.tiles[3], need_r, d.ws[2]);
+  return w;
+}
+template Work<double>& prepare_work<double>(Problem&, int, bool);
+template Work<float>& prepare_work<float>(Problem&, int, bool);
+
+template <class T>
 static void prepare_items(std::vector<RhsIO>& io, std::vector<SolveItem<T>>& it, bool need_r) {
   std::map<Problem*, int> used;
   it.clear();
   for (auto& r : io) {
-    const int k = used[r.p]++;
-    Work<T>& w = work_slot<T>(*r.p, k);
-    DevProb<T> d = make_desc<T>(*r.p, PRE_V, ZOP_GREEN, POST_BORN, nullptr, nullptr, nullptr,
-                                nullptr, nullptr, nullptr);
-    w.ensure(r.p->ctx, r.p->nvox, r.p->S, d.tiles[3], need_r, d.ws[2]);
+    Work<T>& w = prepare_work<T>(*r.p, used[r.p]++, need_r);
     SolveItem<T> x;
     x.p = r.p;
     x.w = &w;
@@ -271,7 +349,15 @@ static void prepare_items(std::vector<RhsIO>& io, std::vector<SolveItem<T>>& it,
   }
 }
 
-static IterOpts iter_opts(const cbs_solve_opts& o, int maxit) {
+// eta: the reduction the outer test still needs (x 0.3), kept within
+// [1e-3, 0.1]: tighter inner targets stall the FP32 Born iteration on the slow
+// modes left by the previous correction (measured: 1e-4 does not finish in
+// 4000 sweeps where 1e-3 needs 927 in total)
+double refine_inner_tol(double need, double user_inner_tol) {
+  return user_inner_tol > 0 ? user_inner_tol : std::min(0.1, std::max(1e-3, 0.3 * need));
+}
+
+IterOpts iter_opts(const cbs_solve_opts& o, int maxit) {
   IterOpts io;
   io.tol = o.tol;
   io.abstol = o.abstol;
@@ -284,7 +370,7 @@ static IterOpts iter_opts(const cbs_solve_opts& o, int maxit) {
   return io;
 }
 
-static void fill_stats(cbs_solve_stats& st, const RState& rs, const Problem& p, int prec) {
+void fill_stats(cbs_solve_stats& st, const RState& rs, const Problem& p, int prec) {
   st.iters += rs.iters;
   if (prec == CBS_FP64) st.iters_fp64 += rs.iters;
   st.converged = rs.done == 1;
@@ -298,6 +384,43 @@ void run_method(Ctx* ctx, std::vector<SolveItem<T>>& it, const IterOpts& io, int
   if (method == CBS_BORN) born_solve<T>(ctx, it, io, s, states, result);
   else if (method == CBS_BICGSTAB) bicgstab_solve<T>(ctx, it, io, s, states, result);
   else gmres_solve<T>(ctx, it, io, s, states, result);
+}
+
+// Born or BiCGStab over a set where every item has its own option block
+template <class T>
+static void run_each(Ctx* ctx, std::vector<SolveItem<T>>& it, std::vector<IterOpts>& ios,
+                     int method, hipStream_t s, std::vector<RState>& states,
+                     std::vector<const Cx<T>*>& result) {
+  const int n = (int)it.size();
+  std::vector<Finished> done;
+  auto drain = [&](auto& R) {
+    R.ctx = ctx;
+    R.s = s;
+    R.k = runner_chunk(method, ios[0].check_every);
+    R.use_graph = ios[0].use_graph != 0;
+    for (int i = 0; i < n; ++i) {
+      std::vector<long long> key{i};
+      std::vector<SolveItem<T>> one{it[i]};
+      R.add(key, one, ios[i]);
+    }
+    while (!R.empty()) {
+      R.chunk();
+      R.collect(done);
+    }
+  };
+  if (method == CBS_BORN) {
+    BornRunner<T> R;
+    drain(R);
+  } else {
+    BicgRunner<T> R;
+    drain(R);
+  }
+  states.resize(n);
+  result.resize(n);
+  for (auto& f : done) {
+    states[f.key] = f.st;
+    result[f.key] = reinterpret_cast<const Cx<T>*>(f.x);
+  }
 }
 
 void solve_set(Ctx* ctx, std::vector<RhsIO>& io, const cbs_solve_opts& o,
@@ -422,30 +545,35 @@ void solve_set(Ctx* ctx, std::vector<RhsIO>& io, const cbs_solve_opts& o,
         }
       }
       if (go.empty()) break;
-      // inner FP32 solve of A d = r (d0 = 0) to a relative tolerance eta
+      // inner FP32 solve of A d = r (d0 = 0) to a relative tolerance eta_i
+      // of its own (each right-hand side refines as it would alone)
       std::vector<SolveItem<float>> a32;
-      int inner_max = 0;
-      // eta: the reduction the outer test still needs (x 0.3), kept within
-      // [1e-3, 0.1]: tighter inner targets stall the FP32 Born iteration on
-      // the slow modes left by the previous correction (measured: 1e-4 does
-      // not finish in 4000 sweeps where 1e-3 needs 927 in total)
-      double need = 1.0;
-      for (size_t j = 0; j < idx.size(); ++j)
-        if (active[idx[j]]) need = std::min(need, std::sqrt(thr2[idx[j]] / std::max(g2[j], 1e-300)));
-      for (int i : go) {
+      std::vector<IterOpts> ios;
+      for (size_t j = 0; j < idx.size(); ++j) {
+        const int i = idx[j];
+        if (!active[i]) continue;
         launch_convert<float, double>(it32[i].w->B.as<Cx<float>>(), rs[i], 3 * io[i].p->nvox, s);
         a32.push_back(it32[i]);
-        inner_max = std::max(inner_max, o.maxiters - used[i]);
+        IterOpts io32 = iter_opts(o, std::max(1, o.maxiters - used[i]));
+        io32.tol = refine_inner_tol(std::sqrt(thr2[i] / std::max(g2[j], 1e-300)), o.inner_tol);
+        io32.abstol = 0.0;
+        io32.rule = CBS_STOP_RHS;
+        io32.x0_zero = 1;
+        ios.push_back(io32);
       }
-      const double eta = o.inner_tol > 0 ? o.inner_tol : std::min(0.1, std::max(1e-3, 0.3 * need));
-      IterOpts io32 = iter_opts(o, std::max(1, inner_max));
-      io32.tol = eta;
-      io32.abstol = 0.0;
-      io32.rule = CBS_STOP_RHS;
-      io32.x0_zero = 1;
       std::vector<RState> st;
       std::vector<const Cx<float>*> res;
-      run_method<float>(ctx, a32, io32, o.method, s, st, res);
+      if (o.method == CBS_GMRES) {
+        // GMRES runs the set with one option block: the tightest eta of the set
+        IterOpts io32 = ios[0];
+        for (auto& x : ios) {
+          io32.tol = std::min(io32.tol, x.tol);
+          io32.maxit = std::max(io32.maxit, x.maxit);
+        }
+        gmres_solve<float>(ctx, a32, io32, s, st, res);
+      } else {
+        run_each<float>(ctx, a32, ios, o.method, s, st, res);
+      }
       for (size_t j = 0; j < go.size(); ++j) {
         const int i = go[j];
         launch_accum<double, float>(xs[i], res[j], 3 * io[i].p->nvox, s);
@@ -525,6 +653,26 @@ void time_sweeps(Problem& p, int reps, double* ms, double* bytes) {
 template void time_sweeps<double>(Problem&, int, double*, double*);
 template void time_sweeps<float>(Problem&, int, double*, double*);
 
+template void read_states<double>(std::vector<SolveItem<double>>&, std::vector<RState>&,
+                                  hipStream_t);
+template void read_states<float>(std::vector<SolveItem<float>>&, std::vector<RState>&,
+                                 hipStream_t);
+template void init_states<double>(Ctx*, std::vector<SolveItem<double>>&, double, double, int,
+                                  int, hipStream_t, StateSet&);
+template void init_states<float>(Ctx*, std::vector<SolveItem<float>>&, double, double, int, int,
+                                 hipStream_t, StateSet&);
+template void green_set<double>(Ctx*, std::vector<SolveItem<double>>&,
+                                std::vector<const Cx<double>*>&, std::vector<Cx<double>*>&,
+                                hipStream_t);
+template void green_set<float>(Ctx*, std::vector<SolveItem<float>>&,
+                               std::vector<const Cx<float>*>&, std::vector<Cx<float>*>&,
+                               hipStream_t);
+template void copy_dev<double>(Cx<double>*, const Cx<double>*, long long, hipStream_t);
+template void copy_dev<float>(Cx<float>*, const Cx<float>*, long long, hipStream_t);
+template struct RunnerBase<double>;
+template struct RunnerBase<float>;
+template struct BornRunner<double>;
+template struct BornRunner<float>;
 template void born_solve<double>(Ctx*, std::vector<SolveItem<double>>&, const IterOpts&,
                                  hipStream_t, std::vector<RState>&,
                                  std::vector<const Cx<double>*>&);

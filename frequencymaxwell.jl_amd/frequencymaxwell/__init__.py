@@ -11,7 +11,8 @@ from .boundary_conditions import (AbsorbingBoundaryCondition, AbstractBoundaryCo
                                   PeriodicBoundaryCondition, TanhProfile, TukeyProfile,
                                   padding_pixels, requires_averaging, requires_padding,
                                   subpixel_shift)
-from .engine import CBSProblem, GreenSpec, make_opts, solve_batch as solve_problems
+from .engine import (CBSProblem, CBSServer, GreenSpec, make_opts, request_footprint,
+                     solve_batch as solve_problems)
 from .fields import ElectromagneticField, crop_to_ROI
 from .medium import Medium, grid_size, permittivity, permittivity_bg, phantom_bead
 from .solver import (CBSLinearOperator, CBSPreconditioner, ConvergentBornSolver, DyadicGreen,
@@ -29,12 +30,12 @@ def is_backend_available(device) -> bool:
 
 __all__ = [
     "AbsorbingBoundaryCondition", "AbstractBoundaryCondition", "AttenuationProfile",
-    "CBSError", "CBSLinearOperator", "CBSPreconditioner", "CBSProblem", "Context",
+    "CBSError", "CBSLinearOperator", "CBSPreconditioner", "CBSProblem", "CBSServer", "Context",
     "ConvergentBornSolver", "DyadicGreen", "ElectromagneticField", "ExponentialProfile",
     "GreenSpec", "Medium", "PeriodicBoundaryCondition", "PlaneWaveSource", "TanhProfile",
     "TukeyProfile", "compute_ROI", "compute_magnetic_field", "conv", "crop_to_ROI",
     "default_context", "domain_size", "generate_incident_field", "grid_size", "grid_spacing",
     "h_scale", "is_backend_available", "load_library", "make_opts", "padding_pixels",
     "permittivity", "permittivity_bg", "phantom_bead", "requires_averaging", "requires_padding",
-    "reset", "solve", "solve_batch", "solve_problems", "source_wavelength", "subpixel_shift",
+    "request_footprint", "reset", "solve", "solve_batch", "solve_problems", "source_wavelength", "subpixel_shift",
 ]

@@ -138,8 +138,7 @@ def _declare(lib):
         "cbs_server_stop": (s, [_P]),
         "cbs_submit": (s, [_P, C.POINTER(Request), C.POINTER(C.c_uint64)]),
         "cbs_poll": (s, [_P, C.c_uint64, C.POINTER(C.c_int32), C.POINTER(SolveStats)]),
-        "cbs_wait": (s, [_P, C.c_uint64, _D, _D, C.c_double, C.c_double,
-                         C.POINTER(SolveStats)]),
+        "cbs_wait": (s, [_P, C.c_uint64, _D, _D, C.POINTER(SolveStats)]),
         "cbs_cancel": (s, [_P, C.c_uint64]),
         "cbs_time_sweeps": (s, [_P, C.c_int32, C.c_int32, _D, _D]),
         "cbs_stream_copy": (s, [_P, C.c_uint64, C.c_int32, _D]),

@@ -70,6 +70,21 @@ class GreenSpec:
     eps_imag: float
 
 
+def make_desc(spec: GreenSpec, precision="fp64") -> L.ProblemDesc:
+    """``cbs_problem_desc`` of a Green operator."""
+    d = L.ProblemDesc()
+    for k in range(3):
+        d.n[k] = int(spec.shape[k])
+        d.res[k] = float(spec.resolution[k])
+        d.shift[k] = float(spec.shifts[k])
+    d.averaging = int(bool(spec.use_averaging))
+    d.k2_re = float(complex(spec.k_square).real)
+    d.k2_im = float(complex(spec.k_square).imag)
+    d.eps_imag = float(spec.eps_imag)
+    d.precision = precision_code(precision)
+    return d
+
+
 class CBSProblem:
     """A padded potential V uploaded to the GPU plus its Green operator."""
 
@@ -79,16 +94,7 @@ class CBSProblem:
         self.spec = spec
         self.shape = tuple(int(s) for s in spec.shape)
         V = L.as_field(V, self.shape)
-        d = L.ProblemDesc()
-        for k in range(3):
-            d.n[k] = self.shape[k]
-            d.res[k] = float(spec.resolution[k])
-            d.shift[k] = float(spec.shifts[k])
-        d.averaging = int(bool(spec.use_averaging))
-        d.k2_re = float(complex(spec.k_square).real)
-        d.k2_im = float(complex(spec.k_square).imag)
-        d.eps_imag = float(spec.eps_imag)
-        d.precision = precision_code(precision)
+        d = make_desc(spec, precision)
         self.desc = d
         h = C.c_void_p()
         L.check(self._lib.cbs_problem_create(self.ctx.handle, C.byref(d), L.dptr(V), C.byref(h)))
@@ -221,3 +227,74 @@ def solve_batch(problems, S_list, opts: L.SolveOpts, want_h: bool = True, ctx=No
                                 ArrD(*[L.dptr(a) for a in Hs]) if want_h else None,
                                 C.byref(opts), sts))
     return Es, Hs, list(sts)
+
+
+def request_footprint(spec: GreenSpec, opts: L.SolveOpts) -> int:
+    """Bytes of HBM a served request holds while it is resident."""
+    out = C.c_uint64()
+    d = make_desc(spec, opts.precision)
+    L.check(L.load().cbs_request_footprint(C.byref(d), C.byref(opts), C.byref(out)))
+    return out.value
+
+
+class CBSServer:
+    """Request loop on one GPU (``cbs_server_*``): submitted solves are admitted
+    by HBM footprint, share every sweep launch with the other resident solves
+    (grids may differ) and leave as soon as their own stopping rule is met."""
+
+    def __init__(self, max_active: int = 8, ctx: L.Context | None = None):
+        self.ctx = ctx or L.default_context()
+        self._lib = L.load()
+        self._shapes = {}
+        L.check(self._lib.cbs_server_start(self.ctx.handle, int(max_active)))
+        self.running = True
+
+    def submit(self, V, spec: GreenSpec, S, opts: L.SolveOpts) -> int:
+        """Queue one solve (V and S are copied); returns its ticket."""
+        shape = tuple(int(x) for x in spec.shape)
+        V = L.as_field(V, shape)
+        S = L.as_field(S, shape + (3,))
+        req = L.Request()
+        req.desc = make_desc(spec, opts.precision)
+        req.V = L.dptr(V)
+        req.S = L.dptr(S)
+        req.opts = opts
+        t = C.c_uint64()
+        L.check(self._lib.cbs_submit(self.ctx.handle, C.byref(req), C.byref(t)))
+        self._shapes[t.value] = (shape + (3,), bool(opts.compute_h))
+        return t.value
+
+    def poll(self, ticket: int):
+        """(done, stats or None)."""
+        done = C.c_int32()
+        st = L.SolveStats()
+        L.check(self._lib.cbs_poll(self.ctx.handle, int(ticket), C.byref(done), C.byref(st)))
+        return bool(done.value), (st if done.value else None)
+
+    def wait(self, ticket: int):
+        """Block until the solve ends: (E_scat, H_scat or None, stats)."""
+        info = self._shapes.pop(int(ticket), None)
+        if info is None:  # not submitted here: the library reports the ticket
+            L.check(self._lib.cbs_wait(self.ctx.handle, int(ticket), None, None, None))
+            return None, None, None
+        shape, want_h = info
+        E = np.empty(shape, dtype=np.complex128, order="F")
+        H = np.empty(shape, dtype=np.complex128, order="F") if want_h else None
+        st = L.SolveStats()
+        L.check(self._lib.cbs_wait(self.ctx.handle, int(ticket), L.dptr(E),
+                                   L.dptr(H) if H is not None else None, C.byref(st)))
+        return E, H, st
+
+    def cancel(self, ticket: int):
+        L.check(self._lib.cbs_cancel(self.ctx.handle, int(ticket)))
+
+    def stop(self):
+        if self.running:
+            L.check(self._lib.cbs_server_stop(self.ctx.handle))
+            self.running = False
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.stop()

@@ -191,24 +191,40 @@ cbs_status cbs_time_sweeps(cbs_problem *p, int32_t precision, int32_t reps, doub
  * byte counters. */
 cbs_status cbs_stream_copy(cbs_ctx *ctx, uint64_t bytes, int32_t reps, double *ms);
 
-/* ---- serving: continuous batching with HBM-footprint admission ---------- */
+/* ---- serving: continuous batching with HBM-footprint admission ----------
+ * No counterpart in the reference (it solves one problem per call).  A
+ * server thread owns a HIP stream; admitted requests share every sweep
+ * launch (one work table over all of them, grids may differ), join at the
+ * next chunk boundary when a slot and HBM are free, and leave as soon as
+ * their own stopping rule is met.  Each request's arithmetic is independent
+ * of the others': its result is the same alone and in any mix.
+ * Supported per request: method CBS_BORN or CBS_BICGSTAB, precision CBS_FP64
+ * or CBS_FP32_REFINE. */
 typedef struct {
   cbs_problem_desc desc;  /* grid and Green parameters                       */
-  const double *V;        /* potential, copied at submit                     */
-  const double *S;        /* source term, copied at submit                   */
-  cbs_solve_opts opts;    /* per-request tolerance / method / precision      */
+  const double *V;        /* potential (Nx*Ny*Nz complex), copied at submit  */
+  const double *S;        /* source term (Nx*Ny*Nz*3 complex), copied at submit */
+  cbs_solve_opts opts;    /* per-request tolerance / method / precision; H is
+                             computed with opts.h_scale when opts.compute_h  */
 } cbs_request;
 
 /* Bytes of HBM a request of this description holds while resident. */
 cbs_status cbs_request_footprint(const cbs_problem_desc *desc, const cbs_solve_opts *opts,
                                  uint64_t *bytes);
+/* Start the server thread; at most max_active requests resident at once.
+ * Admission also respects the context's HBM budget (cbs_ctx_create). */
 cbs_status cbs_server_start(cbs_ctx *ctx, int32_t max_active);
+/* Stop the thread; pending and running requests end with CBS_ECANCELED. */
 cbs_status cbs_server_stop(cbs_ctx *ctx);
+/* Queue a request (copies V and S); returns immediately with a ticket.
+ * CBS_ENOMEM if the request can never fit the budget. */
 cbs_status cbs_submit(cbs_ctx *ctx, const cbs_request *req, uint64_t *ticket);
-/* done = 1 when the result is ready; E/H (padded fields) may be NULL. */
+/* done = 1 when the result is ready (stats filled), 0 while queued/running. */
 cbs_status cbs_poll(cbs_ctx *ctx, uint64_t ticket, int32_t *done, cbs_solve_stats *stats);
+/* Block until the request ends; copy E_scat (and H_scat if computed; either
+ * may be NULL) and release the ticket. */
 cbs_status cbs_wait(cbs_ctx *ctx, uint64_t ticket, double *E_scat, double *H_scat,
-                    double h_re, double h_im, cbs_solve_stats *stats);
+                    cbs_solve_stats *stats);
 cbs_status cbs_cancel(cbs_ctx *ctx, uint64_t ticket);
 
 #ifdef __cplusplus
