@@ -198,13 +198,16 @@ def read_traffic(summary: str, kernel: str):
 def serving_leg(fm, args):
     """The mixed-size request stream, continuously batched and one at a time."""
     print("[bench] serving stream ...", file=sys.stderr, flush=True)
-    pool, cb = serve_stream(fm, args.serve_requests, args.serve_rate, args.serve_active, 7,
-                            args.maxiters)
-    print(f"[bench] continuous batching: p50 {cb['p50_ms']:.0f} ms p99 {cb['p99_ms']:.0f} ms",
-          file=sys.stderr, flush=True)
-    _, seq = serve_stream(fm, args.serve_requests, args.serve_rate, 1, 7, args.maxiters, pool)
-    print(f"[bench] one at a time: p50 {seq['p50_ms']:.0f} ms p99 {seq['p99_ms']:.0f} ms",
-          file=sys.stderr, flush=True)
+    pool, cb, seq = None, None, None
+    if args.serve_mode in ("both", "batched"):
+        pool, cb = serve_stream(fm, args.serve_requests, args.serve_rate, args.serve_active, 7,
+                                args.maxiters)
+        print(f"[bench] continuous batching: p50 {cb['p50_ms']:.0f} ms p99 {cb['p99_ms']:.0f} ms",
+              file=sys.stderr, flush=True)
+    if args.serve_mode in ("both", "sequential"):
+        _, seq = serve_stream(fm, args.serve_requests, args.serve_rate, 1, 7, args.maxiters, pool)
+        print(f"[bench] one at a time: p50 {seq['p50_ms']:.0f} ms p99 {seq['p99_ms']:.0f} ms",
+              file=sys.stderr, flush=True)
     return {"workload": "Poisson arrivals, grids " + ", ".join("x".join(map(str, g)) for g in SERVE_GRIDS)
                         + " (absorbing 16 px padding each side), 1-10 spheres n 1.36-1.60, "
                           "plane wave within NA 1.0, tol 1e-4 or 1e-6, Born FP32+refine",
@@ -231,10 +234,11 @@ def main():
                     help="profiler run: only the timed FP32 sweeps and the 1 GiB calibration copy")
     ap.add_argument("--profile-reps", type=int, default=20)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-stream leg")
-    ap.add_argument("--serve-requests", type=int, default=24)
-    ap.add_argument("--serve-rate", type=float, default=1.5, help="arrivals per second")
+    ap.add_argument("--serve-requests", type=int, default=16)
+    ap.add_argument("--serve-rate", type=float, default=0.6, help="arrivals per second")
     ap.add_argument("--serve-active", type=int, default=8)
     ap.add_argument("--serve-only", action="store_true")
+    ap.add_argument("--serve-mode", default="both", choices=["both", "batched", "sequential"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build measures the single-GPU path only "

@@ -137,8 +137,15 @@ struct Pipe {
   std::vector<DevProb<T>> h;
   DevBuf d_probs, d_bstart;
   PipeLaunch<T> L{};
+  // per pass and group: members (indices into h, ascending) and the host copy
+  // of the group's block offsets
+  std::vector<int> gmem[4][kMaxGroups];
+  std::vector<int> gbs[4][kMaxGroups];
   void build(Ctx* c, hipStream_t s);
   void launch(hipStream_t s) const { launch_pipeline<T>(L, s); }
+  // the launch restricted to the first n entries of h (every group launches
+  // only the blocks of its members below n)
+  PipeLaunch<T> prefix(int n) const;
 };
 
 // device arrays describing the RHS of a Born/Krylov solve (for finalize kernels)
@@ -166,10 +173,16 @@ void operator_apply_dev(Problem& p, Cx<T>* y, const Cx<T>* x, double ar, double 
                         double bi, hipStream_t s);
 template <class T>
 void curl_apply_dev(Problem& p, const Cx<T>* E, Cx<T>* H, double sr, double si, hipStream_t s);
+// the same with a caller-provided spectrum workspace W (S * 3 * ws[2] complex)
+template <class T>
+void curl_apply_ws(Problem& p, const Cx<T>* E, Cx<T>* H, double sr, double si, Cx<T>* W,
+                   hipStream_t s);
 
 void validate_desc(const cbs_problem_desc& d);
-// set up problem p of context c from a host potential V (Nx*Ny*Nz complex):
-// FP64 tables always, FP32 tables and V32 when want32
+// set up problem p of context c: tables and potential buffers (V64, and V32
+// when want32), contents of V left to the caller
+void problem_alloc(Problem& p, Ctx* c, const cbs_problem_desc& d, bool want32);
+// problem_alloc plus the upload of a host potential V (Nx*Ny*Nz complex)
 void problem_init(Problem& p, Ctx* c, const cbs_problem_desc& d, const double* V, bool want32,
                   hipStream_t s);
 uint64_t footprint_bytes(const cbs_problem_desc& d, const cbs_solve_opts& o);

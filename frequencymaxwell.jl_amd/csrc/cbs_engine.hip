@@ -202,6 +202,7 @@ void Pipe<T>::build(Ctx* c, hipStream_t s) {
   struct Grp {
     int pass, off_p, off_b, n, blocks, len;
     size_t lds;
+    std::vector<int> mem;
   };
   std::vector<Grp> grps;
   std::vector<DevProb<T>> hp;
@@ -210,13 +211,10 @@ void Pipe<T>::build(Ctx* c, hipStream_t s) {
   const int axis_of[4] = {0, 1, 2, 0};
   for (int pass = 0; pass < 4; ++pass) {
     const int ax = axis_of[pass];
-    // group key: the axis length where a specialised kernel exists (>= 32,
-    // compile-time plan), else 0 (per-block switch kernel)
+    // group key: the axis length where a specialised kernel exists
+    // (compile-time plan), else 0 (per-block switch kernel)
     std::vector<int> key(np), keys;
-    for (int i = 0; i < np; ++i) {
-      const int u = h[i].rt[ax] ? 0 : h[i].n[ax];
-      key[i] = u >= 32 ? u : 0;
-    }
+    for (int i = 0; i < np; ++i) key[i] = h[i].rt[ax] ? 0 : h[i].n[ax];
     for (int i = 0; i < np; ++i) {
       if (std::find(keys.begin(), keys.end(), key[i]) != keys.end()) continue;
       if (key[i] != 0 && (int)keys.size() >= kMaxGroups - 1) {
@@ -228,9 +226,10 @@ void Pipe<T>::build(Ctx* c, hipStream_t s) {
     for (int i = 0; i < np; ++i)
       if (std::find(keys.begin(), keys.end(), key[i]) == keys.end()) key[i] = 0;
     for (int k : keys) {
-      Grp g{pass, (int)hp.size(), (int)hb.size(), 0, 0, k, 0};
+      Grp g{pass, (int)hp.size(), (int)hb.size(), 0, 0, k, 0, {}};
       for (int i = 0; i < np; ++i) {
         if (key[i] != k) continue;
+        g.mem.push_back(i);
         hp.push_back(h[i]);
         hb.push_back(g.blocks);
         g.blocks += h[i].tiles[pass];
@@ -254,7 +253,14 @@ void Pipe<T>::build(Ctx* c, hipStream_t s) {
   CBS_HIP(hipMemcpyAsync(d_bstart.p, hb.data(), hb.size() * sizeof(int), hipMemcpyHostToDevice,
                          s));
   CBS_HIP(hipStreamSynchronize(s));
+  for (int pass = 0; pass < 4; ++pass)
+    for (int i = 0; i < kMaxGroups; ++i) {
+      gmem[pass][i].clear();
+      gbs[pass][i].clear();
+    }
   for (const Grp& g : grps) {
+    gmem[g.pass][L.ng[g.pass]] = g.mem;
+    gbs[g.pass][L.ng[g.pass]].assign(hb.begin() + g.off_b, hb.begin() + g.off_b + g.n + 1);
     PassLaunch<T>& G = L.g[g.pass][L.ng[g.pass]++];
     G.probs = d_probs.as<DevProb<T>>() + g.off_p;
     G.bstart = d_bstart.as<int>() + g.off_b;
@@ -264,6 +270,25 @@ void Pipe<T>::build(Ctx* c, hipStream_t s) {
     G.len = g.len;
   }
 }
+template <class T>
+PipeLaunch<T> Pipe<T>::prefix(int n) const {
+  if (n >= (int)h.size()) return L;
+  PipeLaunch<T> P = L;
+  for (int pass = 0; pass < 4; ++pass) {
+    P.ng[pass] = 0;
+    for (int i = 0; i < L.ng[pass]; ++i) {
+      const std::vector<int>& m = gmem[pass][i];
+      const int cnt = (int)(std::lower_bound(m.begin(), m.end(), n) - m.begin());
+      if (cnt == 0) continue;
+      PassLaunch<T> G = L.g[pass][i];
+      G.nprob = cnt;
+      G.blocks = gbs[pass][i][cnt];
+      P.g[pass][P.ng[pass]++] = G;
+    }
+  }
+  return P;
+}
+
 template struct Pipe<double>;
 template struct Pipe<float>;
 
@@ -350,10 +375,18 @@ void operator_apply_dev(Problem& p, Cx<T>* y, const Cx<T>* x, double ar, double 
 template <class T>
 void curl_apply_dev(Problem& p, const Cx<T>* E, Cx<T>* H, double sr, double si, hipStream_t s) {
   Work<T>& w = work<T>(p);
+  DevProb<T> probe =
+      make_desc<T>(p, PRE_COPY, ZOP_CURL, POST_STORE, E, nullptr, H, nullptr, nullptr, nullptr);
+  w.ensure(p.ctx, p.nvox, p.S, probe.tiles[3], false, probe.ws[2]);
+  curl_apply_ws<T>(p, E, H, sr, si, w.W.template as<Cx<T>>(), s);
+}
+
+template <class T>
+void curl_apply_ws(Problem& p, const Cx<T>* E, Cx<T>* H, double sr, double si, Cx<T>* W,
+                   hipStream_t s) {
   DevProb<T> d =
       make_desc<T>(p, PRE_COPY, ZOP_CURL, POST_STORE, E, nullptr, H, nullptr, nullptr, nullptr);
-  w.ensure(p.ctx, p.nvox, p.S, d.tiles[3], false, d.ws[2]);
-  d.w = w.W.template as<Cx<T>>();
+  d.w = W;
   // H = scale * IFFT(i k x FFT(E)); the 1/N of the inverse transform is folded in
   d.zs_re = sr / (double)p.nvox;
   d.zs_im = si / (double)p.nvox;
@@ -369,7 +402,9 @@ void curl_apply_dev(Problem& p, const Cx<T>* E, Cx<T>* H, double sr, double si, 
   template void green_apply_dev<T>(Problem&, Cx<T>*, hipStream_t);                          \
   template void operator_apply_dev<T>(Problem&, Cx<T>*, const Cx<T>*, double, double, double, \
                                       double, hipStream_t);                                 \
-  template void curl_apply_dev<T>(Problem&, const Cx<T>*, Cx<T>*, double, double, hipStream_t);
+  template void curl_apply_dev<T>(Problem&, const Cx<T>*, Cx<T>*, double, double, hipStream_t); \
+  template void curl_apply_ws<T>(Problem&, const Cx<T>*, Cx<T>*, double, double, Cx<T>*,       \
+                                 hipStream_t);
 CBS_APPLY_INST(double)
 CBS_APPLY_INST(float)
 
@@ -390,8 +425,7 @@ void validate_desc(const cbs_problem_desc& d) {
   CBS_REQUIRE(d.averaging == 0 || d.averaging == 1, "averaging must be 0 or 1");
 }
 
-void problem_init(Problem& p, Ctx* c, const cbs_problem_desc& d, const double* V, bool want32,
-                  hipStream_t s) {
+void problem_alloc(Problem& p, Ctx* c, const cbs_problem_desc& d, bool want32) {
   p.ctx = c;
   p.d = d;
   for (int k = 0; k < 3; ++k) p.n[k] = (int)d.n[k];
@@ -399,17 +433,23 @@ void problem_init(Problem& p, Ctx* c, const cbs_problem_desc& d, const double* V
   p.S = d.averaging ? 2 : 1;
   p.has_ramp = d.shift[0] != 0.0 || d.shift[1] != 0.0 || d.shift[2] != 0.0;
   p.eps_inv = 1.0 / d.eps_imag;
-  for (long long i = 0; i < 2 * p.nvox; ++i)
-    if (!std::isfinite(V[i])) throw Error(CBS_ENUMERIC, "potential holds NaN or Inf");
   p.V64.alloc(c, (size_t)p.nvox * sizeof(Cx<double>));
-  CBS_HIP(hipMemcpy(p.V64.p, V, (size_t)p.nvox * sizeof(Cx<double>), hipMemcpyHostToDevice));
   fill_tables<double>(p);
   if (want32) {
     fill_tables<float>(p);
-    if (!p.V32.p) {
-      p.V32.alloc(c, (size_t)p.nvox * sizeof(Cx<float>));
-      launch_convert<float, double>(p.V32.as<Cx<float>>(), p.V64.as<Cx<double>>(), p.nvox, s);
-    }
+    p.V32.alloc(c, (size_t)p.nvox * sizeof(Cx<float>));
+  }
+}
+
+void problem_init(Problem& p, Ctx* c, const cbs_problem_desc& d, const double* V, bool want32,
+                  hipStream_t s) {
+  const long long nv = (long long)d.n[0] * d.n[1] * d.n[2];
+  for (long long i = 0; i < 2 * nv; ++i)
+    if (!std::isfinite(V[i])) throw Error(CBS_ENUMERIC, "potential holds NaN or Inf");
+  problem_alloc(p, c, d, want32);
+  CBS_HIP(hipMemcpy(p.V64.p, V, (size_t)p.nvox * sizeof(Cx<double>), hipMemcpyHostToDevice));
+  if (want32) {
+    launch_convert<float, double>(p.V32.as<Cx<float>>(), p.V64.as<Cx<double>>(), p.nvox, s);
     CBS_HIP(hipStreamSynchronize(s));
   }
 }
@@ -435,7 +475,7 @@ uint64_t footprint_bytes(const cbs_problem_desc& d, const cbs_solve_opts& o) {
     const uint64_t m = o.gmres_restart > 0 ? (uint64_t)o.gmres_restart : (uint64_t)o.maxiters;
     b += fk * (std::min<uint64_t>(m, 400) + 3);
   }
-  if (o.compute_h) b += f64;  // curl output
+  if (o.compute_h) b += f64 + (o.precision == CBS_FP32 ? W64 : 0);  // curl output, FP64 workspace
   return b + (4u << 20);      // tables, partial sums, solver state
 }
 

@@ -11,10 +11,12 @@ constexpr int kThreads = 256;     // threads per block of every pass kernel
 constexpr int kMaxStages = 12;    // radix stages of one axis transform
 constexpr int kLdsBudget = 65536; // LDS bytes a pass tile may use (two blocks per CU)
 constexpr int kNumPart = 5;       // doubles of partial sums written per xinv tile
-#ifndef CBS_XLINES
-#define CBS_XLINES 4  // measured best of 16 / 8 / 4 on the 160x160x96 and 256x256x128 grids
-#endif
-constexpr int kXLines = CBS_XLINES;  // most x lines per x-pass tile
+constexpr int kXLines = 4;        // x lines per x-pass tile where no tuned width exists
+
+// Axis lengths with compile-time specialised transforms (one kernel per
+// length and sweep); any other length runs the runtime-plan transform.
+#define CBS_CT_LENGTHS(X) \
+  X(8) X(16) X(32) X(64) X(96) X(128) X(160) X(192) X(224) X(256) X(288) X(320)
 
 // ---------------------------------------------------------------------------
 // complex arithmetic (interleaved re, im; 8 or 16 bytes, naturally aligned)
@@ -128,11 +130,32 @@ __host__ __device__ constexpr int plan_maxradix(int n) {
   return m;
 }
 
-// Lengths with compile-time specialised transforms.  Any other length runs
-// the runtime-plan transform (ping-pong LDS, generic radices).
+// Lengths with compile-time specialised transforms.
 __host__ __device__ constexpr bool is_ct_length(int n) {
-  return n == 8 || n == 16 || n == 32 || n == 64 || n == 96 || n == 128 || n == 160 ||
-         n == 192 || n == 256 || n == 288;
+#define CBS_IS_CT_(L) if (n == L) return true;
+  CBS_CT_LENGTHS(CBS_IS_CT_)
+#undef CBS_IS_CT_
+  return false;
+}
+
+// one measured tile width (cbs_tuned.h, written by frequencymaxwell/_tune.py)
+struct TileTune {
+  int pass, n, eb, S, p;
+};
+
+}  // namespace cbs
+#include "cbs_tuned.h"
+namespace cbs {
+
+// Tuned tile width of a specialised length: pass 0 = x lines, 1 = y columns,
+// 2 = z columns; eb = bytes per complex; S = pipelines (x pass only).
+// 0 when the table has no entry (the heuristic widths apply).
+__host__ __device__ constexpr int tuned_width(int pass, int n, int eb, int S) {
+  for (int i = 0; i < kTileTuneN; ++i) {
+    const TileTune& t = kTileTune[i];
+    if (t.pass == pass && t.n == n && t.eb == eb && (pass != 0 || t.S == S)) return t.p;
+  }
+  return 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -147,18 +170,42 @@ __host__ __device__ constexpr int tile_p(int n, int elem_bytes, int pencils_per_
   while (p > 1 && bufs * (n * (pencils_per_p * p + pad)) * elem_bytes > kLdsBudget) p /= 2;
   return p;
 }
+// Widest tile first: a tuning build forces one width for every length
+// (CBS_TUNE_XP / _YP / _ZP), otherwise the tuned table, otherwise the
+// heuristic; tile_p then halves it until the tile fits the LDS budget.
+__host__ __device__ constexpr int width_x(int nx, int eb, int S, bool rt) {
+#ifdef CBS_TUNE_XP
+  return CBS_TUNE_XP;
+#else
+  return (!rt && tuned_width(0, nx, eb, S)) ? tuned_width(0, nx, eb, S) : kXLines;
+#endif
+}
+__host__ __device__ constexpr int width_y(int ny, int eb, bool rt) {
+#ifdef CBS_TUNE_YP
+  return CBS_TUNE_YP;
+#else
+  return (!rt && tuned_width(1, ny, eb, 0)) ? tuned_width(1, ny, eb, 0) : (eb == 8 ? 32 : 16);
+#endif
+}
+__host__ __device__ constexpr int width_z(int nz, int eb, bool rt) {
+#ifdef CBS_TUNE_ZP
+  return CBS_TUNE_ZP;
+#else
+  return (!rt && tuned_width(2, nz, eb, 0)) ? tuned_width(2, nz, eb, 0) : (eb == 8 ? 16 : 8);
+#endif
+}
 // x-axis passes: 3 components * P lines per tile; specialised lengths run the
 // S pipelines one after the other through the tile, runtime plans hold all S.
 __host__ __device__ constexpr int xlines(int nx, int eb, int S, bool rt) {
-  return tile_p(nx, eb, 3 * (rt ? S : 1), kXLines, rt, 1);
+  return tile_p(nx, eb, 3 * (rt ? S : 1), width_x(nx, eb, S, rt), rt, 1);
 }
 // y pass: P columns of one component
 __host__ __device__ constexpr int ycols(int ny, int eb, bool rt) {
-  return tile_p(ny, eb, 1, eb == 8 ? 32 : 16, rt, 0);
+  return tile_p(ny, eb, 1, width_y(ny, eb, rt), rt, 0);
 }
 // z pass: P columns of three components (Green / curl need all three)
 __host__ __device__ constexpr int zcols(int nz, int eb, bool rt) {
-  return tile_p(nz, eb, 3, eb == 8 ? 16 : 8, rt, 0);
+  return tile_p(nz, eb, 3, width_z(nz, eb, rt), rt, 0);
 }
 __host__ __device__ constexpr int xpad_stride(int q) { return q + 1; }
 

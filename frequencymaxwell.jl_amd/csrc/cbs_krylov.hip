@@ -272,6 +272,7 @@ void BicgRunner<T>::add(const std::vector<long long>& keys, std::vector<SolveIte
 template <class T>
 void BicgRunner<T>::rebuild() {
   this->drop_graph();
+  this->sort_items();
   const int n = (int)this->items.size();
   hipStream_t s = this->s;
   Ctx* ctx = this->ctx;
@@ -306,7 +307,7 @@ void BicgRunner<T>::rebuild() {
   this->gath.ensure(ctx, (size_t)n * 4 * sizeof(double));
   if (this->use_graph) {
     CBS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    for (int j = 0; j < this->k; ++j) iteration();
+    for (int j = 0; j < this->k; ++j) iteration(n);
     CBS_HIP(hipStreamEndCapture(s, &this->graph));
     CBS_HIP(hipGraphInstantiate(&this->gexec, this->graph, nullptr, nullptr, 0));
   }
@@ -314,38 +315,44 @@ void BicgRunner<T>::rebuild() {
 }
 
 template <class T>
-void BicgRunner<T>::iteration() {
+void BicgRunner<T>::iteration(int na) {
   const int n = (int)this->items.size();
   hipStream_t s = this->s;
-  const dim3 vgrid((unsigned)maxchunk, (unsigned)n);
+  const dim3 vgrid((unsigned)maxchunk, (unsigned)na);
   const BiVec<T>* d_vs = dv.as<BiVec<T>>();
+  const PipeLaunch<T> L1 = na == n ? m1.L : m1.prefix(na);
+  const PipeLaunch<T> L2 = na == n ? m2.L : m2.prefix(na);
   hipLaunchKernelGGL((k_bicg_vec<T, BI_P>), vgrid, dim3(kThreads), 0, s, d_vs);
-  m1.launch(s);
-  bicg_fin(fin_mv, n, BF_ALPHA, 0, s);
+  launch_pipeline<T>(L1, s);
+  bicg_fin(fin_mv, na, BF_ALPHA, 0, s);
   hipLaunchKernelGGL((k_bicg_vec<T, BI_S>), vgrid, dim3(kThreads), 0, s, d_vs);
-  bicg_fin(fin_vec, n, BF_S, 0, s);
-  m2.launch(s);
-  bicg_fin(fin_mv, n, BF_OMEGA, 0, s);
+  bicg_fin(fin_vec, na, BF_S, 0, s);
+  launch_pipeline<T>(L2, s);
+  bicg_fin(fin_mv, na, BF_OMEGA, 0, s);
   hipLaunchKernelGGL((k_bicg_vec<T, BI_XR>), vgrid, dim3(kThreads), 0, s, d_vs);
-  bicg_fin(fin_vec, n, BF_R, 0, s);
+  bicg_fin(fin_vec, na, BF_R, 0, s);
 }
 
 template <class T>
-void BicgRunner<T>::chunk() {
+void BicgRunner<T>::launch(int kk) {
   const int n = (int)this->items.size();
   if (n == 0) return;
+  if (kk <= 0) kk = this->k;
   if (this->dirty) rebuild();
-  hipStream_t s = this->s;
-  if (this->gexec) CBS_HIP(hipGraphLaunch(this->gexec, s));
-  else
-    for (int j = 0; j < this->k; ++j) iteration();
+  const std::vector<int> bud = this->budgets(kk, false);
+  const int K = bud[0];
+  if (this->gexec && K == this->k && bud[n - 1] == K) {
+    CBS_HIP(hipGraphLaunch(this->gexec, this->s));
+  } else {
+    int na = n;
+    for (int j = 0; j < K; ++j) {
+      while (na > 0 && bud[na - 1] <= j) --na;
+      iteration(na);
+    }
+  }
   CBS_HIP(hipGetLastError());
-  for (auto& m : this->items) m.launched += this->k;
-  this->hg.resize((size_t)n * 4);
-  launch_gather_state(fin_vec.sts.as<RState*>(), n, this->gath.template as<double>(), s);
-  CBS_HIP(hipMemcpyAsync(this->hg.data(), this->gath.p, this->hg.size() * sizeof(double),
-                         hipMemcpyDeviceToHost, s));
-  CBS_HIP(hipStreamSynchronize(s));
+  for (int i = 0; i < n; ++i) this->items[i].launched += bud[i];
+  this->launch_gather(fin_vec.sts.as<RState*>());
 }
 
 template <class T>

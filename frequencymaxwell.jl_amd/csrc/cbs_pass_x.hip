@@ -17,16 +17,10 @@ k_xfwd(const DevProb<T>* __restrict__ probs, const int* __restrict__ bstart, int
     xfwd_tile<T, NC, S>(pr, tile, tid);
   } else {
     switch (pr.rt[0] ? 0 : pr.n[0]) {
-      case 8: xfwd_tile<T, 8, S>(pr, tile, tid); break;
-      case 16: xfwd_tile<T, 16, S>(pr, tile, tid); break;
-      case 32: xfwd_tile<T, 32, S>(pr, tile, tid); break;
-      case 64: xfwd_tile<T, 64, S>(pr, tile, tid); break;
-      case 96: xfwd_tile<T, 96, S>(pr, tile, tid); break;
-      case 128: xfwd_tile<T, 128, S>(pr, tile, tid); break;
-      case 160: xfwd_tile<T, 160, S>(pr, tile, tid); break;
-      case 192: xfwd_tile<T, 192, S>(pr, tile, tid); break;
-      case 256: xfwd_tile<T, 256, S>(pr, tile, tid); break;
-      case 288: xfwd_tile<T, 288, S>(pr, tile, tid); break;
+#define CBS_CASE_(L) \
+      case L: xfwd_tile<T, L, S>(pr, tile, tid); break;
+      CBS_CT_LENGTHS(CBS_CASE_)
+#undef CBS_CASE_
       default: xfwd_tile<T, 0, S>(pr, tile, tid); break;
     }
   }
@@ -44,16 +38,10 @@ k_xinv(const DevProb<T>* __restrict__ probs, const int* __restrict__ bstart, int
     xinv_tile<T, NC, S>(pr, tile, tid);
   } else {
     switch (pr.rt[0] ? 0 : pr.n[0]) {
-      case 8: xinv_tile<T, 8, S>(pr, tile, tid); break;
-      case 16: xinv_tile<T, 16, S>(pr, tile, tid); break;
-      case 32: xinv_tile<T, 32, S>(pr, tile, tid); break;
-      case 64: xinv_tile<T, 64, S>(pr, tile, tid); break;
-      case 96: xinv_tile<T, 96, S>(pr, tile, tid); break;
-      case 128: xinv_tile<T, 128, S>(pr, tile, tid); break;
-      case 160: xinv_tile<T, 160, S>(pr, tile, tid); break;
-      case 192: xinv_tile<T, 192, S>(pr, tile, tid); break;
-      case 256: xinv_tile<T, 256, S>(pr, tile, tid); break;
-      case 288: xinv_tile<T, 288, S>(pr, tile, tid); break;
+#define CBS_CASE_(L) \
+      case L: xinv_tile<T, L, S>(pr, tile, tid); break;
+      CBS_CT_LENGTHS(CBS_CASE_)
+#undef CBS_CASE_
       default: xinv_tile<T, 0, S>(pr, tile, tid); break;
     }
   }
@@ -73,28 +61,20 @@ static void go_xinv(const PassLaunch<T>& G, hipStream_t s) {
 template <class T, int S>
 static void xfwd_len(const PassLaunch<T>& G, hipStream_t s) {
   switch (G.len) {
-    case 32: go_xfwd<T, S, 32>(G, s); break;
-    case 64: go_xfwd<T, S, 64>(G, s); break;
-    case 96: go_xfwd<T, S, 96>(G, s); break;
-    case 128: go_xfwd<T, S, 128>(G, s); break;
-    case 160: go_xfwd<T, S, 160>(G, s); break;
-    case 192: go_xfwd<T, S, 192>(G, s); break;
-    case 256: go_xfwd<T, S, 256>(G, s); break;
-    case 288: go_xfwd<T, S, 288>(G, s); break;
+#define CBS_CASE_(L) \
+    case L: go_xfwd<T, S, L>(G, s); break;
+      CBS_CT_LENGTHS(CBS_CASE_)
+#undef CBS_CASE_
     default: go_xfwd<T, S, 0>(G, s); break;
   }
 }
 template <class T, int S>
 static void xinv_len(const PassLaunch<T>& G, hipStream_t s) {
   switch (G.len) {
-    case 32: go_xinv<T, S, 32>(G, s); break;
-    case 64: go_xinv<T, S, 64>(G, s); break;
-    case 96: go_xinv<T, S, 96>(G, s); break;
-    case 128: go_xinv<T, S, 128>(G, s); break;
-    case 160: go_xinv<T, S, 160>(G, s); break;
-    case 192: go_xinv<T, S, 192>(G, s); break;
-    case 256: go_xinv<T, S, 256>(G, s); break;
-    case 288: go_xinv<T, S, 288>(G, s); break;
+#define CBS_CASE_(L) \
+    case L: go_xinv<T, S, L>(G, s); break;
+      CBS_CT_LENGTHS(CBS_CASE_)
+#undef CBS_CASE_
     default: go_xinv<T, S, 0>(G, s); break;
   }
 }

@@ -17,16 +17,10 @@ k_ypass(const DevProb<T>* __restrict__ probs, const int* __restrict__ bstart, in
     ypass_tile<T, NC, DIR>(pr, tile, tid);
   } else {
     switch (pr.rt[1] ? 0 : pr.n[1]) {
-      case 8: ypass_tile<T, 8, DIR>(pr, tile, tid); break;
-      case 16: ypass_tile<T, 16, DIR>(pr, tile, tid); break;
-      case 32: ypass_tile<T, 32, DIR>(pr, tile, tid); break;
-      case 64: ypass_tile<T, 64, DIR>(pr, tile, tid); break;
-      case 96: ypass_tile<T, 96, DIR>(pr, tile, tid); break;
-      case 128: ypass_tile<T, 128, DIR>(pr, tile, tid); break;
-      case 160: ypass_tile<T, 160, DIR>(pr, tile, tid); break;
-      case 192: ypass_tile<T, 192, DIR>(pr, tile, tid); break;
-      case 256: ypass_tile<T, 256, DIR>(pr, tile, tid); break;
-      case 288: ypass_tile<T, 288, DIR>(pr, tile, tid); break;
+#define CBS_CASE_(L) \
+      case L: ypass_tile<T, L, DIR>(pr, tile, tid); break;
+      CBS_CT_LENGTHS(CBS_CASE_)
+#undef CBS_CASE_
       default: ypass_tile<T, 0, DIR>(pr, tile, tid); break;
     }
   }
@@ -44,16 +38,10 @@ k_zpass(const DevProb<T>* __restrict__ probs, const int* __restrict__ bstart, in
     zpass_tile<T, NC>(pr, tile, tid);
   } else {
     switch (pr.rt[2] ? 0 : pr.n[2]) {
-      case 8: zpass_tile<T, 8>(pr, tile, tid); break;
-      case 16: zpass_tile<T, 16>(pr, tile, tid); break;
-      case 32: zpass_tile<T, 32>(pr, tile, tid); break;
-      case 64: zpass_tile<T, 64>(pr, tile, tid); break;
-      case 96: zpass_tile<T, 96>(pr, tile, tid); break;
-      case 128: zpass_tile<T, 128>(pr, tile, tid); break;
-      case 160: zpass_tile<T, 160>(pr, tile, tid); break;
-      case 192: zpass_tile<T, 192>(pr, tile, tid); break;
-      case 256: zpass_tile<T, 256>(pr, tile, tid); break;
-      case 288: zpass_tile<T, 288>(pr, tile, tid); break;
+#define CBS_CASE_(L) \
+      case L: zpass_tile<T, L>(pr, tile, tid); break;
+      CBS_CT_LENGTHS(CBS_CASE_)
+#undef CBS_CASE_
       default: zpass_tile<T, 0>(pr, tile, tid); break;
     }
   }
@@ -73,14 +61,10 @@ static void go_z(const PassLaunch<T>& G, hipStream_t s) {
 template <class T, int DIR>
 static void y_len(const PassLaunch<T>& G, hipStream_t s) {
   switch (G.len) {
-    case 32: go_y<T, DIR, 32>(G, s); break;
-    case 64: go_y<T, DIR, 64>(G, s); break;
-    case 96: go_y<T, DIR, 96>(G, s); break;
-    case 128: go_y<T, DIR, 128>(G, s); break;
-    case 160: go_y<T, DIR, 160>(G, s); break;
-    case 192: go_y<T, DIR, 192>(G, s); break;
-    case 256: go_y<T, DIR, 256>(G, s); break;
-    case 288: go_y<T, DIR, 288>(G, s); break;
+#define CBS_CASE_(L) \
+    case L: go_y<T, DIR, L>(G, s); break;
+      CBS_CT_LENGTHS(CBS_CASE_)
+#undef CBS_CASE_
     default: go_y<T, DIR, 0>(G, s); break;
   }
 }
@@ -98,14 +82,10 @@ void launch_zpass(const PipeLaunch<T>& L, hipStream_t s) {
   for (int i = 0; i < L.ng[2]; ++i) {
     const PassLaunch<T>& G = L.g[2][i];
     switch (G.len) {
-      case 32: go_z<T, 32>(G, s); break;
-      case 64: go_z<T, 64>(G, s); break;
-      case 96: go_z<T, 96>(G, s); break;
-      case 128: go_z<T, 128>(G, s); break;
-      case 160: go_z<T, 160>(G, s); break;
-      case 192: go_z<T, 192>(G, s); break;
-      case 256: go_z<T, 256>(G, s); break;
-      case 288: go_z<T, 288>(G, s); break;
+#define CBS_CASE_(L) \
+      case L: go_z<T, L>(G, s); break;
+      CBS_CT_LENGTHS(CBS_CASE_)
+#undef CBS_CASE_
       default: go_z<T, 0>(G, s); break;
     }
   }

@@ -1,15 +1,19 @@
 // Serving loop of the CBS engine (no counterpart in the reference, which solves
-// one problem per call).  One server thread per context owns a HIP stream and
-// eight stepwise solver sets (Born / BiCGStab x FP64 / FP32 x periodic /
-// averaged Green function).  Every tick it
+// one problem per call).  One server thread per context owns two HIP streams
+// (compute, field transfers) and eight stepwise solver sets (Born / BiCGStab x
+// FP64 / FP32 x periodic / averaged Green function).  Every tick it
 //   1. admits queued requests in arrival order while a slot is free and the
 //      request's footprint fits the HBM capacity (the context budget, or the
 //      device memory free at server start),
-//   2. runs the FP64 outer residual of every refinement request that needs one
-//      (one shared launch per averaging mode),
-//   3. advances every non-empty set by one chunk of iterations (shared sweep
-//      launches over all members; grids may differ),
-//   4. retires finished requests: H = curl E when asked, copy to host, free.
+//   2. finishes the admission of requests whose fields have arrived (G S, the
+//      first residual) and runs the FP64 outer residual of every refinement
+//      request that needs one (one shared launch per averaging mode),
+//   3. launches one chunk of iterations of every non-empty set (shared sweep
+//      launches over all members; grids may differ) and, while the GPU runs
+//      it, stages the next upload and the last download through pinned host
+//      buffers on the transfer stream,
+//   4. waits for the chunk, moves finished members on (next refinement step,
+//      or retirement: H = curl E when asked, then the download).
 // Each member has its own solver state, tolerance and iteration limit, and the
 // kernels never mix members, so a request's result does not depend on what
 // else is in flight.
@@ -25,13 +29,23 @@ namespace cbs {
 using Clock = std::chrono::steady_clock;
 
 enum JobState { J_QUEUED = 0, J_ACTIVE = 1, J_DONE = 2 };
-enum Phase { PH_KRYLOV = 0, PH_RESID = 1, PH_INNER = 2, PH_RETIRE = 3 };
+enum Phase {
+  PH_WAIT_LOAD = 0,  // device buffers allocated, fields not yet sent
+  PH_LOADING = 1,    // fields on the transfer stream
+  PH_KRYLOV = 2,     // member of a solver set (FP64 or FP32 method)
+  PH_RESID = 3,      // refinement: waiting for its FP64 residual
+  PH_INNER = 4,      // refinement: FP32 correction solve in a set
+  PH_RETIRE = 5,     // solved: H and download to start
+  PH_WAIT_DOWN = 6,  // H launched, download not started
+  PH_UNLOADING = 7,  // fields on the transfer stream
+  PH_DOWN_DONE = 8   // fields in the pinned buffer
+};
 
 struct Job {
   uint64_t ticket = 0;
   cbs_problem_desc desc{};
   cbs_solve_opts o{};
-  std::vector<double> V, S;  // host copies taken at submit (dropped at admission)
+  std::vector<double> V, S;  // host copies taken at submit (dropped once sent)
   uint64_t footprint = 0;
   Clock::time_point t_submit;
   // shared with client threads (under Server::mu)
@@ -43,25 +57,65 @@ struct Job {
   std::vector<double> E, H;
   // server thread only
   std::unique_ptr<Problem> p;
-  DevBuf Sd, X;  // FP64 source term and iterate / result
-  int phase = PH_KRYLOV;
+  DevBuf Sd, X, Hd;  // FP64 source term, iterate / result, curl result
+  DevBuf Wh;         // FP64 curl workspace of FP32 requests
+  int phase = PH_WAIT_LOAD;
   double bn2 = 0, thr2 = -1.0;
   int used = 0, refines = 0;
   SolveItem<double> it64;
   SolveItem<float> it32;
   int set = -1;  // solver set the job is a member of (-1: none)
+  hipEvent_t ev = nullptr;  // end of the compute work before the download
 };
 
 static int set_index(bool fp32, int method, int S) {
   return (fp32 ? 4 : 0) + (method == CBS_BICGSTAB ? 2 : 0) + (S - 1);
 }
 
+// grow-only pinned host buffer
+struct Pinned {
+  char* p = nullptr;
+  size_t bytes = 0;
+  void ensure(size_t n) {
+    if (bytes >= n) return;
+    if (p) CBS_HIP(hipHostFree(p));
+    p = nullptr;
+    bytes = 0;
+    CBS_HIP(hipHostMalloc((void**)&p, n, hipHostMallocDefault));
+    bytes = n;
+  }
+  ~Pinned() {
+    if (p) (void)hipHostFree(p);
+  }
+};
+
+// host copy split over a few threads (pageable <-> pinned staging)
+static void par_copy(void* dst, const void* src, size_t bytes) {
+  const size_t kPiece = (size_t)64 << 20;
+  const int nt = (int)std::min<size_t>(8, (bytes + kPiece - 1) / kPiece);
+  if (nt <= 1) {
+    std::memcpy(dst, src, bytes);
+    return;
+  }
+  std::vector<std::thread> th;
+  const size_t per = (bytes + nt - 1) / nt;
+  for (int t = 0; t < nt; ++t) {
+    const size_t o = (size_t)t * per;
+    if (o >= bytes) break;
+    const size_t n = std::min(per, bytes - o);
+    th.emplace_back([=] { std::memcpy((char*)dst + o, (const char*)src + o, n); });
+  }
+  for (auto& x : th) x.join();
+}
+
 struct Server {
   Ctx* ctx = nullptr;
   int max_active = 1;
   uint64_t capacity = 0;  // admission limit in bytes
-  int chunk_born = 8, chunk_bicg = 4;
-  hipStream_t stream = nullptr;
+  double target_ms = 20.0;  // aimed duration of one chunk
+  hipStream_t stream = nullptr, io = nullptr;
+  hipEvent_t ev_up = nullptr, ev_down = nullptr;
+  Pinned up, down;
   std::thread th;
   std::mutex mu;
   std::condition_variable cv, cv_done;
@@ -71,6 +125,8 @@ struct Server {
   std::deque<std::shared_ptr<Job>> queue;
   std::map<uint64_t, std::shared_ptr<Job>> jobs;  // tickets not yet released by wait
   std::vector<std::shared_ptr<Job>> active;       // written by the server thread only
+  Job* up_job = nullptr;    // fields in flight to the device
+  Job* down_job = nullptr;  // fields in flight to the host
   BornRunner<double> born64[2];
   BicgRunner<double> bicg64[2];
   BornRunner<float> born32[2];
@@ -86,16 +142,26 @@ struct Server {
     }
   }
   void loop();
-  void admit(Job& j);
+  void tick();
+  void begin_admit(Job& j);
+  void start_upload(Job& j);
+  void finish_admit(Job& j);
   void join_set(Job& j, bool fp32, const IterOpts& io);
   void leave_set(Job& j);
   void resid_step();
-  void run_sets();
-  void retire(Job& j);
+  void launch_sets();
+  void wait_sets();
+  void begin_retire(Job& j);
+  void start_download(Job& j);
   void release(Job& j);
   void finish_locked(Job& j, cbs_status st, const std::string& err);
   void end_job(const std::shared_ptr<Job>& j, cbs_status st, const std::string& err);
   void fail_active(cbs_status st, const std::string& err);
+  std::shared_ptr<Job> find_active(Job* j) {
+    for (auto& a : active)
+      if (a.get() == j) return a;
+    return nullptr;
+  }
 };
 
 // ---------------------------------------------------------------------------
@@ -111,11 +177,26 @@ void Server::finish_locked(Job& j, cbs_status st, const std::string& err) {
 }
 
 void Server::release(Job& j) {
+  // transfers of this job must have ended before its buffers go
+  if (up_job == &j) {
+    (void)hipEventSynchronize(ev_up);
+    up_job = nullptr;
+  }
+  if (down_job == &j) {
+    (void)hipEventSynchronize(ev_down);
+    down_job = nullptr;
+  }
+  if (j.ev) {
+    (void)hipEventDestroy(j.ev);
+    j.ev = nullptr;
+  }
   j.it64 = SolveItem<double>{};
   j.it32 = SolveItem<float>{};
   j.p.reset();
   j.Sd.release();
   j.X.release();
+  j.Hd.release();
+  j.Wh.release();
 }
 
 // a job leaves the active list with status st
@@ -157,49 +238,81 @@ void Server::leave_set(Job& j) {
   j.set = -1;
 }
 
-void Server::admit(Job& j) {
-  Ctx* c = ctx;
-  hipStream_t s = stream;
+// device buffers of an admitted job (the fields follow on the transfer stream)
+void Server::begin_admit(Job& j) {
   j.p.reset(new Problem());
-  Problem& p = *j.p;
-  problem_init(p, c, j.desc, j.V.data(), j.o.precision != CBS_FP64, s);
-  const long long nc = 3 * p.nvox;
-  j.Sd.alloc(c, (size_t)nc * sizeof(Cx<double>));
-  j.X.alloc(c, (size_t)nc * sizeof(Cx<double>));
-  CBS_HIP(hipMemcpyAsync(j.Sd.p, j.S.data(), (size_t)nc * sizeof(Cx<double>),
-                         hipMemcpyHostToDevice, s));
-  CBS_HIP(hipStreamSynchronize(s));
+  problem_alloc(*j.p, ctx, j.desc, j.o.precision != CBS_FP64);
+  const size_t nc = (size_t)j.p->nvox * 3;
+  j.Sd.alloc(ctx, nc * sizeof(Cx<double>));
+  j.X.alloc(ctx, nc * sizeof(Cx<double>));
+  if (j.o.precision == CBS_FP64) {
+    Work<double>& w = prepare_work<double>(*j.p, 0, false);
+    j.it64 = SolveItem<double>{j.p.get(), &w, w.B.as<Cx<double>>()};
+  } else if (j.o.precision == CBS_FP32) {
+    Work<float>& w = prepare_work<float>(*j.p, 0, false);
+    j.it32 = SolveItem<float>{j.p.get(), &w, w.B.as<Cx<float>>()};
+  } else {
+    Work<double>& w = prepare_work<double>(*j.p, 0, true);
+    Work<float>& w32 = prepare_work<float>(*j.p, 0, false);
+    j.it64 = SolveItem<double>{j.p.get(), &w, w.B.as<Cx<double>>()};
+    j.it32 = SolveItem<float>{j.p.get(), &w32, w32.B.as<Cx<float>>()};
+  }
+  if (j.o.compute_h) {
+    j.Hd.alloc(ctx, nc * sizeof(Cx<double>));
+    if (!j.it64.w) {  // FP32 requests: an FP64 workspace for the curl
+      DevProb<double> d = make_desc<double>(*j.p, PRE_COPY, ZOP_CURL, POST_STORE, nullptr,
+                                            nullptr, nullptr, nullptr, nullptr, nullptr);
+      j.Wh.alloc(ctx, (size_t)j.p->S * 3 * (size_t)d.ws[2] * sizeof(Cx<double>));
+    }
+  }
+  j.phase = PH_WAIT_LOAD;
+}
+
+// V and S through the pinned buffer to the device (the host copy overlaps the
+// chunk running on the compute stream)
+void Server::start_upload(Job& j) {
+  const size_t vb = j.V.size() * sizeof(double), sb = j.S.size() * sizeof(double);
+  up.ensure(vb + sb);
+  par_copy(up.p, j.V.data(), vb);
+  par_copy(up.p + vb, j.S.data(), sb);
+  CBS_HIP(hipMemcpyAsync(j.p->V64.p, up.p, vb, hipMemcpyHostToDevice, io));
+  CBS_HIP(hipMemcpyAsync(j.Sd.p, up.p + vb, sb, hipMemcpyHostToDevice, io));
+  CBS_HIP(hipEventRecord(ev_up, io));
   std::vector<double>().swap(j.V);
   std::vector<double>().swap(j.S);
+  up_job = &j;
+  j.phase = PH_LOADING;
+}
+
+// fields on the device: right-hand side and first state (compute stream)
+void Server::finish_admit(Job& j) {
+  Ctx* c = ctx;
+  hipStream_t s = stream;
+  Problem& p = *j.p;
+  const long long nc = 3 * p.nvox;
   const cbs_solve_opts& o = j.o;
+  if (o.precision != CBS_FP64)
+    launch_convert<float, double>(p.V32.as<Cx<float>>(), p.V64.as<Cx<double>>(), p.nvox, s);
   if (o.precision == CBS_FP64) {
-    Work<double>& w = prepare_work<double>(p, 0, false);
-    j.it64 = SolveItem<double>{&p, &w, w.B.as<Cx<double>>()};
     std::vector<SolveItem<double>> one{j.it64};
     std::vector<const Cx<double>*> src{j.Sd.as<Cx<double>>()};
-    std::vector<Cx<double>*> dst{w.B.as<Cx<double>>()};
+    std::vector<Cx<double>*> dst{j.it64.w->B.as<Cx<double>>()};
     green_set<double>(c, one, src, dst, s);  // rhs = G_avg S (convergent_born.jl:715-723)
     join_set(j, false, iter_opts(o, o.maxiters));
     j.phase = PH_KRYLOV;
   } else if (o.precision == CBS_FP32) {
-    Work<float>& w = prepare_work<float>(p, 0, false);
-    j.it32 = SolveItem<float>{&p, &w, w.B.as<Cx<float>>()};
-    Cx<float>* tmp = w.X[1].as<Cx<float>>();
+    Cx<float>* tmp = j.it32.w->X[1].as<Cx<float>>();
     launch_convert<float, double>(tmp, j.Sd.as<Cx<double>>(), nc, s);
     std::vector<SolveItem<float>> one{j.it32};
     std::vector<const Cx<float>*> src{tmp};
-    std::vector<Cx<float>*> dst{w.B.as<Cx<float>>()};
+    std::vector<Cx<float>*> dst{j.it32.w->B.as<Cx<float>>()};
     green_set<float>(c, one, src, dst, s);
     join_set(j, true, iter_opts(o, o.maxiters));
     j.phase = PH_KRYLOV;
   } else {
-    Work<double>& w = prepare_work<double>(p, 0, true);
-    Work<float>& w32 = prepare_work<float>(p, 0, false);
-    j.it64 = SolveItem<double>{&p, &w, w.B.as<Cx<double>>()};
-    j.it32 = SolveItem<float>{&p, &w32, w32.B.as<Cx<float>>()};
     std::vector<SolveItem<double>> one{j.it64};
     std::vector<const Cx<double>*> src{j.Sd.as<Cx<double>>()};
-    std::vector<Cx<double>*> dst{w.B.as<Cx<double>>()};
+    std::vector<Cx<double>*> dst{j.it64.w->B.as<Cx<double>>()};
     green_set<double>(c, one, src, dst, s);
     std::vector<double> bn2;
     gnorm64(c, one, bn2, s);
@@ -267,13 +380,39 @@ void Server::resid_step() {
   }
 }
 
-void Server::run_sets() {
+// one chunk of every non-empty set.  Every member gets an equal share of
+// target_ms of GPU time: its iteration budget is that share over its cost per
+// iteration (FP32 sweeps move ~62 x 8 bytes per voxel, FP64 twice that,
+// BiCGStab two operator applications), so small grids are not held to the
+// pace of large ones.
+void Server::launch_sets() {
+  int members = 0;
+  each_set([&](auto& R, int) { members += (int)R.items.size(); });
+  if (members == 0) return;
+  const double share = target_ms / members;
+  each_set([&](auto& R, int idx) {
+    if (R.empty()) return;
+    const bool bicg = (idx & 2) != 0;
+    const double per_vox = 62.0 * ((idx & 4) ? 8.0 : 16.0) * (bicg ? 2.0 : 1.0);
+    int K = 0;
+    for (auto& m : R.items) {
+      const double iter_ms = (double)m.it.p->nvox * per_vox / 3.0e9;  // 3 TB/s sustained
+      int k = (int)std::floor(share / std::max(iter_ms, 1e-6));
+      k = bicg ? std::max(1, std::min(32, k)) : std::max(2, std::min(64, k & ~1));
+      m.budget = k;
+      K = std::max(K, k);
+    }
+    R.launch(K);
+  });
+}
+
+void Server::wait_sets() {
   std::map<uint64_t, Job*> by_ticket;
   for (auto& j : active) by_ticket[j->ticket] = j.get();
   std::vector<Finished> fin;
   each_set([&](auto& R, int idx) {
     if (R.empty()) return;
-    R.chunk();
+    R.wait();
     fin.clear();
     R.collect(fin);
     const bool fp32 = idx >= 4;
@@ -307,39 +446,94 @@ void Server::run_sets() {
   CBS_HIP(hipGetLastError());
 }
 
-void Server::retire(Job& j) {
+// H = curl E when asked (compute stream), then the download may start
+void Server::begin_retire(Job& j) {
   Problem& p = *j.p;
-  const size_t nc = (size_t)p.nvox * 3;
-  std::vector<double> E(2 * nc), H;
-  if (j.o.compute_h) {
-    DevBuf hd;
-    hd.alloc(ctx, nc * sizeof(Cx<double>));
-    curl_apply_dev<double>(p, j.X.as<Cx<double>>(), hd.as<Cx<double>>(), j.o.h_scale_re,
-                           j.o.h_scale_im, stream);  // convergent_born.jl:784-800
-    H.resize(2 * nc);
-    CBS_HIP(hipMemcpyAsync(H.data(), hd.p, nc * sizeof(Cx<double>), hipMemcpyDeviceToHost,
-                           stream));
-    CBS_HIP(hipStreamSynchronize(stream));
-  }
-  CBS_HIP(hipMemcpyAsync(E.data(), j.X.p, nc * sizeof(Cx<double>), hipMemcpyDeviceToHost,
-                         stream));
-  CBS_HIP(hipStreamSynchronize(stream));
+  if (j.o.compute_h)  // convergent_born.jl:784-800, in the finished solve's workspace
+    curl_apply_ws<double>(p, j.X.as<Cx<double>>(), j.Hd.as<Cx<double>>(), j.o.h_scale_re,
+                          j.o.h_scale_im,
+                          j.it64.w ? j.it64.w->W.as<Cx<double>>() : j.Wh.as<Cx<double>>(),
+                          stream);
+  CBS_HIP(hipEventCreateWithFlags(&j.ev, hipEventDisableTiming));
+  CBS_HIP(hipEventRecord(j.ev, stream));
   j.stats.voxel_iters = (int64_t)j.stats.iters * p.nvox;
-  std::lock_guard<std::mutex> lk(mu);
-  j.E.swap(E);
-  j.H.swap(H);
+  j.phase = PH_WAIT_DOWN;
+}
+
+void Server::start_download(Job& j) {
+  const size_t fb = (size_t)j.p->nvox * 3 * sizeof(Cx<double>);
+  down.ensure(j.o.compute_h ? 2 * fb : fb);
+  CBS_HIP(hipStreamWaitEvent(io, j.ev, 0));
+  CBS_HIP(hipMemcpyAsync(down.p, j.X.p, fb, hipMemcpyDeviceToHost, io));
+  if (j.o.compute_h)
+    CBS_HIP(hipMemcpyAsync(down.p + fb, j.Hd.p, fb, hipMemcpyDeviceToHost, io));
+  CBS_HIP(hipEventRecord(ev_down, io));
+  down_job = &j;
+  j.phase = PH_UNLOADING;
 }
 
 void Server::fail_active(cbs_status st, const std::string& err) {
   // a failed shared launch leaves every member's state unknown
+  (void)hipStreamSynchronize(stream);
+  (void)hipStreamSynchronize(io);
   std::vector<std::shared_ptr<Job>> all = active;
   for (auto& j : all) end_job(j, st, err);
   each_set([&](auto& R, int) {
     R.items.clear();
     R.dirty = true;
     R.hg.clear();
+    R.pending = false;
     R.drop_graph();
   });
+}
+
+void Server::tick() {
+  // transfers that have ended
+  if (up_job && hipEventQuery(ev_up) == hipSuccess) {
+    Job* j = up_job;
+    up_job = nullptr;
+    finish_admit(*j);
+  }
+  Job* landed = nullptr;
+  if (down_job && hipEventQuery(ev_down) == hipSuccess) {
+    landed = down_job;
+    down_job = nullptr;
+    landed->phase = PH_DOWN_DONE;
+  }
+  resid_step();
+  for (auto& j : active)
+    if (j->phase == PH_RETIRE) begin_retire(*j);
+  launch_sets();
+  // host side of the transfers while the chunk runs
+  if (landed) {
+    const size_t fb = (size_t)landed->p->nvox * 3 * sizeof(Cx<double>);
+    std::vector<double> E(fb / sizeof(double)), H;
+    par_copy(E.data(), down.p, fb);
+    if (landed->o.compute_h) {
+      H.resize(fb / sizeof(double));
+      par_copy(H.data(), down.p + fb, fb);
+    }
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      landed->E.swap(E);
+      landed->H.swap(H);
+    }
+    auto sp = find_active(landed);
+    if (sp) end_job(sp, CBS_OK, "");
+  }
+  if (!down_job)
+    for (auto& j : active)
+      if (j->phase == PH_WAIT_DOWN) {
+        start_download(*j);
+        break;
+      }
+  if (!up_job)
+    for (auto& j : active)
+      if (j->phase == PH_WAIT_LOAD) {
+        start_upload(*j);
+        break;
+      }
+  wait_sets();
 }
 
 void Server::loop() {
@@ -375,7 +569,7 @@ void Server::loop() {
     for (auto& j : adm) {
       if (j->state == J_DONE) continue;  // canceled above
       try {
-        admit(*j);
+        begin_admit(*j);
       } catch (const Error& e) {
         end_job(j, e.code, e.what());
       } catch (const std::exception& e) {
@@ -383,15 +577,7 @@ void Server::loop() {
       }
     }
     try {
-      resid_step();
-      run_sets();
-      std::vector<std::shared_ptr<Job>> done;
-      for (auto& j : active)
-        if (j->phase == PH_RETIRE) done.push_back(j);
-      for (auto& j : done) {
-        retire(*j);
-        end_job(j, CBS_OK, "");
-      }
+      tick();
     } catch (const Error& e) {
       fail_active(e.code, e.what());
     } catch (const std::exception& e) {
@@ -415,6 +601,9 @@ void server_start(Ctx& c, int max_active) {
   CBS_REQUIRE(!sv.running, "server already running");
   CBS_HIP(hipSetDevice(c.device));
   if (!sv.stream) CBS_HIP(hipStreamCreateWithFlags(&sv.stream, hipStreamNonBlocking));
+  if (!sv.io) CBS_HIP(hipStreamCreateWithFlags(&sv.io, hipStreamNonBlocking));
+  if (!sv.ev_up) CBS_HIP(hipEventCreateWithFlags(&sv.ev_up, hipEventDisableTiming));
+  if (!sv.ev_down) CBS_HIP(hipEventCreateWithFlags(&sv.ev_down, hipEventDisableTiming));
   sv.ctx = &c;
   sv.max_active = max_active;
   if (c.budget) {
@@ -429,8 +618,7 @@ void server_start(Ctx& c, int max_active) {
     R.ctx = &c;
     R.s = sv.stream;
     R.use_graph = false;
-    R.k = runner_chunk((idx & 2) ? CBS_BICGSTAB : CBS_BORN,
-                       (idx & 2) ? sv.chunk_bicg : sv.chunk_born);
+    R.k = runner_chunk((idx & 2) ? CBS_BICGSTAB : CBS_BORN, 8);
   });
   sv.stop = false;
   sv.running = true;
@@ -454,12 +642,16 @@ void server_destroy(Ctx& c) {
   server_stop(c);
   std::lock_guard<std::mutex> g(c.mu);
   if (!c.server) return;
-  if (c.server->stream) {
-    (void)hipSetDevice(c.device);
-    (void)hipStreamSynchronize(c.server->stream);
-    (void)hipStreamDestroy(c.server->stream);
-  }
-  delete c.server;
+  (void)hipSetDevice(c.device);
+  Server* sv = c.server;
+  for (hipStream_t st : {sv->stream, sv->io})
+    if (st) {
+      (void)hipStreamSynchronize(st);
+      (void)hipStreamDestroy(st);
+    }
+  for (hipEvent_t e : {sv->ev_up, sv->ev_down})
+    if (e) (void)hipEventDestroy(e);
+  delete sv;
   c.server = nullptr;
 }
 

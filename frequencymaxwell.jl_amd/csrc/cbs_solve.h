@@ -91,6 +91,7 @@ struct RunnerBase {
     SolveItem<T> it;
     long long launched;  // iterations launched for this item
     long long cap;       // safety bound on launched iterations
+    int budget = 0;      // iterations in the next chunk (0: the chunk length)
   };
   Ctx* ctx = nullptr;
   hipStream_t s = nullptr;
@@ -101,16 +102,29 @@ struct RunnerBase {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   DevBuf gath;
+  double* hgp = nullptr;   // pinned landing buffer of the state gather
+  size_t hgp_n = 0;
+  bool pending = false;    // a chunk has been launched and not waited for
   std::vector<double> hg;  // (done, iters, r2, ans) per item after the last chunk
   RunnerBase() = default;
   RunnerBase(const RunnerBase&) = delete;
   RunnerBase& operator=(const RunnerBase&) = delete;
-  ~RunnerBase() { drop_graph(); }
+  ~RunnerBase();
   void drop_graph();
   bool empty() const { return items.empty(); }
+  // after the chunk's launches: gather (done, iters, r2, ans) of every item
+  void launch_gather(RState* const* sts);
+  // block until the launched chunk and its gather are complete
+  void wait();
   // finished items (device done flag set, or cap reached) leave the set
   void collect(std::vector<Finished>& out, bool born);
   void remove(long long key);
+  // members in ascending cost (voxels), so that the members still running in
+  // any iteration of a chunk with per-member budgets form a prefix
+  void sort_items();
+  // per-member iterations of a chunk of K (non-increasing along the items,
+  // even when even_only)
+  std::vector<int> budgets(int K, bool even_only) const;
 };
 
 // Born: x <- x + gamma (b - A x), buffers alternate X[0] -> X[1] -> X[0];
@@ -121,7 +135,12 @@ struct BornRunner : RunnerBase<T> {
   StateSet fin;
   void add(const std::vector<long long>& keys, std::vector<SolveItem<T>>& xs,
            const IterOpts& io);
-  void chunk();
+  // k iterations of every member (k even; 0: the runner's own k), no wait
+  void launch(int k = 0);
+  void chunk() {
+    launch();
+    this->wait();
+  }
   void collect(std::vector<Finished>& out) { RunnerBase<T>::collect(out, true); }
   void rebuild();
 };
@@ -135,10 +154,14 @@ struct BicgRunner : RunnerBase<T> {
   long long maxchunk = 1;
   void add(const std::vector<long long>& keys, std::vector<SolveItem<T>>& xs,
            const IterOpts& io);
-  void chunk();
+  void launch(int k = 0);
+  void chunk() {
+    launch();
+    this->wait();
+  }
   void collect(std::vector<Finished>& out) { RunnerBase<T>::collect(out, false); }
   void rebuild();
-  void iteration();
+  void iteration(int na);  // one iteration of the first na members
 };
 
 }  // namespace cbs

@@ -102,6 +102,35 @@ void RunnerBase<T>::drop_graph() {
 }
 
 template <class T>
+RunnerBase<T>::~RunnerBase() {
+  drop_graph();
+  if (hgp) (void)hipHostFree(hgp);
+}
+
+template <class T>
+void RunnerBase<T>::launch_gather(RState* const* sts) {
+  const size_t n = items.size();
+  if (hgp_n < 4 * n) {
+    if (hgp) CBS_HIP(hipHostFree(hgp));
+    hgp = nullptr;
+    hgp_n = 0;
+    CBS_HIP(hipHostMalloc((void**)&hgp, 4 * n * sizeof(double), hipHostMallocDefault));
+    hgp_n = 4 * n;
+  }
+  launch_gather_state(sts, (int)n, gath.template as<double>(), s);
+  CBS_HIP(hipMemcpyAsync(hgp, gath.p, 4 * n * sizeof(double), hipMemcpyDeviceToHost, s));
+  pending = true;
+}
+
+template <class T>
+void RunnerBase<T>::wait() {
+  if (!pending) return;
+  CBS_HIP(hipStreamSynchronize(s));
+  pending = false;
+  hg.assign(hgp, hgp + 4 * items.size());
+}
+
+template <class T>
 void RunnerBase<T>::collect(std::vector<Finished>& out, bool born) {
   if (hg.size() != 4 * items.size()) return;  // no gather since the last change
   std::vector<Item> keep;
@@ -125,6 +154,26 @@ void RunnerBase<T>::collect(std::vector<Finished>& out, bool born) {
     dirty = true;
   }
   hg.clear();
+}
+
+template <class T>
+void RunnerBase<T>::sort_items() {
+  std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& b) {
+    if (a.it.p->nvox != b.it.p->nvox) return a.it.p->nvox < b.it.p->nvox;
+    return a.key < b.key;
+  });
+}
+
+template <class T>
+std::vector<int> RunnerBase<T>::budgets(int K, bool even_only) const {
+  std::vector<int> b(items.size());
+  for (size_t i = 0; i < items.size(); ++i) {
+    int v = items[i].budget > 0 ? std::min(items[i].budget, K) : K;
+    if (even_only) v = std::max(2, v & ~1);
+    if (i > 0) v = std::min(v, b[i - 1]);
+    b[i] = v;
+  }
+  return b;
 }
 
 template <class T>
@@ -164,6 +213,7 @@ void BornRunner<T>::add(const std::vector<long long>& keys, std::vector<SolveIte
 template <class T>
 void BornRunner<T>::rebuild() {
   this->drop_graph();
+  this->sort_items();
   const int n = (int)this->items.size();
   std::vector<RState*> sts;
   std::vector<double*> parts;
@@ -202,26 +252,35 @@ void BornRunner<T>::rebuild() {
 }
 
 template <class T>
-void BornRunner<T>::chunk() {
+void BornRunner<T>::launch(int kk) {
   const int n = (int)this->items.size();
   if (n == 0) return;
+  if (kk <= 0) kk = this->k;
   if (this->dirty) rebuild();
-  if (this->gexec) {
+  const std::vector<int> bud = this->budgets(kk, true);
+  const int K = bud[0];
+  if (this->gexec && K == this->k && bud[n - 1] == K) {
     CBS_HIP(hipGraphLaunch(this->gexec, this->s));
   } else {
-    for (int j = 0; j < this->k; ++j) {
-      pipe[j & 1].launch(this->s);
+    // iteration j runs the members whose budget exceeds j: a prefix
+    int na = n;
+    PipeLaunch<T> Lp[2] = {pipe[0].L, pipe[1].L};
+    for (int j = 0; j < K; ++j) {
+      int m = na;
+      while (m > 0 && bud[m - 1] <= j) --m;
+      if (m != na) {
+        na = m;
+        Lp[0] = pipe[0].prefix(na);
+        Lp[1] = pipe[1].prefix(na);
+      }
+      launch_pipeline<T>(Lp[j & 1], this->s);
       launch_born_finalize(fin.sts.as<RState*>(), fin.parts.as<double*>(), fin.ntiles.as<int>(),
-                           n, j & 1, this->s);
+                           na, j & 1, this->s);
     }
   }
   CBS_HIP(hipGetLastError());
-  for (auto& m : this->items) m.launched += this->k;
-  this->hg.resize((size_t)n * 4);
-  launch_gather_state(fin.sts.as<RState*>(), n, this->gath.template as<double>(), this->s);
-  CBS_HIP(hipMemcpyAsync(this->hg.data(), this->gath.p, this->hg.size() * sizeof(double),
-                         hipMemcpyDeviceToHost, this->s));
-  CBS_HIP(hipStreamSynchronize(this->s));
+  for (int i = 0; i < n; ++i) this->items[i].launched += bud[i];
+  this->launch_gather(fin.sts.as<RState*>());
 }
 
 // iterations per runner chunk: Born needs an even count (>= check_every)

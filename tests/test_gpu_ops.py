@@ -18,6 +18,10 @@ FP64_TOL = 1e-12
 FP32_TOL = 2e-6
 
 
+# every compile-time specialised length, plus two runtime-plan lengths
+CT_LENGTHS = (8, 16, 32, 64, 96, 128, 160, 192, 224, 256, 288, 320)
+
+
 def _absorbing(o):
     return o.AbsorbingBC(thickness=400e-9, attenuation_thickness=300e-9)
 
@@ -101,3 +105,29 @@ def test_curl_fp64(fm, oracle, grid):
     got = prob.curl_apply(e, oracle.h_scale(lam))
     want = oracle.magnetic_field(e, RES, lam)
     assert rel(got, want) <= FP64_TOL
+
+
+@pytest.mark.parametrize("L", CT_LENGTHS + (200, 250))
+@pytest.mark.parametrize("avg", [False, True])
+def test_green_every_length(fm, oracle, L, avg):
+    """Every specialised transform length on every axis (the other two axes
+    short runtime-plan lengths), FP64 and FP32, with and without the +/-
+    subpixel averaging."""
+    k_bg = 2 * np.pi * 1.33 / 500e-9
+    eps = 0.3 * k_bg ** 2
+    for axis in range(3):
+        shape = [8, 6, 4]
+        shape[axis] = L
+        shape = tuple(shape)
+        print(f"case green every length {shape} avg={avg}", flush=True)
+        gp = oracle.GreenParams(k_square=k_bg ** 2 + 1j * eps,
+                                freq_res=tuple(2 * np.pi / (shape[d] * RES[d]) for d in range(3)),
+                                shifts=(0.25,) * 3 if avg else (0.0,) * 3, use_averaging=avg)
+        spec = fm.GreenSpec(shape=shape, resolution=RES, shifts=gp.shifts, use_averaging=avg,
+                            k_square=gp.k_square, eps_imag=eps)
+        V = np.asfortranarray(np.full(shape, -1j * eps, dtype=np.complex128))
+        u = random_field(shape, 7 + axis)
+        want = oracle.green_conv(u, gp)
+        for prec, tol in (("fp64", FP64_TOL), ("fp32", FP32_TOL)):
+            prob = fm.CBSProblem(V, spec, precision=prec)
+            assert rel(prob.green_apply(u), want) <= tol, (shape, prec)
