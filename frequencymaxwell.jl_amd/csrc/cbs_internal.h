@@ -151,6 +151,9 @@ namespace cbs {
 // 2 = z columns; eb = bytes per complex; S = pipelines (x pass only).
 // 0 when the table has no entry (the heuristic widths apply).
 __host__ __device__ constexpr int tuned_width(int pass, int n, int eb, int S) {
+#ifdef CBS_TUNE_OFF
+  return 0;  // heuristic widths only (reference build for A/B timing)
+#endif
   for (int i = 0; i < kTileTuneN; ++i) {
     const TileTune& t = kTileTune[i];
     if (t.pass == pass && t.n == n && t.eb == eb && (pass != 0 || t.S == S)) return t.p;

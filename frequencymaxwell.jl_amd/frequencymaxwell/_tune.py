@@ -38,7 +38,7 @@ TUNED_H = CSRC / "cbs_tuned.h"
 # (x lines, y columns, z columns) forced in one variant build each
 CANDIDATES = [(1, 4, 2), (2, 8, 4), (4, 16, 8), (8, 32, 16)]
 LENGTHS = (32, 64, 96, 128, 160, 192, 224, 256, 288, 320)
-CROSS = 96  # the two other axes of a tuning grid
+CROSS = 160  # the two other axes of a tuning grid
 KERNELS = ("xfwd", "y_fwd", "z", "y_inv", "xinv")
 
 
