@@ -211,6 +211,15 @@ __host__ __device__ constexpr int zcols(int nz, int eb, bool rt) {
   return tile_p(nz, eb, 3, width_z(nz, eb, rt), rt, 0);
 }
 __host__ __device__ constexpr int xpad_stride(int q) { return q + 1; }
+// every pass tile starts its LDS with the twiddles of its axis (n complex,
+// 16-byte aligned); the tile data follows
+__host__ __device__ constexpr int tw_bytes(int n, int eb) { return (n * eb + 15) / 16 * 16; }
+// z tiles also stage the k values of the tile (n along z, P along x, one y)
+// as doubles behind the spectra
+__host__ __device__ constexpr int zk_offset(int nz, int P, int eb, bool rt) {
+  return (((rt ? 2 : 1) * nz * 3 * P * eb) + 15) / 16 * 16;
+}
+__host__ __device__ constexpr int zk_count(int nz, int P) { return nz + P + 1; }
 
 // ---------------------------------------------------------------------------
 // device descriptors
@@ -316,15 +325,16 @@ inline int tile_lds_bytes(const DevProb<T>& p, int pass) {
     case 3: {
       const bool rt = p.rt[0] != 0;
       const int P = xlines(nx, eb, p.S, rt);
-      return (rt ? 2 : 1) * nx * xpad_stride(3 * (rt ? p.S : 1) * P) * eb;
+      return tw_bytes(nx, eb) + (rt ? 2 : 1) * nx * xpad_stride(3 * (rt ? p.S : 1) * P) * eb;
     }
     case 1: {
       const bool rt = p.rt[1] != 0;
-      return (rt ? 2 : 1) * ny * ycols(ny, eb, rt) * eb;
+      return tw_bytes(ny, eb) + (rt ? 2 : 1) * ny * ycols(ny, eb, rt) * eb;
     }
     default: {
       const bool rt = p.rt[2] != 0;
-      return (rt ? 2 : 1) * nz * 3 * zcols(nz, eb, rt) * eb;
+      const int P = zcols(nz, eb, rt);
+      return tw_bytes(nz, eb) + zk_offset(nz, P, eb, rt) + zk_count(nz, P) * (int)sizeof(double);
     }
   }
 }

@@ -15,6 +15,19 @@ namespace cbs {
 
 extern __shared__ __align__(16) unsigned char g_smem[];
 
+// twiddles of one axis into the head of the tile LDS (read after the tile's
+// first barrier); returns the start of the tile data
+template <class T>
+__device__ __forceinline__ Cx<T>* stage_twiddles(const Cx<T>* __restrict__ tw, int n, int tid) {
+  Cx<T>* t = reinterpret_cast<Cx<T>*>(g_smem);
+  for (int i = tid; i < n; i += kThreads) t[i] = tw[i];
+  return reinterpret_cast<Cx<T>*>(g_smem + tw_bytes(n, (int)sizeof(Cx<T>)));
+}
+template <class T>
+__device__ __forceinline__ const Cx<T>* lds_twiddles() {
+  return reinterpret_cast<const Cx<T>*>(g_smem);
+}
+
 __device__ __forceinline__ int find_prob(const int* __restrict__ bstart, int nprob, int blk) {
   int lo = 0, hi = nprob - 1;
   while (lo < hi) {
@@ -52,7 +65,7 @@ __device__ __forceinline__ void xfwd_tile(const DevProb<T>& pr, int tile, int ti
   constexpr int eb = (int)sizeof(Cx<T>);
   const int n = CT ? NC : pr.n[0];
   const int P = xlines(n, eb, S, !CT);
-  Cx<T>* lds = reinterpret_cast<Cx<T>*>(g_smem);
+  Cx<T>* lds = stage_twiddles<T>(pr.tw[0], n, tid);
   const int ny = pr.n[1];
   const int nlines = ny * pr.n[2];
   const int l0 = tile * P;
@@ -101,7 +114,7 @@ __device__ __forceinline__ void xfwd_tile(const DevProb<T>& pr, int tile, int ti
         }
       }
       __syncthreads();
-      fft_ct<T, NC, Qc, QSc, -1>(lds, pr.tw[0], tid);
+      fft_ct<T, NC, Qc, QSc, -1>(lds, lds_twiddles<T>(), tid);
 #pragma unroll
       for (int k = 0; k < E; ++k) {
         const int e = tid + k * kThreads;
@@ -145,7 +158,7 @@ __device__ __forceinline__ void xfwd_tile(const DevProb<T>& pr, int tile, int ti
       }
     }
     __syncthreads();
-    Cx<T>* res = fft_rt<T, -1>(lds, lds + n * QS, n, Q, QS, pr.plan[0], pr.tw[0], tid);
+    Cx<T>* res = fft_rt<T, -1>(lds, lds + n * QS, n, Q, QS, pr.plan[0], lds_twiddles<T>(), tid);
     for (int e = tid; e < P * n; e += kThreads) {
       const int m = e % n, p = e / n;
       if (p < np) {
@@ -226,7 +239,7 @@ __device__ __forceinline__ void ypass_tile(const DevProb<T>& pr, int tile, int t
   constexpr int eb = (int)sizeof(Cx<T>);
   const int n = CT ? NC : pr.n[1];
   const int P = ycols(n, eb, !CT);
-  Cx<T>* lds = reinterpret_cast<Cx<T>*>(g_smem);
+  Cx<T>* lds = stage_twiddles<T>(pr.tw[1], n, tid);
   const int nx = pr.n[0];
   const int nxb = (nx + P - 1) / P;
   const int xb = tile % nxb;
@@ -241,9 +254,9 @@ __device__ __forceinline__ void ypass_tile(const DevProb<T>& pr, int tile, int t
   Cx<T>* res = lds;
   if constexpr (CT) {
     constexpr int Pc = ycols(CT ? NC : 8, eb, false);
-    fft_ct<T, NC, Pc, Pc, DIR>(lds, pr.tw[1], tid);
+    fft_ct<T, NC, Pc, Pc, DIR>(lds, lds_twiddles<T>(), tid);
   } else {
-    res = fft_rt<T, DIR>(lds, lds + n * P, n, P, P, pr.plan[1], pr.tw[1], tid);
+    res = fft_rt<T, DIR>(lds, lds + n * P, n, P, P, pr.plan[1], lds_twiddles<T>(), tid);
   }
   tile_store(res, base, P, npx, n, pr.ws[0], P, 0, tid);
 }
@@ -252,31 +265,63 @@ __device__ __forceinline__ void ypass_tile(const DevProb<T>& pr, int tile, int t
 // sweep C: z forward transform, k-space multiply, z inverse transform
 // ---------------------------------------------------------------------------
 template <class T>
-__device__ __forceinline__ void kspace_op(const DevProb<T>& pr, int s, int x, int y, int z,
+__device__ __forceinline__ void kspace_op(const DevProb<T>& pr, double kxd, double kyd, double kzd,
                                           Cx<T>& e0, Cx<T>& e1, Cx<T>& e2) {
-  const Cx<double> E0 = cvt<double>(e0), E1 = cvt<double>(e1), E2 = cvt<double>(e2);
-  const Cx<double> zs = mk<double>(pr.zs_re, pr.zs_im);
-  if (pr.zop == ZOP_GREEN) {
-    // G(k) = (I - k k^T / k_c^2) / (|k|^2 - k_c^2)   (dyadic_green.jl:203-219)
-    const double kx = pr.kk[0][s][x], ky = pr.kk[1][s][y], kz = pr.kk[2][s][z];
-    const Cx<double> ik2c = mk<double>(pr.ik2c_re, pr.ik2c_im);
-    const Cx<double> kdote = ik2c * (kx * E0 + ky * E1 + kz * E2);
-    const double dre = kx * kx + ky * ky + kz * kz - pr.k2c_re, dim = -pr.k2c_im;
-    const double inv = 1.0 / (dre * dre + dim * dim);
-    // a = zs / den
-    const Cx<double> a = zs * mk<double>(dre * inv, -dim * inv);
-    e0 = cvt<T>(a * (E0 - kx * kdote));
-    e1 = cvt<T>(a * (E1 - ky * kdote));
-    e2 = cvt<T>(a * (E2 - kz * kdote));
+  if constexpr (sizeof(T) == 4) {
+    // complex FP32: the Green tensor and the curl in single precision; the
+    // reciprocal of |k|^2 - k_c^2 by Smith's method (no squares of k^2)
+    const float kx = (float)kxd, ky = (float)kyd, kz = (float)kzd;
+    const Cx<float> zs = mk<float>((float)pr.zs_re, (float)pr.zs_im);
+    if (pr.zop == ZOP_GREEN) {
+      // G(k) = (I - k k^T / k_c^2) / (|k|^2 - k_c^2)   (dyadic_green.jl:203-219)
+      const Cx<float> ik2c = mk<float>((float)pr.ik2c_re, (float)pr.ik2c_im);
+      const Cx<float> kdote = ik2c * (kx * e0 + ky * e1 + kz * e2);
+      const float dre = kx * kx + ky * ky + kz * kz - (float)pr.k2c_re;
+      const float dim = -(float)pr.k2c_im;
+      Cx<float> inv;
+      if (fabsf(dre) >= fabsf(dim)) {
+        const float r = dim / dre, q = 1.0f / (dre + dim * r);
+        inv = mk<float>(q, -r * q);
+      } else {
+        const float r = dre / dim, q = 1.0f / (dre * r + dim);
+        inv = mk<float>(r * q, -q);
+      }
+      const Cx<float> a = zs * inv;
+      const Cx<float> f0 = e0 - kx * kdote, f1 = e1 - ky * kdote, f2 = e2 - kz * kdote;
+      e0 = a * f0;
+      e1 = a * f1;
+      e2 = a * f2;
+    } else {
+      // i k x E   (curl.jl:88-98)
+      const Cx<float> d0 = mul_i(ky * e2 - kz * e1);
+      const Cx<float> d1 = mul_i(kz * e0 - kx * e2);
+      const Cx<float> d2 = mul_i(kx * e1 - ky * e0);
+      e0 = zs * d0;
+      e1 = zs * d1;
+      e2 = zs * d2;
+    }
   } else {
-    // i k x E   (curl.jl:88-98)
-    const double kx = pr.kc[0][x], ky = pr.kc[1][y], kz = pr.kc[2][z];
-    const Cx<double> d0 = mul_i(ky * E2 - kz * E1);
-    const Cx<double> d1 = mul_i(kz * E0 - kx * E2);
-    const Cx<double> d2 = mul_i(kx * E1 - ky * E0);
-    e0 = cvt<T>(zs * d0);
-    e1 = cvt<T>(zs * d1);
-    e2 = cvt<T>(zs * d2);
+    const Cx<double> E0 = e0, E1 = e1, E2 = e2;
+    const Cx<double> zs = mk<double>(pr.zs_re, pr.zs_im);
+    const double kx = kxd, ky = kyd, kz = kzd;
+    if (pr.zop == ZOP_GREEN) {
+      const Cx<double> ik2c = mk<double>(pr.ik2c_re, pr.ik2c_im);
+      const Cx<double> kdote = ik2c * (kx * E0 + ky * E1 + kz * E2);
+      const double dre = kx * kx + ky * ky + kz * kz - pr.k2c_re, dim = -pr.k2c_im;
+      const double inv = 1.0 / (dre * dre + dim * dim);
+      // a = zs / den
+      const Cx<double> a = zs * mk<double>(dre * inv, -dim * inv);
+      e0 = a * (E0 - kx * kdote);
+      e1 = a * (E1 - ky * kdote);
+      e2 = a * (E2 - kz * kdote);
+    } else {
+      const Cx<double> d0 = mul_i(ky * E2 - kz * E1);
+      const Cx<double> d1 = mul_i(kz * E0 - kx * E2);
+      const Cx<double> d2 = mul_i(kx * E1 - ky * E0);
+      e0 = zs * d0;
+      e1 = zs * d1;
+      e2 = zs * d2;
+    }
   }
 }
 
@@ -287,7 +332,7 @@ __device__ __forceinline__ void zpass_tile(const DevProb<T>& pr, int tile, int t
   const int n = CT ? NC : pr.n[2];
   const int P = zcols(n, eb, !CT);
   const int Q = 3 * P;
-  Cx<T>* lds = reinterpret_cast<Cx<T>*>(g_smem);
+  Cx<T>* lds = stage_twiddles<T>(pr.tw[2], n, tid);
   const int nx = pr.n[0], ny = pr.n[1];
   const int nxb = (nx + P - 1) / P;
   const int xb = tile % nxb;
@@ -300,20 +345,35 @@ __device__ __forceinline__ void zpass_tile(const DevProb<T>& pr, int tile, int t
 #pragma unroll
   for (int c = 0; c < 3; ++c)
     tile_load(lds, base + c * pr.ws[2], P, npx, n, pr.ws[1], Q, c * P, tid);
+  // k values of this tile: kt[m] along z, kt[n + p] along x, kt[n + P] at y
+  double* kt = reinterpret_cast<double*>(reinterpret_cast<unsigned char*>(lds) + zk_offset(n, P, eb, !CT));
+  {
+    const bool green = pr.zop == ZOP_GREEN;
+    const double* tz = green ? pr.kk[2][s] : pr.kc[2];
+    const double* tx = green ? pr.kk[0][s] : pr.kc[0];
+    const double* ty = green ? pr.kk[1][s] : pr.kc[1];
+    for (int i = tid; i < zk_count(n, P); i += kThreads) {
+      double v = 0.0;
+      if (i < n) v = tz[i];
+      else if (i < n + P) v = (i - n < npx) ? tx[x0 + i - n] : 0.0;
+      else v = ty[y];
+      kt[i] = v;
+    }
+  }
   __syncthreads();
   Cx<T>* res = lds;
   if constexpr (CT) {
     constexpr int Pc = zcols(CT ? NC : 8, eb, false);
-    fft_ct<T, NC, 3 * Pc, 3 * Pc, -1>(lds, pr.tw[2], tid);
+    fft_ct<T, NC, 3 * Pc, 3 * Pc, -1>(lds, lds_twiddles<T>(), tid);
   } else {
-    res = fft_rt<T, -1>(lds, lds + n * Q, n, Q, Q, pr.plan[2], pr.tw[2], tid);
+    res = fft_rt<T, -1>(lds, lds + n * Q, n, Q, Q, pr.plan[2], lds_twiddles<T>(), tid);
   }
   for (int e = tid; e < P * n; e += kThreads) {
     const int p = e % P, m = e / P;
     if (p < npx) {
       Cx<T>* r = res + m * Q + p;
       Cx<T> e0 = r[0], e1 = r[P], e2 = r[2 * P];
-      kspace_op(pr, s, x0 + p, y, m, e0, e1, e2);
+      kspace_op(pr, kt[n + p], kt[n + P], kt[m], e0, e1, e2);
       r[0] = e0;
       r[P] = e1;
       r[2 * P] = e2;
@@ -322,10 +382,10 @@ __device__ __forceinline__ void zpass_tile(const DevProb<T>& pr, int tile, int t
   __syncthreads();
   if constexpr (CT) {
     constexpr int Pc = zcols(CT ? NC : 8, eb, false);
-    fft_ct<T, NC, 3 * Pc, 3 * Pc, 1>(lds, pr.tw[2], tid);
+    fft_ct<T, NC, 3 * Pc, 3 * Pc, 1>(lds, lds_twiddles<T>(), tid);
   } else {
     Cx<T>* other = res == lds ? lds + n * Q : lds;
-    res = fft_rt<T, 1>(res, other, n, Q, Q, pr.plan[2], pr.tw[2], tid);
+    res = fft_rt<T, 1>(res, other, n, Q, Q, pr.plan[2], lds_twiddles<T>(), tid);
   }
 #pragma unroll
   for (int c = 0; c < 3; ++c)
@@ -398,7 +458,7 @@ __device__ __forceinline__ void xinv_tile(const DevProb<T>& pr, int tile, int ti
   constexpr int eb = (int)sizeof(Cx<T>);
   const int n = CT ? NC : pr.n[0];
   const int P = xlines(n, eb, S, !CT);
-  Cx<T>* lds = reinterpret_cast<Cx<T>*>(g_smem);
+  Cx<T>* lds = stage_twiddles<T>(pr.tw[0], n, tid);
   const int ny = pr.n[1];
   const int nlines = ny * pr.n[2];
   const int l0 = tile * P;
@@ -437,7 +497,7 @@ __device__ __forceinline__ void xinv_tile(const DevProb<T>& pr, int tile, int ti
         }
       }
       __syncthreads();
-      fft_ct<T, NC, Qc, QSc, 1>(lds, pr.tw[0], tid);
+      fft_ct<T, NC, Qc, QSc, 1>(lds, lds_twiddles<T>(), tid);
 #pragma unroll
       for (int k = 0; k < E; ++k) {
         const int e = tid + k * kThreads;
@@ -479,7 +539,7 @@ __device__ __forceinline__ void xinv_tile(const DevProb<T>& pr, int tile, int ti
         lds[m * QS + q * P + p] = p < np ? pr.w[q * pr.ws[2] + wo] : mk<T>(T(0), T(0));
     }
     __syncthreads();
-    Cx<T>* res = fft_rt<T, 1>(lds, lds + n * QS, n, Q, QS, pr.plan[0], pr.tw[0], tid);
+    Cx<T>* res = fft_rt<T, 1>(lds, lds + n * QS, n, Q, QS, pr.plan[0], lds_twiddles<T>(), tid);
     for (int e = tid; e < P * n; e += kThreads) {
       const int m = e % n, p = e / n;
       if (p >= np) continue;
