@@ -210,7 +210,11 @@ __host__ __device__ constexpr int ycols(int ny, int eb, bool rt) {
 __host__ __device__ constexpr int zcols(int nz, int eb, bool rt) {
   return tile_p(nz, eb, 3, width_z(nz, eb, rt), rt, 0);
 }
-__host__ __device__ constexpr int xpad_stride(int q) { return q + 1; }
+#ifndef CBS_XPAD
+#define CBS_XPAD 1
+#endif
+// padded LDS row of the x tiles (q pencils + CBS_XPAD complex)
+__host__ __device__ constexpr int xpad_stride(int q) { return q + CBS_XPAD; }
 // every pass tile starts its LDS with the twiddles of its axis (n complex,
 // 16-byte aligned); the tile data follows
 __host__ __device__ constexpr int tw_bytes(int n, int eb) { return (n * eb + 15) / 16 * 16; }

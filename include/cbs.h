@@ -175,9 +175,11 @@ cbs_status cbs_solve_batch(cbs_ctx *ctx, int32_t n, cbs_problem *const *probs,
                            const cbs_solve_opts *opts, cbs_solve_stats *stats);
 
 /* Measurement aid -- no counterpart in the reference; for benchmarks that
- * need per-kernel durations on the library's own stream.  Runs `reps` Born
- * sweeps of the resident source term (cbs_problem_set_source first) in the
- * given precision and times each of the five sweep kernels with HIP events.
+ * need per-kernel durations on the library's own stream.  Runs one Born
+ * sweep of the resident source term (cbs_problem_set_source first) in the
+ * given precision, then launches each of the five sweep kernels `reps` times
+ * back to back between two HIP events (the problem's work buffers are left
+ * overwritten; a later solve starts afresh).
  * ms[k]: average duration of one launch of kernel k (0 x forward + V x,
  * 1 y forward, 2 z + Green, 3 y inverse, 4 x inverse + Born update);
  * bytes[k]: algorithmic HBM bytes of one launch of kernel k. */
