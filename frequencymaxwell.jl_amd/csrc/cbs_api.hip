@@ -6,6 +6,7 @@
 #include <string>
 
 #include "cbs_engine.h"
+#include "cbs_scene.h"
 #include "cbs_serve.h"
 #include "cbs_solve.h"
 
@@ -444,6 +445,86 @@ cbs_status cbs_stream_copy(cbs_ctx* ctx, uint64_t bytes, int32_t reps, double* m
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     *ms = t / reps;
+  });
+}
+
+cbs_status cbs_scene_build(cbs_ctx* ctx, const cbs_scene_desc* scene, const double* eps,
+                           int32_t nsrc, const cbs_plane_wave* srcs, double* V, double* S,
+                           cbs_scene_info* info) {
+  return guarded([&] {
+    CBS_REQUIRE(ctx && scene && eps, "NULL argument");
+    set_device(ctx->impl);
+    hipStream_t s = ctx->impl.stream;
+    Scene sc;
+    cbs_problem_desc pd;
+    scene_setup(sc, &ctx->impl, *scene, eps, nsrc, srcs, pd, s);
+    DevBuf Vd, Sd;
+    Vd.alloc(&ctx->impl, (size_t)sc.nvox * sizeof(Cx<double>));
+    scene_potential(sc, Vd.as<Cx<double>>(), s);
+    if (S) {
+      Sd.alloc(&ctx->impl, (size_t)sc.nvox * 3 * sizeof(Cx<double>));
+      scene_source(sc, Vd.as<Cx<double>>(), Sd.as<Cx<double>>(), s);
+      CBS_HIP(hipMemcpyAsync(S, Sd.p, (size_t)sc.nvox * 3 * sizeof(Cx<double>),
+                             hipMemcpyDeviceToHost, s));
+    }
+    if (V)
+      CBS_HIP(hipMemcpyAsync(V, Vd.p, (size_t)sc.nvox * sizeof(Cx<double>), hipMemcpyDeviceToHost,
+                             s));
+    CBS_HIP(hipStreamSynchronize(s));
+    scene_info(sc, info);
+  });
+}
+
+cbs_status cbs_solve_scene(cbs_ctx* ctx, const cbs_scene_desc* scene, const double* eps,
+                           int32_t nsrc, const cbs_plane_wave* srcs, const cbs_solve_opts* opts,
+                           double* E_tot, double* H_tot, cbs_solve_stats* stats,
+                           cbs_scene_info* info) {
+  return guarded([&] {
+    CBS_REQUIRE(ctx && scene && eps && opts && E_tot, "NULL argument");
+    set_device(ctx->impl);
+    Ctx& c = ctx->impl;
+    hipStream_t s = c.stream;
+    Scene sc;
+    cbs_problem_desc pd;
+    scene_setup(sc, &c, *scene, eps, nsrc, srcs, pd, s);
+    validate_desc(pd);
+    Problem p;
+    problem_alloc(p, &c, pd, false);
+    scene_potential(sc, p.V64.as<Cx<double>>(), s);
+    cbs_solve_opts o = *opts;
+    if (o.maxiters <= 0) o.maxiters = sc.maxiters;
+    // -i lambda / (2 pi 377)  (convergent_born.jl:797)
+    o.h_scale_re = 0.0;
+    o.h_scale_im = -scene->wavelength / (2.0 * M_PI * 377.0);
+    if (o.precision != CBS_FP64) {
+      fill_tables<float>(p);
+      ensure_v32(p, s);
+    }
+    const size_t fb = (size_t)sc.nvox * 3 * sizeof(Cx<double>);
+    DevBuf Sd, Ed, Hd, Eo, Ho;
+    Sd.alloc(&c, fb);
+    Ed.alloc(&c, fb);
+    scene_source(sc, p.V64.as<Cx<double>>(), Sd.as<Cx<double>>(), s);
+    std::vector<RhsIO> io{RhsIO{&p, Sd.as<Cx<double>>(), Ed.as<Cx<double>>()}};
+    std::vector<cbs_solve_stats> st;
+    solve_set(&c, io, o, st, s);
+    Sd.release();
+    const bool want_h = H_tot && o.compute_h;
+    if (want_h) {
+      Hd.alloc(&c, fb);
+      curl_apply_dev<double>(p, Ed.as<Cx<double>>(), Hd.as<Cx<double>>(), o.h_scale_re,
+                             o.h_scale_im, s);
+    }
+    const size_t ob = (size_t)sc.nroi * 3 * sizeof(Cx<double>);
+    Eo.alloc(&c, ob);
+    if (want_h) Ho.alloc(&c, ob);
+    scene_finish(sc, Ed.as<Cx<double>>(), want_h ? Hd.as<Cx<double>>() : nullptr,
+                 Eo.as<Cx<double>>(), want_h ? Ho.as<Cx<double>>() : nullptr, s);
+    CBS_HIP(hipMemcpyAsync(E_tot, Eo.p, ob, hipMemcpyDeviceToHost, s));
+    if (want_h) CBS_HIP(hipMemcpyAsync(H_tot, Ho.p, ob, hipMemcpyDeviceToHost, s));
+    CBS_HIP(hipStreamSynchronize(s));
+    if (stats) *stats = st[0];
+    scene_info(sc, info);
   });
 }
 

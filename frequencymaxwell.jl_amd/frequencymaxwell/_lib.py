@@ -32,7 +32,8 @@ EXPORTED = (
     "cbs_precond_apply", "cbs_curl_apply", "cbs_solve", "cbs_problem_set_source",
     "cbs_problem_run", "cbs_problem_get_fields", "cbs_solve_multi", "cbs_solve_batch",
     "cbs_request_footprint", "cbs_server_start", "cbs_server_stop", "cbs_submit", "cbs_poll",
-    "cbs_wait", "cbs_cancel", "cbs_time_sweeps", "cbs_stream_copy",
+    "cbs_wait", "cbs_cancel", "cbs_time_sweeps", "cbs_stream_copy", "cbs_scene_build",
+    "cbs_solve_scene",
 )
 
 
@@ -95,6 +96,43 @@ class Request(C.Structure):
     ]
 
 
+class SceneDesc(C.Structure):
+    _fields_ = [
+        ("grid", C.c_int64 * 3),
+        ("res", C.c_double * 3),
+        ("pad", C.c_int32 * 3),
+        ("att_len", C.c_int32 * 3),
+        ("window", C.POINTER(C.c_double) * 3),
+        ("shift", C.c_double * 3),
+        ("averaging", C.c_int32),
+        ("eps_bg", C.c_double),
+        ("wavelength", C.c_double),
+        ("iterations_max", C.c_int32),
+        ("precision", C.c_int32),
+    ]
+
+
+class PlaneWave(C.Structure):
+    _fields_ = [
+        ("k_vec", C.c_double * 3),
+        ("e_perp", C.c_double * 6),
+        ("h_perp", C.c_double * 6),
+        ("amplitude", C.c_double),
+        ("phase", C.c_double),
+    ]
+
+
+class SceneInfo(C.Structure):
+    _fields_ = [
+        ("n", C.c_int64 * 3),
+        ("eps_imag", C.c_double),
+        ("k2_re", C.c_double),
+        ("k2_im", C.c_double),
+        ("bornmax", C.c_int32),
+        ("maxiters", C.c_int32),
+    ]
+
+
 class CBSError(RuntimeError):
     """A HIP-library failure (status code in ``.code``)."""
 
@@ -142,6 +180,11 @@ def _declare(lib):
         "cbs_cancel": (s, [_P, C.c_uint64]),
         "cbs_time_sweeps": (s, [_P, C.c_int32, C.c_int32, _D, _D]),
         "cbs_stream_copy": (s, [_P, C.c_uint64, C.c_int32, _D]),
+        "cbs_scene_build": (s, [_P, C.POINTER(SceneDesc), _D, C.c_int32, C.POINTER(PlaneWave),
+                                _D, _D, C.POINTER(SceneInfo)]),
+        "cbs_solve_scene": (s, [_P, C.POINTER(SceneDesc), _D, C.c_int32, C.POINTER(PlaneWave),
+                                C.POINTER(SolveOpts), _D, _D, C.POINTER(SolveStats),
+                                C.POINTER(SceneInfo)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

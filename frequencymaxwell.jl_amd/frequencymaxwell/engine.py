@@ -298,3 +298,40 @@ class CBSServer:
 
     def __exit__(self, *exc):
         self.stop()
+
+
+# -- device-side setup and teardown (cbs_scene_build / cbs_solve_scene) ------
+def scene_build(desc: L.SceneDesc, eps, waves, ctx=None):
+    """The padded potential V and source term S as the GPU builds them from the
+    ROI permittivity; returns (V, S, SceneInfo)."""
+    ctx = ctx or L.default_context()
+    eps = L.as_field(eps, tuple(desc.grid))
+    n = _padded(desc)
+    V = np.empty(n, dtype=np.complex128, order="F")
+    S = np.empty(n + (3,), dtype=np.complex128, order="F")
+    info = L.SceneInfo()
+    arr = (L.PlaneWave * len(waves))(*waves)
+    L.check(L.load().cbs_scene_build(ctx.handle, C.byref(desc), L.dptr(eps), len(waves), arr,
+                                     L.dptr(V), L.dptr(S), C.byref(info)))
+    return V, S, info
+
+
+def solve_scene(desc: L.SceneDesc, eps, waves, opts: L.SolveOpts, want_h: bool = True, ctx=None):
+    """The whole solve on the GPU: returns the total fields cropped to the ROI
+    (E, H or None), the solve statistics and the SceneInfo."""
+    ctx = ctx or L.default_context()
+    g = tuple(desc.grid)
+    eps = L.as_field(eps, g)
+    E = np.empty(g + (3,), dtype=np.complex128, order="F")
+    H = np.empty(g + (3,), dtype=np.complex128, order="F") if want_h else None
+    st = L.SolveStats()
+    info = L.SceneInfo()
+    arr = (L.PlaneWave * len(waves))(*waves)
+    L.check(L.load().cbs_solve_scene(ctx.handle, C.byref(desc), L.dptr(eps), len(waves), arr,
+                                     C.byref(opts), L.dptr(E), L.dptr(H) if H is not None else None,
+                                     C.byref(st), C.byref(info)))
+    return E, H, st, info
+
+
+def _padded(desc) -> tuple:
+    return tuple(int(desc.grid[d]) + 2 * int(desc.pad[d]) for d in range(3))

@@ -39,14 +39,16 @@ class ConvergentBornSolver:
     ``precision`` ("fp64" | "fp32" | "fp32_refine"), ``stop_rule``
     ("krylov": reltol on the initial residual plus abstol = tol*1e-2 as
     LinearSolve passes it; "rhs": tol * ||gamma b||), ``maxiters`` (default
-    max(2 Bornmax, 100)), ``gmres_restart``, ``use_graph``, ``check_every``.
+    max(2 Bornmax, 100)), ``gmres_restart``, ``use_graph``, ``check_every``,
+    ``device_setup`` (potential, incident field, source term and the final
+    incident add + crop on the GPU; False keeps them on the host).
     """
 
     def __init__(self, *, resolution, grid_size, boundary_conditions, iterations_max: int = -1,
                  tolerance: float = 1e-6, linear_solver="gmres", preconditioner: str = "none",
                  device: str = "mi355x", precision: str = "fp64", stop_rule: str = "krylov",
                  maxiters: int | None = None, gmres_restart: int = 0, use_graph: bool = False,
-                 check_every: int = 1):
+                 check_every: int = 1, device_setup: bool = True):
         resolution = tuple(float(r) for r in resolution)
         grid_size = tuple(int(g) for g in grid_size)
         if len(resolution) != 3 or len(grid_size) != 3:
@@ -81,6 +83,7 @@ class ConvergentBornSolver:
         self.gmres_restart = int(gmres_restart)
         self.use_graph = bool(use_graph)
         self.check_every = int(check_every)
+        self.device_setup = bool(device_setup)
         # state
         self.Green_function: DyadicGreen | None = None
         self.Bornmax = 0
@@ -191,21 +194,37 @@ def _pad_array(arr: np.ndarray, pad) -> np.ndarray:
     return out
 
 
-def _apply_attenuation_to_potential(solver: ConvergentBornSolver, potential: np.ndarray) -> None:
-    """:893-944 -- multiply V by a 1-D window along every absorbing axis."""
-    roi = solver.compute_ROI()
+def attenuation_windows(solver: ConvergentBornSolver):
+    """Per axis, the sharpness-blended attenuation window of
+    _apply_attenuation_to_potential! (:893-944): length L = min(pad,
+    round(attenuation_thickness / (2 dx))); empty where the axis does not attenuate."""
+    out = []
     for dim in range(3):
         bc = solver.boundary_conditions[dim]
         if not requires_padding(bc) or not isinstance(bc, AbsorbingBoundaryCondition):
+            out.append(np.zeros(0))
             continue
         max_L = padding_pixels(bc, solver.resolution[dim])
+        if max_L == 0:
+            out.append(np.zeros(0))
+            continue
         att = int(np.rint(bc.attenuation_thickness / (solver.resolution[dim] * 2)))
         L_ = min(max_L, att)
-        if max_L == 0:
+        out.append(create_attenuation_window(bc.profile, L_) * bc.sharpness + (1 - bc.sharpness))
+    return out
+
+
+def _apply_attenuation_to_potential(solver: ConvergentBornSolver, potential: np.ndarray) -> None:
+    """:893-944 -- multiply V by a 1-D window along every absorbing axis:
+    filter = [w; ones; reverse(w)] over the padded axis."""
+    for dim, w in enumerate(attenuation_windows(solver)):
+        bc = solver.boundary_conditions[dim]
+        if not requires_padding(bc) or not isinstance(bc, AbsorbingBoundaryCondition):
             continue
-        w = create_attenuation_window(bc.profile, L_) * bc.sharpness + (1 - bc.sharpness)
-        roi_size = roi[2 * dim + 1] - roi[2 * dim] + 1
-        filt = np.concatenate([w, np.ones(roi_size + 2 * (max_L - L_)), w[::-1]])
+        if padding_pixels(bc, solver.resolution[dim]) == 0:
+            continue
+        n = potential.shape[dim]
+        filt = np.concatenate([w, np.ones(n - 2 * w.size), w[::-1]])
         shape = [1, 1, 1]
         shape[dim] = filt.size
         potential *= filt.reshape(shape)
@@ -263,6 +282,56 @@ def _prepare(solver, sources, medium):
     return lam, pot, eps_imag, bornmax, problem
 
 
+def scene(solver: ConvergentBornSolver, sources, medium: Medium, precision=None):
+    """The scene description of solve(solver, sources, medium) for the device
+    setup: ROI permittivity, padding and attenuation windows, Green shifts and
+    the plane waves (unit vectors from plane_wave.jl:218-232).  Returns
+    (SceneDesc, eps, [PlaneWave], keep) -- keep holds the window arrays the
+    description points to."""
+    from .engine import precision_code
+    from .sources import AbstractCurrentSource, plane_wave_vectors
+
+    if grid_size(medium) != solver.grid_size:  # :572-573
+        raise ValueError(f"medium grid_size {grid_size(medium)} must match solver grid_size {solver.grid_size}")
+    lam = source_wavelength(sources)
+    eps_bg = permittivity_bg(medium)
+    srcs = [sources] if isinstance(sources, AbstractCurrentSource) else list(sources)
+    if not srcs:
+        raise ValueError("sources vector cannot be empty")
+    for s_ in srcs[1:]:
+        if not math.isclose(s_.wavelength, lam, rel_tol=1e-10, abs_tol=0.0):
+            raise ValueError("All sources must have the same wavelength for coherent interference")
+    d = L.SceneDesc()
+    wins = [np.ascontiguousarray(w, dtype=np.float64) for w in attenuation_windows(solver)]
+    pad = solver.padding
+    for k in range(3):
+        d.grid[k] = solver.grid_size[k]
+        d.res[k] = solver.resolution[k]
+        d.pad[k] = pad[k]
+        d.att_len[k] = wins[k].size
+        d.window[k] = wins[k].ctypes.data_as(L._D) if wins[k].size else None
+        d.shift[k] = subpixel_shift(solver.boundary_conditions[k])
+    d.averaging = int(any(requires_averaging(b) for b in solver.boundary_conditions))
+    d.eps_bg = float(eps_bg)
+    d.wavelength = float(lam)
+    d.iterations_max = int(solver.iterations_max)
+    prec = precision if precision is not None else solver.precision
+    d.precision = L.CBS_FP64 if precision_code(prec) == L.CBS_FP64 else L.CBS_FP32
+    waves = []
+    for src in srcs:
+        k_vec, e_perp, h_perp = plane_wave_vectors(src, eps_bg)
+        w = L.PlaneWave()
+        for c in range(3):
+            w.k_vec[c] = float(k_vec[c])
+            w.e_perp[2 * c], w.e_perp[2 * c + 1] = complex(e_perp[c]).real, complex(e_perp[c]).imag
+            w.h_perp[2 * c], w.h_perp[2 * c + 1] = complex(h_perp[c]).real, complex(h_perp[c]).imag
+        w.amplitude = float(src.amplitude)
+        w.phase = float(src.phase)
+        waves.append(w)
+    eps = np.asfortranarray(permittivity(medium), dtype=np.complex128)
+    return d, eps, waves, wins
+
+
 def _record(solver, st: L.SolveStats):
     solver.iteration_count = int(st.iters)
     solver.residual_history.append(float(st.resid))
@@ -272,7 +341,28 @@ def _record(solver, st: L.SolveStats):
 
 
 def solve(solver: ConvergentBornSolver, sources, medium: Medium) -> ElectromagneticField:
-    """solve(solver, sources, medium) -> total field cropped to the ROI (:565-600)."""
+    """solve(solver, sources, medium) -> total field cropped to the ROI (:565-600).
+
+    With ``solver.device_setup`` (the default) the potential, the incident
+    field, the source term, H and the incident add + crop run on the GPU
+    (cbs_solve_scene); otherwise they run on the host as in the reference and
+    only the operator layer and the solve go to the GPU."""
+    if solver.device_setup:
+        from .engine import solve_scene
+
+        d, eps, waves, keep = scene(solver, sources, medium)
+        lam = d.wavelength
+        opts = _solve_opts(solver, 0, lam)
+        if solver.maxiters is None:
+            opts.maxiters = 0  # the library applies max(2 Bornmax, 100)
+        E, H, st, info = solve_scene(d, eps, waves, opts)
+        del keep
+        solver.Green_function = None
+        solver.Bornmax = int(info.bornmax)
+        solver.iteration_count = 0
+        solver.residual_history = []
+        _record(solver, st)
+        return ElectromagneticField(E, H, solver.resolution, lam)
     lam, pot, eps_imag, bornmax, problem = _prepare(solver, sources, medium)
     inc = generate_incident_field(sources, solver, permittivity_bg=permittivity_bg(medium))
     S = pot[..., None] * inc.E + complex(0, eps_imag) * inc.E  # :585-586

@@ -193,6 +193,62 @@ cbs_status cbs_time_sweeps(cbs_problem *p, int32_t precision, int32_t reps, doub
  * byte counters. */
 cbs_status cbs_stream_copy(cbs_ctx *ctx, uint64_t bytes, int32_t reps, double *ms);
 
+/* ---- device-side setup and teardown (scene) -----------------------------
+ * The reference's whole solve(solver, sources, medium)
+ * (src/solvers/convergent_born.jl:565-600) with its host stages moved to the
+ * GPU: _initialize_solver! (:619-675: potential, padding, eps, attenuation
+ * :893-944), generate_incident_field (src/sources/plane_wave.jl:198-268,
+ * coherent sum field_generation.jl:42-78), the source term (:585-586), the
+ * magnetic field (:784-800) and E_inc/H_inc add + crop_to_ROI
+ * (src/fields/electromagnetic_field.jl:252-303).  The caller keeps the
+ * scalar boundary-condition and plane-wave algebra (padding_pixels, window
+ * values, unit vectors), which is O(1) host work. */
+typedef struct {
+  int64_t grid[3];          /* ROI grid (gx, gy, gz): the permittivity array     */
+  double res[3];            /* voxel size per axis                               */
+  int32_t pad[3];           /* padding pixels on each side (padding_pixels)      */
+  int32_t att_len[3];       /* attenuation window length L per axis (0: none)    */
+  const double *window[3];  /* sharpness-blended window w*sharp + (1 - sharp),
+                               att_len[d] values each (boundary_conditions.jl
+                               :632-658 and convergent_born.jl:911-937)          */
+  double shift[3];          /* Green subpixel shift per axis                     */
+  int32_t averaging;        /* +/- shift averaging (any axis absorbing)          */
+  double eps_bg;            /* background permittivity                           */
+  double wavelength;        /* vacuum wavelength                                 */
+  int32_t iterations_max;   /* solver iterations_max (< 0: the Bornmax rule)     */
+  int32_t precision;        /* working precision of the Green operator          */
+} cbs_scene_desc;
+
+typedef struct {
+  double k_vec[3];          /* k_bg * unit propagation direction                 */
+  double e_perp[6];         /* transverse unit E polarisation (x, y, z complex)  */
+  double h_perp[6];         /* k_hat x e_perp / (377 / sqrt(eps_bg))             */
+  double amplitude;         /* A                                                 */
+  double phase;             /* phi: E = A exp(i (k.r + phi)) e_perp              */
+} cbs_plane_wave;
+
+typedef struct {
+  int64_t n[3];             /* padded grid                                       */
+  double eps_imag;          /* max(2^-20, 1.01 max |V0|)                         */
+  double k2_re, k2_im;      /* k_square = k_bg^2 + i eps_imag                    */
+  int32_t bornmax;          /* Bornmax (convergent_born.jl:653-660)              */
+  int32_t maxiters;         /* max(2 Bornmax, 100)                               */
+} cbs_scene_info;
+
+/* Setup only: the padded potential V (Nx*Ny*Nz complex) and source term S
+ * (Nx*Ny*Nz*3 complex) as the GPU builds them; either may be NULL. */
+cbs_status cbs_scene_build(cbs_ctx *ctx, const cbs_scene_desc *scene, const double *eps,
+                           int32_t nsrc, const cbs_plane_wave *srcs, double *V, double *S,
+                           cbs_scene_info *info);
+/* The whole solve on the device.  eps: ROI permittivity (gx*gy*gz complex).
+ * E_tot, H_tot: total fields cropped to the ROI (gx*gy*gz*3 complex); H_tot
+ * may be NULL.  opts->maxiters <= 0 takes info.maxiters (the reference rule);
+ * opts->h_scale is replaced by -i wavelength / (2 pi 377). */
+cbs_status cbs_solve_scene(cbs_ctx *ctx, const cbs_scene_desc *scene, const double *eps,
+                           int32_t nsrc, const cbs_plane_wave *srcs, const cbs_solve_opts *opts,
+                           double *E_tot, double *H_tot, cbs_solve_stats *stats,
+                           cbs_scene_info *info);
+
 /* ---- serving: continuous batching with HBM-footprint admission ----------
  * No counterpart in the reference (it solves one problem per call).  A
  * server thread owns a HIP stream; admitted requests share every sweep

@@ -55,17 +55,19 @@ def test_born_converged_fp64_rhs_rule(fm, oracle):
     assert oracle.true_residual(E, rhs, init.potential, init.eps_imag, init.green) <= 1.01e-11
 
 
-def test_solve_api_bead_fp64(fm, oracle):
+@pytest.mark.parametrize("device_setup", [True, False])
+def test_solve_api_bead_fp64(fm, oracle, device_setup):
     """The reference's solve(solver, source, medium) path end to end: potential,
-    incident field, source term, G S, 300 Born sweeps, curl, crop."""
-    print("case solve api bead fp64", flush=True)
+    incident field, source term, G S, 300 Born sweeps, curl, crop -- with the
+    setup and teardown on the GPU (cbs_solve_scene) and on the host."""
+    print(f"case solve api bead fp64 device_setup={device_setup}", flush=True)
     grid = (32, 32, 16)
     med = fm.phantom_bead(grid, [1.5 ** 2], 5.0, permittivity_bg=N_BG ** 2)
     src = fm.PlaneWaveSource(wavelength=500e-9, polarization=[1, 0, 0], k_vector=[0, 0, 1])
     solver = fm.ConvergentBornSolver(resolution=RES, grid_size=grid,
                                      boundary_conditions=fm.PeriodicBoundaryCondition(),
                                      tolerance=0.0, linear_solver="born", stop_rule="rhs",
-                                     maxiters=300)
+                                     maxiters=300, device_setup=device_setup)
     with pytest.warns(UserWarning):
         out = fm.solve(solver, src, med)
     s = oracle.Solver(resolution=RES, grid_size=grid, boundary_conditions=oracle.PeriodicBC(),
@@ -78,15 +80,16 @@ def test_solve_api_bead_fp64(fm, oracle):
     assert rel(out.H, ref.H) <= 1e-10
 
 
-def test_solve_api_absorbing_fp64(fm, oracle):
-    print("case solve api absorbing fp64", flush=True)
+@pytest.mark.parametrize("device_setup", [True, False])
+def test_solve_api_absorbing_fp64(fm, oracle, device_setup):
+    print(f"case solve api absorbing fp64 device_setup={device_setup}", flush=True)
     grid = (16, 16, 12)
     med = fm.phantom_bead(grid, [1.45 ** 2], 3.0, permittivity_bg=N_BG ** 2)
     bc = fm.AbsorbingBoundaryCondition(thickness=400e-9, attenuation_thickness=300e-9)
     src = fm.PlaneWaveSource(wavelength=500e-9, polarization=[0, 1, 0], k_vector=[0, 0, 1])
     solver = fm.ConvergentBornSolver(resolution=RES, grid_size=grid, boundary_conditions=bc,
                                      tolerance=0.0, linear_solver="born", stop_rule="rhs",
-                                     maxiters=300)
+                                     maxiters=300, device_setup=device_setup)
     with pytest.warns(UserWarning):
         out = fm.solve(solver, src, med)
     s = oracle.Solver(resolution=RES, grid_size=grid,

@@ -57,6 +57,15 @@ int main(void) {
   F(cbs_solve_stats, resid) F(cbs_solve_stats, resid_true_fp64) F(cbs_solve_stats, time_ms) F(cbs_solve_stats, bytes_moved)
   F(cbs_solve_stats, voxel_iters)
   F(cbs_request, desc) F(cbs_request, V) F(cbs_request, S) F(cbs_request, opts)
+  printf("cbs_scene_desc %zu\ncbs_plane_wave %zu\ncbs_scene_info %zu\n", sizeof(cbs_scene_desc),
+         sizeof(cbs_plane_wave), sizeof(cbs_scene_info));
+  F(cbs_scene_desc, grid) F(cbs_scene_desc, res) F(cbs_scene_desc, pad) F(cbs_scene_desc, att_len)
+  F(cbs_scene_desc, window) F(cbs_scene_desc, shift) F(cbs_scene_desc, averaging) F(cbs_scene_desc, eps_bg)
+  F(cbs_scene_desc, wavelength) F(cbs_scene_desc, iterations_max) F(cbs_scene_desc, precision)
+  F(cbs_plane_wave, k_vec) F(cbs_plane_wave, e_perp) F(cbs_plane_wave, h_perp) F(cbs_plane_wave, amplitude)
+  F(cbs_plane_wave, phase)
+  F(cbs_scene_info, n) F(cbs_scene_info, eps_imag) F(cbs_scene_info, k2_re) F(cbs_scene_info, k2_im)
+  F(cbs_scene_info, bornmax) F(cbs_scene_info, maxiters)
   return 0;
 }
 """
@@ -71,7 +80,9 @@ def test_ctypes_layout_matches_c(fm, tmp_path):
     out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
     L = fm._lib
     structs = {"cbs_problem_desc": L.ProblemDesc, "cbs_solve_opts": L.SolveOpts,
-               "cbs_solve_stats": L.SolveStats, "cbs_request": L.Request}
+               "cbs_solve_stats": L.SolveStats, "cbs_request": L.Request,
+               "cbs_scene_desc": L.SceneDesc, "cbs_plane_wave": L.PlaneWave,
+               "cbs_scene_info": L.SceneInfo}
     for line in filter(None, out):
         key, val = line.split()
         if "." in key:
