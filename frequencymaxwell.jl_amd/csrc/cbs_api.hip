@@ -559,6 +559,15 @@ cbs_status cbs_submit(cbs_ctx* ctx, const cbs_request* req, uint64_t* ticket) {
   });
 }
 
+cbs_status cbs_submit_scene(cbs_ctx* ctx, const cbs_scene_desc* scene, const double* eps,
+                            int32_t nsrc, const cbs_plane_wave* srcs, const cbs_solve_opts* opts,
+                            uint64_t* ticket) {
+  return guarded([&] {
+    CBS_REQUIRE(ctx && scene && eps && opts && ticket, "NULL argument");
+    *ticket = server_submit_scene(ctx->impl, *scene, eps, nsrc, srcs, *opts);
+  });
+}
+
 cbs_status cbs_poll(cbs_ctx* ctx, uint64_t ticket, int32_t* done, cbs_solve_stats* stats) {
   return guarded([&] {
     CBS_REQUIRE(ctx && done, "NULL argument");

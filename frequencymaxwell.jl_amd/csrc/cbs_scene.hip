@@ -194,10 +194,9 @@ void validate_scene(const cbs_scene_desc& d, int nsrc, const cbs_plane_wave* src
   }
 }
 
-void scene_setup(Scene& sc, Ctx* c, const cbs_scene_desc& d, const double* eps, int nsrc,
-                 const cbs_plane_wave* srcs, cbs_problem_desc& pd, hipStream_t s) {
+void scene_prepare(Scene& sc, const cbs_scene_desc& d, int nsrc, const cbs_plane_wave* srcs,
+                   cbs_problem_desc& pd) {
   validate_scene(d, nsrc, srcs);
-  sc.ctx = c;
   sc.d = d;
   sc.nroi = 1;
   sc.nvox = 1;
@@ -207,51 +206,17 @@ void scene_setup(Scene& sc, Ctx* c, const cbs_scene_desc& d, const double* eps, 
     sc.nroi *= sc.g[k];
     sc.nvox *= sc.n[k];
   }
-  for (long long i = 0; i < 2 * sc.nroi; ++i)
-    if (!std::isfinite(eps[i])) throw Error(CBS_ENUMERIC, "permittivity holds NaN or Inf");
   sc.k_bg = 2.0 * M_PI * std::sqrt(d.eps_bg) / d.wavelength;  // convergent_born.jl:630
-  sc.eps.alloc(c, (size_t)sc.nroi * sizeof(Cx<double>));
-  CBS_HIP(hipMemcpyAsync(sc.eps.p, eps, (size_t)sc.nroi * sizeof(Cx<double>),
-                         hipMemcpyHostToDevice, s));
-  std::vector<double> wins;
+  sc.win_h.clear();
   for (int k = 0; k < 3; ++k)
     for (int i = 0; i < d.att_len[k]; ++i) {
       CBS_REQUIRE(std::isfinite(d.window[k][i]), "attenuation window holds NaN or Inf");
-      wins.push_back(d.window[k][i]);
+      sc.win_h.push_back(d.window[k][i]);
     }
-  if (wins.empty()) wins.push_back(1.0);
-  sc.win.alloc(c, wins.size() * sizeof(double));
-  CBS_HIP(hipMemcpyAsync(sc.win.p, wins.data(), wins.size() * sizeof(double),
-                         hipMemcpyHostToDevice, s));
+  if (sc.win_h.empty()) sc.win_h.push_back(1.0);
+  for (int k = 0; k < 3; ++k) sc.d.window[k] = nullptr;  // host pointers are not kept
   sc.nsrc = nsrc;
-  sc.srcs.alloc(c, (size_t)nsrc * sizeof(cbs_plane_wave));
-  CBS_HIP(hipMemcpyAsync(sc.srcs.p, srcs, (size_t)nsrc * sizeof(cbs_plane_wave),
-                         hipMemcpyHostToDevice, s));
-  // eps_imag = max(2^-20, 1.01 max|V0|)  (:641; the zero padding adds nothing)
-  const int nb = (int)((sc.nroi + kSceneChunk - 1) / kSceneChunk);
-  DevBuf part;
-  part.alloc(c, (size_t)(nb + 1) * sizeof(double));
-  SceneK k = scene_k(sc);
-  hipLaunchKernelGGL(k_v0_absmax, dim3(nb), dim3(kThreads), 0, s, k, sc.nroi,
-                     part.as<double>());
-  hipLaunchKernelGGL(k_max_of, dim3(1), dim3(kThreads), 0, s, part.as<double>(), nb,
-                     part.as<double>() + nb);
-  CBS_HIP(hipGetLastError());
-  double vmax = 0.0;
-  CBS_HIP(hipMemcpyAsync(&vmax, part.as<double>() + nb, sizeof(double), hipMemcpyDeviceToHost, s));
-  CBS_HIP(hipStreamSynchronize(s));
-  sc.eps_imag = std::max(std::ldexp(1.0, -20), vmax * 1.01);
-  // Bornmax and the maxiters rule (:653-660, :727-750)
-  if (d.iterations_max < 0) {
-    const double steps = std::fabs(2 * sc.k_bg / sc.eps_imag);
-    double ext = 0;
-    for (int q = 0; q < 3; ++q) ext += std::pow((double)sc.n[q] * d.res[q], 2);
-    ext = std::sqrt(ext);
-    sc.bornmax = (int)std::ceil(ext / steps / 2 + 1) * 2 * std::abs(d.iterations_max);
-  } else {
-    sc.bornmax = d.iterations_max;
-  }
-  sc.maxiters = std::max(2 * sc.bornmax, 100);
+  sc.srcs_h.assign(srcs, srcs + nsrc);
   std::memset(&pd, 0, sizeof(pd));
   for (int q = 0; q < 3; ++q) {
     pd.n[q] = sc.n[q];
@@ -260,9 +225,61 @@ void scene_setup(Scene& sc, Ctx* c, const cbs_scene_desc& d, const double* eps, 
   }
   pd.averaging = d.averaging;
   pd.k2_re = sc.k_bg * sc.k_bg;  // k_square = k_bg^2 + i eps_imag (:663)
+  pd.precision = d.precision;
+}
+
+void scene_alloc(Scene& sc, Ctx* c, hipStream_t s) {
+  sc.ctx = c;
+  sc.eps.alloc(c, (size_t)sc.nroi * sizeof(Cx<double>));
+  sc.win.alloc(c, sc.win_h.size() * sizeof(double));
+  CBS_HIP(hipMemcpyAsync(sc.win.p, sc.win_h.data(), sc.win_h.size() * sizeof(double),
+                         hipMemcpyHostToDevice, s));
+  sc.srcs.alloc(c, (size_t)sc.nsrc * sizeof(cbs_plane_wave));
+  CBS_HIP(hipMemcpyAsync(sc.srcs.p, sc.srcs_h.data(), (size_t)sc.nsrc * sizeof(cbs_plane_wave),
+                         hipMemcpyHostToDevice, s));
+  const int nb = (int)((sc.nroi + kSceneChunk - 1) / kSceneChunk);
+  sc.part.alloc(c, (size_t)(nb + 1) * sizeof(double));
+}
+
+void scene_finalize(Scene& sc, cbs_problem_desc& pd, hipStream_t s) {
+  // eps_imag = max(2^-20, 1.01 max|V0|)  (:641; the zero padding adds nothing)
+  const int nb = (int)((sc.nroi + kSceneChunk - 1) / kSceneChunk);
+  SceneK k = scene_k(sc);
+  hipLaunchKernelGGL(k_v0_absmax, dim3(nb), dim3(kThreads), 0, s, k, sc.nroi,
+                     sc.part.as<double>());
+  hipLaunchKernelGGL(k_max_of, dim3(1), dim3(kThreads), 0, s, sc.part.as<double>(), nb,
+                     sc.part.as<double>() + nb);
+  CBS_HIP(hipGetLastError());
+  double vmax = 0.0;
+  CBS_HIP(hipMemcpyAsync(&vmax, sc.part.as<double>() + nb, sizeof(double), hipMemcpyDeviceToHost,
+                         s));
+  CBS_HIP(hipStreamSynchronize(s));
+  CBS_REQUIRE(std::isfinite(vmax), "permittivity holds NaN or Inf");
+  sc.eps_imag = std::max(std::ldexp(1.0, -20), vmax * 1.01);
+  // Bornmax and the maxiters rule (:653-660, :727-750)
+  if (sc.d.iterations_max < 0) {
+    const double steps = std::fabs(2 * sc.k_bg / sc.eps_imag);
+    double ext = 0;
+    for (int q = 0; q < 3; ++q) ext += std::pow((double)sc.n[q] * sc.d.res[q], 2);
+    ext = std::sqrt(ext);
+    sc.bornmax = (int)std::ceil(ext / steps / 2 + 1) * 2 * std::abs(sc.d.iterations_max);
+  } else {
+    sc.bornmax = sc.d.iterations_max;
+  }
+  sc.maxiters = std::max(2 * sc.bornmax, 100);
   pd.k2_im = sc.eps_imag;
   pd.eps_imag = sc.eps_imag;
-  pd.precision = d.precision;
+}
+
+void scene_setup(Scene& sc, Ctx* c, const cbs_scene_desc& d, const double* eps, int nsrc,
+                 const cbs_plane_wave* srcs, cbs_problem_desc& pd, hipStream_t s) {
+  scene_prepare(sc, d, nsrc, srcs, pd);
+  for (long long i = 0; i < 2 * sc.nroi; ++i)
+    if (!std::isfinite(eps[i])) throw Error(CBS_ENUMERIC, "permittivity holds NaN or Inf");
+  scene_alloc(sc, c, s);
+  CBS_HIP(hipMemcpyAsync(sc.eps.p, eps, (size_t)sc.nroi * sizeof(Cx<double>),
+                         hipMemcpyHostToDevice, s));
+  scene_finalize(sc, pd, s);
 }
 
 static unsigned nblk(long long n) { return (unsigned)((n + kThreads - 1) / kThreads); }

@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstring>
 
+#include "cbs_scene.h"
 #include "cbs_serve.h"
 #include "cbs_solve.h"
 
@@ -66,6 +67,13 @@ struct Job {
   SolveItem<float> it32;
   int set = -1;  // solver set the job is a member of (-1: none)
   hipEvent_t ev = nullptr;  // end of the compute work before the download
+  // scene requests: V holds the ROI permittivity; the device builds the
+  // potential and the source term and returns cropped total fields
+  std::unique_ptr<Scene> sc;
+  DevBuf Eo, Ho;  // cropped totals
+  size_t out_bytes() const {
+    return (size_t)(sc ? sc->nroi : p->nvox) * 3 * sizeof(Cx<double>);
+  }
 };
 
 static int set_index(bool fp32, int method, int S) {
@@ -144,6 +152,7 @@ struct Server {
   void loop();
   void tick();
   void begin_admit(Job& j);
+  void prepare_job_work(Job& j);
   void start_upload(Job& j);
   void finish_admit(Job& j);
   void join_set(Job& j, bool fp32, const IterOpts& io);
@@ -197,6 +206,9 @@ void Server::release(Job& j) {
   j.X.release();
   j.Hd.release();
   j.Wh.release();
+  j.Eo.release();
+  j.Ho.release();
+  j.sc.reset();
 }
 
 // a job leaves the active list with status st
@@ -238,13 +250,8 @@ void Server::leave_set(Job& j) {
   j.set = -1;
 }
 
-// device buffers of an admitted job (the fields follow on the transfer stream)
-void Server::begin_admit(Job& j) {
-  j.p.reset(new Problem());
-  problem_alloc(*j.p, ctx, j.desc, j.o.precision != CBS_FP64);
-  const size_t nc = (size_t)j.p->nvox * 3;
-  j.Sd.alloc(ctx, nc * sizeof(Cx<double>));
-  j.X.alloc(ctx, nc * sizeof(Cx<double>));
+// work slots of the job's problem in the precisions its method runs
+void Server::prepare_job_work(Job& j) {
   if (j.o.precision == CBS_FP64) {
     Work<double>& w = prepare_work<double>(*j.p, 0, false);
     j.it64 = SolveItem<double>{j.p.get(), &w, w.B.as<Cx<double>>()};
@@ -257,14 +264,32 @@ void Server::begin_admit(Job& j) {
     j.it64 = SolveItem<double>{j.p.get(), &w, w.B.as<Cx<double>>()};
     j.it32 = SolveItem<float>{j.p.get(), &w32, w32.B.as<Cx<float>>()};
   }
-  if (j.o.compute_h) {
-    j.Hd.alloc(ctx, nc * sizeof(Cx<double>));
-    if (!j.it64.w) {  // FP32 requests: an FP64 workspace for the curl
-      DevProb<double> d = make_desc<double>(*j.p, PRE_COPY, ZOP_CURL, POST_STORE, nullptr,
-                                            nullptr, nullptr, nullptr, nullptr, nullptr);
-      j.Wh.alloc(ctx, (size_t)j.p->S * 3 * (size_t)d.ws[2] * sizeof(Cx<double>));
-    }
+  if (j.o.compute_h && !j.it64.w) {  // FP32 requests: an FP64 workspace for the curl
+    DevProb<double> d = make_desc<double>(*j.p, PRE_COPY, ZOP_CURL, POST_STORE, nullptr, nullptr,
+                                          nullptr, nullptr, nullptr, nullptr);
+    j.Wh.alloc(ctx, (size_t)j.p->S * 3 * (size_t)d.ws[2] * sizeof(Cx<double>));
   }
+}
+
+// device buffers of an admitted job (the fields follow on the transfer stream)
+void Server::begin_admit(Job& j) {
+  long long nvox = 1;
+  for (int q = 0; q < 3; ++q) nvox *= j.desc.n[q];
+  const size_t fb = (size_t)nvox * 3 * sizeof(Cx<double>);
+  if (j.sc) {
+    // the problem waits for eps_imag, known once the permittivity is on the device
+    scene_alloc(*j.sc, ctx, io);
+    const size_t ob = (size_t)j.sc->nroi * 3 * sizeof(Cx<double>);
+    j.Eo.alloc(ctx, ob);
+    if (j.o.compute_h) j.Ho.alloc(ctx, ob);
+  } else {
+    j.p.reset(new Problem());
+    problem_alloc(*j.p, ctx, j.desc, j.o.precision != CBS_FP64);
+    prepare_job_work(j);
+  }
+  j.Sd.alloc(ctx, fb);
+  j.X.alloc(ctx, fb);
+  if (j.o.compute_h) j.Hd.alloc(ctx, fb);
   j.phase = PH_WAIT_LOAD;
 }
 
@@ -274,9 +299,9 @@ void Server::start_upload(Job& j) {
   const size_t vb = j.V.size() * sizeof(double), sb = j.S.size() * sizeof(double);
   up.ensure(vb + sb);
   par_copy(up.p, j.V.data(), vb);
-  par_copy(up.p + vb, j.S.data(), sb);
-  CBS_HIP(hipMemcpyAsync(j.p->V64.p, up.p, vb, hipMemcpyHostToDevice, io));
-  CBS_HIP(hipMemcpyAsync(j.Sd.p, up.p + vb, sb, hipMemcpyHostToDevice, io));
+  if (sb) par_copy(up.p + vb, j.S.data(), sb);
+  CBS_HIP(hipMemcpyAsync(j.sc ? j.sc->eps.p : j.p->V64.p, up.p, vb, hipMemcpyHostToDevice, io));
+  if (sb) CBS_HIP(hipMemcpyAsync(j.Sd.p, up.p + vb, sb, hipMemcpyHostToDevice, io));
   CBS_HIP(hipEventRecord(ev_up, io));
   std::vector<double>().swap(j.V);
   std::vector<double>().swap(j.S);
@@ -288,6 +313,18 @@ void Server::start_upload(Job& j) {
 void Server::finish_admit(Job& j) {
   Ctx* c = ctx;
   hipStream_t s = stream;
+  if (j.sc) {
+    // potential and source term on the device (convergent_born.jl:619-675, :585-586)
+    scene_finalize(*j.sc, j.desc, s);
+    j.p.reset(new Problem());
+    problem_alloc(*j.p, c, j.desc, j.o.precision != CBS_FP64);
+    prepare_job_work(j);
+    scene_potential(*j.sc, j.p->V64.as<Cx<double>>(), s);
+    scene_source(*j.sc, j.p->V64.as<Cx<double>>(), j.Sd.as<Cx<double>>(), s);
+    if (j.o.maxiters <= 0) j.o.maxiters = j.sc->maxiters;
+    j.o.h_scale_re = 0.0;
+    j.o.h_scale_im = scene_h_scale_im(j.sc->d);
+  }
   Problem& p = *j.p;
   const long long nc = 3 * p.nvox;
   const cbs_solve_opts& o = j.o;
@@ -454,6 +491,9 @@ void Server::begin_retire(Job& j) {
                           j.o.h_scale_im,
                           j.it64.w ? j.it64.w->W.as<Cx<double>>() : j.Wh.as<Cx<double>>(),
                           stream);
+  if (j.sc)  // E_inc / H_inc add and crop (electromagnetic_field.jl:252-303)
+    scene_finish(*j.sc, j.X.as<Cx<double>>(), j.o.compute_h ? j.Hd.as<Cx<double>>() : nullptr,
+                 j.Eo.as<Cx<double>>(), j.o.compute_h ? j.Ho.as<Cx<double>>() : nullptr, stream);
   CBS_HIP(hipEventCreateWithFlags(&j.ev, hipEventDisableTiming));
   CBS_HIP(hipEventRecord(j.ev, stream));
   j.stats.voxel_iters = (int64_t)j.stats.iters * p.nvox;
@@ -461,12 +501,12 @@ void Server::begin_retire(Job& j) {
 }
 
 void Server::start_download(Job& j) {
-  const size_t fb = (size_t)j.p->nvox * 3 * sizeof(Cx<double>);
+  const size_t fb = j.out_bytes();
   down.ensure(j.o.compute_h ? 2 * fb : fb);
   CBS_HIP(hipStreamWaitEvent(io, j.ev, 0));
-  CBS_HIP(hipMemcpyAsync(down.p, j.X.p, fb, hipMemcpyDeviceToHost, io));
+  CBS_HIP(hipMemcpyAsync(down.p, j.sc ? j.Eo.p : j.X.p, fb, hipMemcpyDeviceToHost, io));
   if (j.o.compute_h)
-    CBS_HIP(hipMemcpyAsync(down.p + fb, j.Hd.p, fb, hipMemcpyDeviceToHost, io));
+    CBS_HIP(hipMemcpyAsync(down.p + fb, j.sc ? j.Ho.p : j.Hd.p, fb, hipMemcpyDeviceToHost, io));
   CBS_HIP(hipEventRecord(ev_down, io));
   down_job = &j;
   j.phase = PH_UNLOADING;
@@ -506,7 +546,7 @@ void Server::tick() {
   launch_sets();
   // host side of the transfers while the chunk runs
   if (landed) {
-    const size_t fb = (size_t)landed->p->nvox * 3 * sizeof(Cx<double>);
+    const size_t fb = landed->out_bytes();
     std::vector<double> E(fb / sizeof(double)), H;
     par_copy(E.data(), down.p, fb);
     if (landed->o.compute_h) {
@@ -660,20 +700,18 @@ static Server& running_server(Ctx& c) {
   return *c.server;
 }
 
+static Server& running_server(Ctx& c);
+static void check_serve_opts(const cbs_solve_opts& o, bool scene);
+static uint64_t enqueue(Server& sv, std::shared_ptr<Job> j);
+
 uint64_t server_submit(Ctx& c, const cbs_request& r) {
   Server& sv = running_server(c);
   validate_desc(r.desc);
   CBS_REQUIRE(r.V && r.S, "NULL field in request");
-  const cbs_solve_opts& o = r.opts;
-  CBS_REQUIRE(o.method == CBS_BORN || o.method == CBS_BICGSTAB,
-              "serving runs CBS_BORN and CBS_BICGSTAB requests");
-  CBS_REQUIRE(o.precision >= CBS_FP64 && o.precision <= CBS_FP32_REFINE, "unknown precision");
-  CBS_REQUIRE(o.stop_rule == CBS_STOP_RHS || o.stop_rule == CBS_STOP_KRYLOV, "unknown stop rule");
-  CBS_REQUIRE(o.tol >= 0 && o.tol <= 1, "tolerance must be in [0, 1]");
-  CBS_REQUIRE(o.maxiters >= 1, "maxiters must be >= 1");
+  check_serve_opts(r.opts, false);
   auto j = std::make_shared<Job>();
   j->desc = r.desc;
-  j->o = o;
+  j->o = r.opts;
   const size_t nv = (size_t)r.desc.n[0] * r.desc.n[1] * r.desc.n[2];
   j->V.assign(r.V, r.V + 2 * nv);
   j->S.assign(r.S, r.S + 6 * nv);
@@ -681,7 +719,20 @@ uint64_t server_submit(Ctx& c, const cbs_request& r) {
     if (!std::isfinite(v)) throw Error(CBS_ENUMERIC, "potential holds NaN or Inf");
   for (double v : j->S)
     if (!std::isfinite(v)) throw Error(CBS_ENUMERIC, "source term holds NaN or Inf");
-  j->footprint = footprint_bytes(r.desc, o);
+  j->footprint = footprint_bytes(r.desc, r.opts);
+  return enqueue(sv, j);
+}
+
+static void check_serve_opts(const cbs_solve_opts& o, bool scene) {
+  CBS_REQUIRE(o.method == CBS_BORN || o.method == CBS_BICGSTAB,
+              "serving runs CBS_BORN and CBS_BICGSTAB requests");
+  CBS_REQUIRE(o.precision >= CBS_FP64 && o.precision <= CBS_FP32_REFINE, "unknown precision");
+  CBS_REQUIRE(o.stop_rule == CBS_STOP_RHS || o.stop_rule == CBS_STOP_KRYLOV, "unknown stop rule");
+  CBS_REQUIRE(o.tol >= 0 && o.tol <= 1, "tolerance must be in [0, 1]");
+  CBS_REQUIRE(scene || o.maxiters >= 1, "maxiters must be >= 1");
+}
+
+static uint64_t enqueue(Server& sv, std::shared_ptr<Job> j) {
   std::lock_guard<std::mutex> lk(sv.mu);
   CBS_REQUIRE(sv.running && !sv.stop, "server not running");
   if (j->footprint > sv.capacity)
@@ -694,6 +745,26 @@ uint64_t server_submit(Ctx& c, const cbs_request& r) {
   sv.queue.push_back(j);
   sv.cv.notify_all();
   return j->ticket;
+}
+
+uint64_t server_submit_scene(Ctx& c, const cbs_scene_desc& scene, const double* eps, int nsrc,
+                             const cbs_plane_wave* srcs, const cbs_solve_opts& o) {
+  Server& sv = running_server(c);
+  CBS_REQUIRE(eps, "NULL permittivity");
+  check_serve_opts(o, true);
+  auto j = std::make_shared<Job>();
+  j->sc.reset(new Scene());
+  scene_prepare(*j->sc, scene, nsrc, srcs, j->desc);
+  j->desc.eps_imag = 1.0;  // placeholder for the checks below; set at admission
+  validate_desc(j->desc);
+  j->o = o;
+  const size_t nr = (size_t)j->sc->nroi;
+  j->V.assign(eps, eps + 2 * nr);
+  for (double v : j->V)
+    if (!std::isfinite(v)) throw Error(CBS_ENUMERIC, "permittivity holds NaN or Inf");
+  cbs_solve_opts of = o;
+  j->footprint = footprint_bytes(j->desc, of) + nr * sizeof(Cx<double>) * (1 + 3 * (o.compute_h ? 2 : 1));
+  return enqueue(sv, j);
 }
 
 static std::shared_ptr<Job> find_job(Server& sv, uint64_t ticket) {

@@ -33,7 +33,7 @@ EXPORTED = (
     "cbs_problem_run", "cbs_problem_get_fields", "cbs_solve_multi", "cbs_solve_batch",
     "cbs_request_footprint", "cbs_server_start", "cbs_server_stop", "cbs_submit", "cbs_poll",
     "cbs_wait", "cbs_cancel", "cbs_time_sweeps", "cbs_stream_copy", "cbs_scene_build",
-    "cbs_solve_scene",
+    "cbs_solve_scene", "cbs_submit_scene",
 )
 
 
@@ -185,6 +185,8 @@ def _declare(lib):
         "cbs_solve_scene": (s, [_P, C.POINTER(SceneDesc), _D, C.c_int32, C.POINTER(PlaneWave),
                                 C.POINTER(SolveOpts), _D, _D, C.POINTER(SolveStats),
                                 C.POINTER(SceneInfo)]),
+        "cbs_submit_scene": (s, [_P, C.POINTER(SceneDesc), _D, C.c_int32, C.POINTER(PlaneWave),
+                                 C.POINTER(SolveOpts), C.POINTER(C.c_uint64)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

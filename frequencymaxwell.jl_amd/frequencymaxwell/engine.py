@@ -264,6 +264,19 @@ class CBSServer:
         self._shapes[t.value] = (shape + (3,), bool(opts.compute_h))
         return t.value
 
+    def submit_scene(self, desc: L.SceneDesc, eps, waves, opts: L.SolveOpts) -> int:
+        """Queue a scene (permittivity + plane waves; potential, source term,
+        H and the incident add + crop on the GPU); wait() returns the total
+        fields cropped to the ROI."""
+        g = tuple(int(x) for x in desc.grid)
+        eps = L.as_field(eps, g)
+        arr = (L.PlaneWave * len(waves))(*waves)
+        t = C.c_uint64()
+        L.check(self._lib.cbs_submit_scene(self.ctx.handle, C.byref(desc), L.dptr(eps), len(waves),
+                                           arr, C.byref(opts), C.byref(t)))
+        self._shapes[t.value] = (g + (3,), bool(opts.compute_h))
+        return t.value
+
     def poll(self, ticket: int):
         """(done, stats or None)."""
         done = C.c_int32()

@@ -277,6 +277,13 @@ cbs_status cbs_server_stop(cbs_ctx *ctx);
 /* Queue a request (copies V and S); returns immediately with a ticket.
  * CBS_ENOMEM if the request can never fit the budget. */
 cbs_status cbs_submit(cbs_ctx *ctx, const cbs_request *req, uint64_t *ticket);
+/* Queue a scene (device-side setup and teardown, see cbs_solve_scene): eps is
+ * the ROI permittivity, copied at submit; cbs_wait then returns the total
+ * fields cropped to the ROI (gx*gy*gz*3 complex).  opts->maxiters <= 0 takes
+ * the reference budget max(2 Bornmax, 100); H uses -i wavelength/(2 pi 377). */
+cbs_status cbs_submit_scene(cbs_ctx *ctx, const cbs_scene_desc *scene, const double *eps,
+                            int32_t nsrc, const cbs_plane_wave *srcs, const cbs_solve_opts *opts,
+                            uint64_t *ticket);
 /* done = 1 when the result is ready (stats filled), 0 while queued/running. */
 cbs_status cbs_poll(cbs_ctx *ctx, uint64_t ticket, int32_t *done, cbs_solve_stats *stats);
 /* Block until the request ends; copy E_scat (and H_scat if computed; either

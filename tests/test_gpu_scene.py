@@ -10,48 +10,12 @@ Bars: V and the scalars to 1e-14 relative (same operations, hypot and sincos
 differ by an ulp), S to 1e-13, the fields to 1e-10 (as the solve tests).
 """
 
-import math
-
 import numpy as np
 import pytest
 
-from cases import N_BG, RES, rel
+from cases import N_BG, RES, rel, scene_bcs as _bcs, scene_medium as _medium, scene_sources as _sources
 
 pytestmark = pytest.mark.gpu
-
-
-def _bcs(fm, oracle, kind):
-    if kind == "periodic":
-        return fm.PeriodicBoundaryCondition(), oracle.PeriodicBC()
-    if kind == "absorbing":
-        return (fm.AbsorbingBoundaryCondition(thickness=400e-9, attenuation_thickness=300e-9),
-                oracle.AbsorbingBC(thickness=400e-9, attenuation_thickness=300e-9))
-    # z only absorbing, exponential profile, sharpness 0.7; x, y periodic
-    f = fm.AbsorbingBoundaryCondition(thickness=500e-9, attenuation_thickness=400e-9,
-                                      sharpness=0.7, profile=fm.ExponentialProfile)
-    o = oracle.AbsorbingBC(thickness=500e-9, attenuation_thickness=400e-9, sharpness=0.7,
-                           profile="exponential")
-    return ((fm.PeriodicBoundaryCondition(), fm.PeriodicBoundaryCondition(), f),
-            (oracle.PeriodicBC(), oracle.PeriodicBC(), o))
-
-
-def _sources(fm, oracle, many):
-    th, ph = 0.4, 1.1
-    k = [math.sin(th) * math.cos(ph), math.sin(th) * math.sin(ph), math.cos(th)]
-    p1 = [-math.sin(ph), math.cos(ph), 0.0]
-    specs = [dict(polarization=[1, 0, 0], k_vector=[0, 0, 1])]
-    if many:
-        specs = [dict(polarization=p1, k_vector=k, amplitude=0.7, phase=0.3),
-                 dict(polarization=[0, 1j, 0], k_vector=[0, 0, 1], amplitude=1.2, phase=-1.0)]
-    return ([fm.PlaneWaveSource(wavelength=500e-9, **s) for s in specs],
-            [oracle.PlaneWave(wavelength=500e-9, **s) for s in specs])
-
-
-def _medium(fm, grid, seed):
-    rng = np.random.default_rng(seed)
-    n = 1.33 + 0.2 * rng.random(grid)
-    eps = np.asfortranarray((n ** 2 + 0.01j * rng.random(grid)).astype(np.complex128))
-    return fm.Medium(eps, N_BG ** 2)
 
 
 @pytest.mark.parametrize("kind", ["periodic", "absorbing", "z_exponential"])

@@ -162,3 +162,66 @@ def test_serve_cancel_stop_poll(fm, oracle):
     with pytest.raises(fm.CBSError) as ei:
         srv.wait(d)
     assert ei.value.code == fm._lib.CBS_ECANCELED
+
+
+def _scene_requests(fm, oracle):
+    """Scene requests of different grids, boundaries, sources and methods."""
+    from cases import scene_bcs, scene_medium, scene_sources
+    from frequencymaxwell import solver as S
+
+    out = []
+    for grid, kind, many, seed, kw in [
+        ((16, 12, 10), "periodic", True, 61, dict(method="born", precision="fp64", tol=0.0,
+                                                   maxiters=40)),
+        ((12, 12, 10), "absorbing", False, 62, dict(method="born", precision="fp32_refine",
+                                                     tol=2e-6, maxiters=3000)),
+        ((20, 12, 8), "z_exponential", True, 63, dict(method="bicgstab", precision="fp64",
+                                                       tol=1e-8, maxiters=400)),
+        ((16, 16, 12), "absorbing", False, 64, dict(method="born", precision="fp64", tol=1e-15,
+                                                     maxiters=0)),
+    ]:
+        bf, bo = scene_bcs(fm, oracle, kind)
+        srcf, srco = scene_sources(fm, oracle, many)
+        med = scene_medium(fm, grid, seed)
+        solver = fm.ConvergentBornSolver(resolution=RES, grid_size=grid, boundary_conditions=bf)
+        d, eps, waves, keep = S.scene(solver, srcf, med)
+        out.append((d, eps, waves, keep, fm.make_opts(stop_rule="rhs", **kw), (grid, bo, srco, med)))
+    return out
+
+
+def test_serve_scenes_alone_mix_and_direct(fm, oracle):
+    """Scene requests (device-side setup and teardown) give the same cropped
+    total fields alone, in a mix and through cbs_solve_scene; the fixed-budget
+    one matches the oracle's whole solve."""
+    print("case serve scenes", flush=True)
+    from frequencymaxwell import engine
+
+    reqs = _scene_requests(fm, oracle)
+    res = {}
+    for active in (1, 8):
+        srv = fm.CBSServer(max_active=active)
+        try:
+            if active == 1:
+                res[active] = [srv.wait(srv.submit_scene(d, e, w, o)) for d, e, w, _, o, _ in reqs]
+            else:
+                ts = [srv.submit_scene(d, e, w, o) for d, e, w, _, o, _ in reqs]
+                res[active] = [srv.wait(t) for t in ts]
+        finally:
+            srv.stop()
+    for i, (d, e, w, _, o, meta) in enumerate(reqs):
+        (E1, H1, s1), (E2, H2, s2) = res[1][i], res[8][i]
+        print(f"  scene {i}: iters {s1.iters}", flush=True)
+        assert E1.shape == meta[0] + (3,)
+        assert s1.iters == s2.iters
+        assert np.array_equal(E1, E2) and np.array_equal(H1, H2), f"scene {i} differs in the mix"
+        E3, H3, s3, info = engine.solve_scene(d, e, w, o)
+        assert s3.iters == s1.iters
+        assert np.array_equal(E3, E1) and np.array_equal(H3, H1), f"scene {i}: served != direct"
+        if i == 3:  # the reference budget max(2 Bornmax, 100)
+            assert s1.iters == info.maxiters
+    grid, bo, srco, med = reqs[0][5]
+    s = oracle.Solver(resolution=RES, grid_size=grid, boundary_conditions=bo, tolerance=0.0)
+    ref = oracle.solve(s, srco, med.permittivity, N_BG ** 2, method="born", rule="rhs", maxiters=40)
+    E1, H1, _ = res[1][0]
+    assert rel(E1, ref.E) <= 1e-10
+    assert rel(H1, ref.H) <= 1e-10
