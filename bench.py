@@ -65,23 +65,26 @@ SERVE_GRIDS = [(64, 64, 64), (96, 96, 64), (128, 128, 64), (192, 192, 96), (256,
 
 
 def random_spheres(fm, grid, rng):
-    """1-10 spheres, n uniform in 1.36-1.60, radii 4-16 px, centres in the interior."""
+    """1-10 spheres, n uniform in 1.36-1.60, radii 4-min(16, g/4) px, centres in the interior."""
     eps_bg = 1.33 ** 2
     ph = np.full(grid, complex(eps_bg), dtype=np.complex128, order="F")
-    axes = [np.arange(1, g + 1, dtype=np.float64) for g in grid]
     for _ in range(int(rng.integers(1, 11))):
         r = float(rng.uniform(4.0, min(16.0, min(grid) / 4)))
         c = [float(rng.uniform(r + 1, g - r)) for g in grid]
         n = float(rng.uniform(1.36, 1.60))
-        d2 = ((axes[0][:, None, None] - c[0]) ** 2 + (axes[1][None, :, None] - c[1]) ** 2
-              + (axes[2][None, None, :] - c[2]) ** 2)
-        ph[d2 <= r * r] = n * n
+        lo = [max(0, int(math.floor(c[d] - r)) - 1) for d in range(3)]
+        hi = [min(grid[d], int(math.ceil(c[d] + r)) + 1) for d in range(3)]
+        ax = [np.arange(lo[d] + 1, hi[d] + 1, dtype=np.float64) - c[d] for d in range(3)]
+        d2 = ax[0][:, None, None] ** 2 + ax[1][None, :, None] ** 2 + ax[2][None, None, :] ** 2
+        box = ph[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]]
+        box[d2 <= r * r] = n * n
     return fm.Medium(ph, eps_bg)
 
 
 def serve_request(fm, grid, rng, tol):
-    """Padded potential, Green parameters and source term of one served solve:
-    random-sphere phantom, plane wave at a random angle within NA 1.0, s-polarised."""
+    """One served solve as a scene: random-sphere phantom, plane wave at a random
+    angle within NA 1.0, s-polarised; the potential, incident field and source
+    term are built on the GPU."""
     from frequencymaxwell import solver as S
 
     med = random_spheres(fm, grid, rng)
@@ -93,22 +96,21 @@ def serve_request(fm, grid, rng, tol):
     k = [sin_t * math.cos(phi), sin_t * math.sin(phi), math.sqrt(1 - sin_t ** 2)]
     pol = [-math.sin(phi), math.cos(phi), 0.0]
     src = fm.PlaneWaveSource(wavelength=500e-9, polarization=pol, k_vector=k)
-    pot, eps_imag, bornmax, k_bg, spec = S._initialize_solver(sol, med, 500e-9)
-    inc = fm.generate_incident_field(src, sol, permittivity_bg=1.33 ** 2)
-    Sterm = np.asfortranarray(pot[..., None] * inc.E + complex(0, eps_imag) * inc.E)
-    return pot, spec, Sterm
+    return S.scene(sol, src, med)
 
 
-def serve_stream(fm, n_req, rate, max_active, seed, maxiters, pool=None):
-    """Poisson arrivals (rate per second) of solves drawn from SERVE_GRIDS with
-    tolerance 1e-4 or 1e-6; returns latency percentiles and throughput."""
+def serve_stream(fm, n_req, rate, max_active, seed, maxiters):
+    """Poisson arrivals (rate per second) of scene solves on grids drawn from
+    SERVE_GRIDS with tolerance 1e-4 or 1e-6; returns latency percentiles and
+    throughput.  The requests are generated before the clock starts."""
     import threading
 
     rng = np.random.default_rng(seed)
-    if pool is None:  # two phantoms / angles per grid, prepared before the clock starts
-        pool = {g: [serve_request(fm, g, rng, 1e-6) for _ in range(2)] for g in SERVE_GRIDS}
-    picks = [(SERVE_GRIDS[int(rng.integers(len(SERVE_GRIDS)))], int(rng.integers(2)),
-              1e-4 if rng.random() < 0.5 else 1e-6) for _ in range(n_req)]
+    reqs = []
+    for _ in range(n_req):
+        g = SERVE_GRIDS[int(rng.integers(len(SERVE_GRIDS)))]
+        tol = 1e-4 if rng.random() < 0.5 else 1e-6
+        reqs.append((serve_request(fm, g, rng, tol), tol))
     gaps = rng.exponential(1.0 / rate, n_req)
     srv = fm.CBSServer(max_active=max_active)
     tickets = [None] * n_req
@@ -116,15 +118,14 @@ def serve_stream(fm, n_req, rate, max_active, seed, maxiters, pool=None):
 
     def feed():
         t = t0
-        for i, (g, j, tol) in enumerate(picks):
+        for i, ((d, eps, waves, keep), tol) in enumerate(reqs):
             t += gaps[i]
             dt = t - time.perf_counter()
             if dt > 0:
                 time.sleep(dt)
-            pot, spec, Sterm = pool[g][j]
             o = fm.make_opts(method="born", precision="fp32_refine", tol=tol, maxiters=maxiters,
-                             compute_h=False)
-            tickets[i] = srv.submit(pot, spec, Sterm, o)
+                             compute_h=True)
+            tickets[i] = srv.submit_scene(d, eps, waves, o)
 
     th = threading.Thread(target=feed)
     th.start()
@@ -137,15 +138,15 @@ def serve_stream(fm, n_req, rate, max_active, seed, maxiters, pool=None):
             lat.append(st.time_ms)
             vi += st.voxel_iters
             conv += int(st.converged)
-            worst = max(worst, st.resid_true_fp64 / picks[i][2])
+            worst = max(worst, st.resid_true_fp64 / reqs[i][1])
     finally:
         th.join()
         srv.stop()
     wall = time.perf_counter() - t0
-    return pool, {
+    return {
         "requests": n_req, "max_active": max_active, "arrival_rate_per_s": rate,
         "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-        "mean_ms": float(np.mean(lat)), "throughput_voxel_iters_per_s": vi / wall,
+        "mean_ms": float(np.mean(lat)), "voxel_iters_per_s": vi / wall,
         "requests_per_s": n_req / wall, "wall_s": wall, "converged": conv,
         "max_resid_over_tol": worst,
     }
@@ -198,17 +199,19 @@ def read_traffic(summary: str, kernel: str):
 def serving_leg(fm, args):
     """The mixed-size request stream, continuously batched and one at a time."""
     print("[bench] serving stream ...", file=sys.stderr, flush=True)
-    pool, cb, seq = None, None, None
+    cb, seq = None, None
     if args.serve_mode in ("both", "batched"):
-        pool, cb = serve_stream(fm, args.serve_requests, args.serve_rate, args.serve_active, 7,
-                                args.maxiters)
+        cb = serve_stream(fm, args.serve_requests, args.serve_rate, args.serve_active, 7,
+                          args.maxiters)
         print(f"[bench] continuous batching: p50 {cb['p50_ms']:.0f} ms p99 {cb['p99_ms']:.0f} ms",
               file=sys.stderr, flush=True)
     if args.serve_mode in ("both", "sequential"):
-        _, seq = serve_stream(fm, args.serve_requests, args.serve_rate, 1, 7, args.maxiters, pool)
+        seq = serve_stream(fm, args.serve_requests, args.serve_rate, 1, 7, args.maxiters)
         print(f"[bench] one at a time: p50 {seq['p50_ms']:.0f} ms p99 {seq['p99_ms']:.0f} ms",
               file=sys.stderr, flush=True)
-    return {"workload": "Poisson arrivals, grids " + ", ".join("x".join(map(str, g)) for g in SERVE_GRIDS)
+    return {"workload": "Poisson arrivals of scene requests (permittivity + plane wave in, cropped "
+                        "E and H out; potential, incident field, source term, H, incident add and "
+                        "crop on the GPU), grids " + ", ".join("x".join(map(str, g)) for g in SERVE_GRIDS)
                         + " (absorbing 16 px padding each side), 1-10 spheres n 1.36-1.60, "
                           "plane wave within NA 1.0, tol 1e-4 or 1e-6, Born FP32+refine",
             "continuous_batching": cb, "one_at_a_time": seq}
@@ -235,7 +238,7 @@ def main():
     ap.add_argument("--profile-reps", type=int, default=20)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-stream leg")
     ap.add_argument("--serve-requests", type=int, default=16)
-    ap.add_argument("--serve-rate", type=float, default=0.6, help="arrivals per second")
+    ap.add_argument("--serve-rate", type=float, default=0.75, help="arrivals per second")
     ap.add_argument("--serve-active", type=int, default=8)
     ap.add_argument("--serve-only", action="store_true")
     ap.add_argument("--serve-mode", default="both", choices=["both", "batched", "sequential"])
