@@ -6,6 +6,7 @@
 #include <string>
 
 #include "cbs_engine.h"
+#include "cbs_field.h"
 #include "cbs_scene.h"
 #include "cbs_serve.h"
 #include "cbs_solve.h"
@@ -475,6 +476,57 @@ cbs_status cbs_scene_build(cbs_ctx* ctx, const cbs_scene_desc* scene, const doub
   });
 }
 
+}  // extern "C"
+
+namespace {
+
+// the whole solve of a scene; the cropped totals stay in Eo / Ho on the device
+void solve_scene_dev(Ctx& c, const cbs_scene_desc& scene, const double* eps, int nsrc,
+                     const cbs_plane_wave* srcs, const cbs_solve_opts& opts, bool want_h_in,
+                     Scene& sc, DevBuf& Eo, DevBuf& Ho, bool& want_h, cbs_solve_stats* stats) {
+  hipStream_t s = c.stream;
+  cbs_problem_desc pd;
+  scene_setup(sc, &c, scene, eps, nsrc, srcs, pd, s);
+  validate_desc(pd);
+  Problem p;
+  problem_alloc(p, &c, pd, false);
+  scene_potential(sc, p.V64.as<Cx<double>>(), s);
+  cbs_solve_opts o = opts;
+  if (o.maxiters <= 0) o.maxiters = sc.maxiters;
+  o.h_scale_re = 0.0;
+  o.h_scale_im = scene_h_scale_im(scene);  // convergent_born.jl:797
+  if (o.precision != CBS_FP64) {
+    fill_tables<float>(p);
+    ensure_v32(p, s);
+  }
+  const size_t fb = (size_t)sc.nvox * 3 * sizeof(Cx<double>);
+  DevBuf Sd, Ed, Hd;
+  Sd.alloc(&c, fb);
+  Ed.alloc(&c, fb);
+  scene_source(sc, p.V64.as<Cx<double>>(), Sd.as<Cx<double>>(), s);
+  std::vector<RhsIO> io{RhsIO{&p, Sd.as<Cx<double>>(), Ed.as<Cx<double>>()}};
+  std::vector<cbs_solve_stats> st;
+  solve_set(&c, io, o, st, s);
+  Sd.release();
+  want_h = want_h_in && o.compute_h;
+  if (want_h) {
+    Hd.alloc(&c, fb);
+    curl_apply_dev<double>(p, Ed.as<Cx<double>>(), Hd.as<Cx<double>>(), o.h_scale_re,
+                           o.h_scale_im, s);
+  }
+  const size_t ob = (size_t)sc.nroi * 3 * sizeof(Cx<double>);
+  Eo.alloc(&c, ob);
+  if (want_h) Ho.alloc(&c, ob);
+  scene_finish(sc, Ed.as<Cx<double>>(), want_h ? Hd.as<Cx<double>>() : nullptr,
+               Eo.as<Cx<double>>(), want_h ? Ho.as<Cx<double>>() : nullptr, s);
+  CBS_HIP(hipStreamSynchronize(s));
+  if (stats) *stats = st[0];
+}
+
+}  // namespace
+
+extern "C" {
+
 cbs_status cbs_solve_scene(cbs_ctx* ctx, const cbs_scene_desc* scene, const double* eps,
                            int32_t nsrc, const cbs_plane_wave* srcs, const cbs_solve_opts* opts,
                            double* E_tot, double* H_tot, cbs_solve_stats* stats,
@@ -482,49 +534,159 @@ cbs_status cbs_solve_scene(cbs_ctx* ctx, const cbs_scene_desc* scene, const doub
   return guarded([&] {
     CBS_REQUIRE(ctx && scene && eps && opts && E_tot, "NULL argument");
     set_device(ctx->impl);
-    Ctx& c = ctx->impl;
-    hipStream_t s = c.stream;
+    hipStream_t s = ctx->impl.stream;
     Scene sc;
-    cbs_problem_desc pd;
-    scene_setup(sc, &c, *scene, eps, nsrc, srcs, pd, s);
-    validate_desc(pd);
-    Problem p;
-    problem_alloc(p, &c, pd, false);
-    scene_potential(sc, p.V64.as<Cx<double>>(), s);
-    cbs_solve_opts o = *opts;
-    if (o.maxiters <= 0) o.maxiters = sc.maxiters;
-    // -i lambda / (2 pi 377)  (convergent_born.jl:797)
-    o.h_scale_re = 0.0;
-    o.h_scale_im = -scene->wavelength / (2.0 * M_PI * 377.0);
-    if (o.precision != CBS_FP64) {
-      fill_tables<float>(p);
-      ensure_v32(p, s);
-    }
-    const size_t fb = (size_t)sc.nvox * 3 * sizeof(Cx<double>);
-    DevBuf Sd, Ed, Hd, Eo, Ho;
-    Sd.alloc(&c, fb);
-    Ed.alloc(&c, fb);
-    scene_source(sc, p.V64.as<Cx<double>>(), Sd.as<Cx<double>>(), s);
-    std::vector<RhsIO> io{RhsIO{&p, Sd.as<Cx<double>>(), Ed.as<Cx<double>>()}};
-    std::vector<cbs_solve_stats> st;
-    solve_set(&c, io, o, st, s);
-    Sd.release();
-    const bool want_h = H_tot && o.compute_h;
-    if (want_h) {
-      Hd.alloc(&c, fb);
-      curl_apply_dev<double>(p, Ed.as<Cx<double>>(), Hd.as<Cx<double>>(), o.h_scale_re,
-                             o.h_scale_im, s);
-    }
-    const size_t ob = (size_t)sc.nroi * 3 * sizeof(Cx<double>);
-    Eo.alloc(&c, ob);
-    if (want_h) Ho.alloc(&c, ob);
-    scene_finish(sc, Ed.as<Cx<double>>(), want_h ? Hd.as<Cx<double>>() : nullptr,
-                 Eo.as<Cx<double>>(), want_h ? Ho.as<Cx<double>>() : nullptr, s);
-    CBS_HIP(hipMemcpyAsync(E_tot, Eo.p, ob, hipMemcpyDeviceToHost, s));
-    if (want_h) CBS_HIP(hipMemcpyAsync(H_tot, Ho.p, ob, hipMemcpyDeviceToHost, s));
+    DevBuf Eo, Ho;
+    bool want_h = false;
+    solve_scene_dev(ctx->impl, *scene, eps, nsrc, srcs, *opts, H_tot != nullptr, sc, Eo, Ho,
+                    want_h, stats);
+    CBS_HIP(hipMemcpyAsync(E_tot, Eo.p, Eo.bytes, hipMemcpyDeviceToHost, s));
+    if (want_h) CBS_HIP(hipMemcpyAsync(H_tot, Ho.p, Ho.bytes, hipMemcpyDeviceToHost, s));
     CBS_HIP(hipStreamSynchronize(s));
-    if (stats) *stats = st[0];
     scene_info(sc, info);
+  });
+}
+
+cbs_status cbs_solve_scene_field(cbs_ctx* ctx, const cbs_scene_desc* scene, const double* eps,
+                                 int32_t nsrc, const cbs_plane_wave* srcs,
+                                 const cbs_solve_opts* opts, cbs_field** out,
+                                 cbs_solve_stats* stats, cbs_scene_info* info) {
+  return guarded([&] {
+    CBS_REQUIRE(ctx && scene && eps && opts && out, "NULL argument");
+    *out = nullptr;
+    set_device(ctx->impl);
+    std::unique_ptr<cbs_field> f(new cbs_field());
+    Scene sc;
+    bool want_h = false;
+    solve_scene_dev(ctx->impl, *scene, eps, nsrc, srcs, *opts, true, sc, f->impl.E, f->impl.H,
+                    want_h, stats);
+    FieldDev& fd = f->impl;
+    fd.ctx = &ctx->impl;
+    fd.nvox = sc.nroi;
+    for (int q = 0; q < 3; ++q) {
+      fd.n[q] = sc.g[q];
+      fd.res[q] = scene->res[q];
+    }
+    fd.wavelength = scene->wavelength;
+    fd.has_h = want_h;
+    scene_info(sc, info);
+    *out = f.release();
+  });
+}
+
+cbs_status cbs_field_create(cbs_ctx* ctx, const int64_t n[3], const double res[3],
+                            double wavelength, const double* E, const double* H, cbs_field** out) {
+  return guarded([&] {
+    CBS_REQUIRE(ctx && n && res && E && out, "NULL argument");
+    *out = nullptr;
+    set_device(ctx->impl);
+    std::unique_ptr<cbs_field> f(new cbs_field());
+    FieldDev& fd = f->impl;
+    fd.ctx = &ctx->impl;
+    fd.nvox = 1;
+    for (int q = 0; q < 3; ++q) {
+      CBS_REQUIRE(n[q] >= 1 && n[q] <= (1 << 16), "field sizes must be positive");
+      CBS_REQUIRE(res[q] > 0 && std::isfinite(res[q]), "resolution must be positive");
+      fd.n[q] = n[q];
+      fd.res[q] = res[q];
+      fd.nvox *= n[q];
+    }
+    CBS_REQUIRE(wavelength > 0 && std::isfinite(wavelength), "wavelength must be positive");
+    fd.wavelength = wavelength;
+    const size_t fb = (size_t)fd.nvox * 3 * sizeof(Cx<double>);
+    fd.E.alloc(&ctx->impl, fb);
+    CBS_HIP(hipMemcpy(fd.E.p, E, fb, hipMemcpyHostToDevice));
+    if (H) {
+      fd.H.alloc(&ctx->impl, fb);
+      CBS_HIP(hipMemcpy(fd.H.p, H, fb, hipMemcpyHostToDevice));
+      fd.has_h = true;
+    }
+    *out = f.release();
+  });
+}
+
+cbs_status cbs_field_destroy(cbs_field* f) {
+  return guarded([&] {
+    if (!f) return;
+    set_device(*f->impl.ctx);
+    CBS_HIP(hipStreamSynchronize(f->impl.ctx->stream));
+    delete f;
+  });
+}
+
+cbs_status cbs_field_get(cbs_field* f, double* E, double* H) {
+  return guarded([&] {
+    CBS_REQUIRE(f, "NULL argument");
+    FieldDev& fd = f->impl;
+    set_device(*fd.ctx);
+    const size_t fb = (size_t)fd.nvox * 3 * sizeof(Cx<double>);
+    if (E) CBS_HIP(hipMemcpy(E, fd.E.p, fb, hipMemcpyDeviceToHost));
+    if (H) {
+      CBS_REQUIRE(fd.has_h, "the field has no H");
+      CBS_HIP(hipMemcpy(H, fd.H.p, fb, hipMemcpyDeviceToHost));
+    }
+  });
+}
+
+cbs_status cbs_field_energy(cbs_field* f, double* energy) {
+  return guarded([&] {
+    CBS_REQUIRE(f && energy, "NULL argument");
+    set_device(*f->impl.ctx);
+    *energy = field_energy_dev(f->impl, f->impl.ctx->stream);
+  });
+}
+
+cbs_status cbs_field_intensity(cbs_field* f, int32_t component, double* out) {
+  return guarded([&] {
+    CBS_REQUIRE(f && out, "NULL argument");
+    FieldDev& fd = f->impl;
+    set_device(*fd.ctx);
+    hipStream_t s = fd.ctx->stream;
+    DevBuf d;
+    d.alloc(fd.ctx, (size_t)fd.nvox * sizeof(double));
+    field_intensity_dev(fd, component, d.as<double>(), s);
+    CBS_HIP(hipMemcpyAsync(out, d.p, d.bytes, hipMemcpyDeviceToHost, s));
+    CBS_HIP(hipStreamSynchronize(s));
+  });
+}
+
+cbs_status cbs_field_poynting(cbs_field* f, double* S) {
+  return guarded([&] {
+    CBS_REQUIRE(f && S, "NULL argument");
+    FieldDev& fd = f->impl;
+    set_device(*fd.ctx);
+    hipStream_t s = fd.ctx->stream;
+    DevBuf d;
+    d.alloc(fd.ctx, (size_t)fd.nvox * 3 * sizeof(Cx<double>));
+    field_poynting_dev(fd, d.as<Cx<double>>(), s);
+    CBS_HIP(hipMemcpyAsync(S, d.p, d.bytes, hipMemcpyDeviceToHost, s));
+    CBS_HIP(hipStreamSynchronize(s));
+  });
+}
+
+cbs_status cbs_field_extract_plane(cbs_field* f, int32_t plane_axis, int64_t plane_index,
+                                   double* E_plane, double* H_plane) {
+  return guarded([&] {
+    CBS_REQUIRE(f, "NULL argument");
+    FieldDev& fd = f->impl;
+    set_device(*fd.ctx);
+    CBS_REQUIRE(plane_axis >= 1 && plane_axis <= 3, "plane_axis must be 1, 2, or 3");
+    CBS_REQUIRE(plane_index >= 1 && plane_index <= fd.n[plane_axis - 1],
+                "plane_index out of bounds");
+    CBS_REQUIRE(!H_plane || fd.has_h, "the field has no H");
+    hipStream_t s = fd.ctx->stream;
+    const size_t pb = (size_t)(fd.nvox / fd.n[plane_axis - 1]) * 3 * sizeof(Cx<double>);
+    DevBuf d;
+    d.alloc(fd.ctx, pb);
+    for (int k = 0; k < 2; ++k) {
+      double* dst = k == 0 ? E_plane : H_plane;
+      if (!dst) continue;
+      const DevBuf& src = k == 0 ? fd.E : fd.H;
+      field_plane_dev(fd, src.as<Cx<double>>(), plane_axis - 1, plane_index - 1,
+                      d.as<Cx<double>>(), s);
+      CBS_HIP(hipMemcpyAsync(dst, d.p, pb, hipMemcpyDeviceToHost, s));
+      CBS_HIP(hipStreamSynchronize(s));
+    }
   });
 }
 

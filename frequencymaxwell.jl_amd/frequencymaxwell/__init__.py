@@ -11,9 +11,10 @@ from .boundary_conditions import (AbsorbingBoundaryCondition, AbstractBoundaryCo
                                   PeriodicBoundaryCondition, TanhProfile, TukeyProfile,
                                   padding_pixels, requires_averaging, requires_padding,
                                   subpixel_shift)
-from .engine import (CBSProblem, CBSServer, GreenSpec, make_opts, request_footprint,
+from .engine import (CBSProblem, CBSServer, DeviceField, GreenSpec, make_opts, request_footprint,
                      solve_batch as solve_problems)
-from .fields import ElectromagneticField, crop_to_ROI
+from .fields import (ElectromagneticField, crop_to_ROI, extract_plane, field_energy,
+                     field_intensity, poynting_vector)
 from .medium import Medium, grid_size, permittivity, permittivity_bg, phantom_bead
 from .solver import (CBSLinearOperator, CBSPreconditioner, ConvergentBornSolver, DyadicGreen,
                      compute_magnetic_field, compute_ROI, conv, domain_size, grid_spacing,
@@ -31,11 +32,12 @@ def is_backend_available(device) -> bool:
 __all__ = [
     "AbsorbingBoundaryCondition", "AbstractBoundaryCondition", "AttenuationProfile",
     "CBSError", "CBSLinearOperator", "CBSPreconditioner", "CBSProblem", "CBSServer", "Context",
+    "DeviceField",
     "ConvergentBornSolver", "DyadicGreen", "ElectromagneticField", "ExponentialProfile",
     "GreenSpec", "Medium", "PeriodicBoundaryCondition", "PlaneWaveSource", "TanhProfile",
     "TukeyProfile", "compute_ROI", "compute_magnetic_field", "conv", "crop_to_ROI",
-    "default_context", "domain_size", "generate_incident_field", "grid_size", "grid_spacing",
+    "default_context", "domain_size", "extract_plane", "field_energy", "field_intensity", "generate_incident_field", "grid_size", "grid_spacing",
     "h_scale", "is_backend_available", "load_library", "make_opts", "padding_pixels",
-    "permittivity", "permittivity_bg", "phantom_bead", "requires_averaging", "requires_padding",
+    "permittivity", "permittivity_bg", "phantom_bead", "poynting_vector", "requires_averaging", "requires_padding",
     "request_footprint", "reset", "solve", "solve_batch", "solve_problems", "source_wavelength", "subpixel_shift",
 ]

@@ -33,7 +33,9 @@ EXPORTED = (
     "cbs_problem_run", "cbs_problem_get_fields", "cbs_solve_multi", "cbs_solve_batch",
     "cbs_request_footprint", "cbs_server_start", "cbs_server_stop", "cbs_submit", "cbs_poll",
     "cbs_wait", "cbs_cancel", "cbs_time_sweeps", "cbs_stream_copy", "cbs_scene_build",
-    "cbs_solve_scene", "cbs_submit_scene",
+    "cbs_solve_scene", "cbs_submit_scene", "cbs_field_create", "cbs_field_destroy", "cbs_field_get",
+    "cbs_field_energy", "cbs_field_intensity", "cbs_field_poynting", "cbs_field_extract_plane",
+    "cbs_solve_scene_field",
 )
 
 
@@ -187,6 +189,16 @@ def _declare(lib):
                                 C.POINTER(SceneInfo)]),
         "cbs_submit_scene": (s, [_P, C.POINTER(SceneDesc), _D, C.c_int32, C.POINTER(PlaneWave),
                                  C.POINTER(SolveOpts), C.POINTER(C.c_uint64)]),
+        "cbs_field_create": (s, [_P, C.POINTER(C.c_int64), _D, C.c_double, _D, _D, C.POINTER(_P)]),
+        "cbs_field_destroy": (s, [_P]),
+        "cbs_field_get": (s, [_P, _D, _D]),
+        "cbs_field_energy": (s, [_P, _D]),
+        "cbs_field_intensity": (s, [_P, C.c_int32, _D]),
+        "cbs_field_poynting": (s, [_P, _D]),
+        "cbs_field_extract_plane": (s, [_P, C.c_int32, C.c_int64, _D, _D]),
+        "cbs_solve_scene_field": (s, [_P, C.POINTER(SceneDesc), _D, C.c_int32, C.POINTER(PlaneWave),
+                                      C.POINTER(SolveOpts), C.POINTER(_P), C.POINTER(SolveStats),
+                                      C.POINTER(SceneInfo)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -348,3 +348,106 @@ def solve_scene(desc: L.SceneDesc, eps, waves, opts: L.SolveOpts, want_h: bool =
 
 def _padded(desc) -> tuple:
     return tuple(int(desc.grid[d]) + 2 * int(desc.pad[d]) for d in range(3))
+
+
+# -- fields resident on the device (cbs_field_*) ------------------------------
+class DeviceField:
+    """E and H kept in HBM, with the reference's field analysis run on the GPU
+    (src/fields/electromagnetic_field.jl:97-237): field_energy,
+    field_intensity, poynting_vector, extract_plane."""
+
+    _COMP = {"total": 0, "x": 1, "y": 2, "z": 3}
+
+    def __init__(self, handle, shape, resolution, wavelength, has_h, ctx):
+        self.handle = handle
+        self.shape = tuple(int(x) for x in shape)
+        self.resolution = tuple(float(r) for r in resolution)
+        self.wavelength = float(wavelength)
+        self.has_h = bool(has_h)
+        self.ctx = ctx
+        self._lib = L.load()
+
+    @classmethod
+    def from_fields(cls, field, ctx=None) -> "DeviceField":
+        """Upload an ElectromagneticField (H may be absent)."""
+        ctx = ctx or L.default_context()
+        E = L.as_field(field.E)
+        H = L.as_field(field.H) if field.H is not None else None
+        n = (C.c_int64 * 3)(*E.shape[:3])
+        res = (C.c_double * 3)(*field.resolution)
+        h = C.c_void_p()
+        L.check(L.load().cbs_field_create(ctx.handle, n, res, float(field.wavelength), L.dptr(E),
+                                          L.dptr(H) if H is not None else None, C.byref(h)))
+        return cls(h, E.shape[:3], field.resolution, field.wavelength, H is not None, ctx)
+
+    def close(self):
+        if self.handle:
+            L.check(self._lib.cbs_field_destroy(self.handle))
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None):
+                self._lib.cbs_field_destroy(self.handle)
+        except Exception:
+            pass
+
+    def fields(self):
+        """Copy back: ElectromagneticField(E, H)."""
+        from .fields import ElectromagneticField
+
+        E = np.empty(self.shape + (3,), dtype=np.complex128, order="F")
+        H = np.empty(self.shape + (3,), dtype=np.complex128, order="F") if self.has_h else None
+        L.check(self._lib.cbs_field_get(self.handle, L.dptr(E), L.dptr(H) if H is not None else None))
+        return ElectromagneticField(E, H if H is not None else np.zeros_like(E), self.resolution,
+                                    self.wavelength)
+
+    def energy(self) -> float:
+        out = C.c_double()
+        L.check(self._lib.cbs_field_energy(self.handle, C.byref(out)))
+        return out.value
+
+    def intensity(self, component="total") -> np.ndarray:
+        c = str(component).lstrip(":")
+        if c not in self._COMP:
+            raise ValueError("component must be :total, :x, :y, or :z")
+        out = np.empty(self.shape, dtype=np.float64, order="F")
+        L.check(self._lib.cbs_field_intensity(self.handle, self._COMP[c],
+                                              out.ctypes.data_as(L._D)))
+        return out
+
+    def poynting(self) -> np.ndarray:
+        S = np.empty(self.shape + (3,), dtype=np.complex128, order="F")
+        L.check(self._lib.cbs_field_poynting(self.handle, L.dptr(S)))
+        return S
+
+    def extract_plane(self, plane_axis: int, plane_index: int):
+        """ElectromagneticField of the plane (1-based index) normal to plane_axis (1..3)."""
+        from .fields import ElectromagneticField
+
+        shp = list(self.shape)
+        if not 1 <= int(plane_axis) <= 3:
+            raise ValueError("plane_axis must be 1, 2, or 3")
+        shp[int(plane_axis) - 1] = 1
+        E = np.empty(tuple(shp) + (3,), dtype=np.complex128, order="F")
+        H = np.empty(tuple(shp) + (3,), dtype=np.complex128, order="F") if self.has_h else None
+        L.check(self._lib.cbs_field_extract_plane(self.handle, int(plane_axis), int(plane_index),
+                                                  L.dptr(E), L.dptr(H) if H is not None else None))
+        return ElectromagneticField(E, H if H is not None else np.zeros_like(E), self.resolution,
+                                    self.wavelength)
+
+
+def solve_scene_field(desc: L.SceneDesc, eps, waves, opts: L.SolveOpts, ctx=None):
+    """cbs_solve_scene with the cropped totals left on the device: returns
+    (DeviceField, stats, SceneInfo)."""
+    ctx = ctx or L.default_context()
+    g = tuple(int(x) for x in desc.grid)
+    eps = L.as_field(eps, g)
+    st = L.SolveStats()
+    info = L.SceneInfo()
+    h = C.c_void_p()
+    arr = (L.PlaneWave * len(waves))(*waves)
+    L.check(L.load().cbs_solve_scene_field(ctx.handle, C.byref(desc), L.dptr(eps), len(waves), arr,
+                                           C.byref(opts), C.byref(h), C.byref(st), C.byref(info)))
+    f = DeviceField(h, g, tuple(desc.res), desc.wavelength, bool(opts.compute_h), ctx)
+    return f, st, info

@@ -125,7 +125,17 @@ def compute_ROI(solver: ConvergentBornSolver):
     return solver.compute_ROI()
 
 
-def domain_size(solver):
+def domain_size(obj):
+    """domain_size of a solver (:1015-1021) or of an ElectromagneticField
+    (electromagnetic_field.jl:97-100)."""
+    if isinstance(obj, ElectromagneticField):
+        from .fields import field_domain_size
+
+        return field_domain_size(obj)
+    return _solver_domain_size(obj)
+
+
+def _solver_domain_size(solver):
     return solver.domain_size()
 
 

@@ -249,6 +249,35 @@ cbs_status cbs_solve_scene(cbs_ctx *ctx, const cbs_scene_desc *scene, const doub
                            double *E_tot, double *H_tot, cbs_solve_stats *stats,
                            cbs_scene_info *info);
 
+/* ---- fields kept on the device ---------------------------------------
+ * The reference's field analysis (src/fields/electromagnetic_field.jl:97-237)
+ * on fields resident in HBM, so a caller can take a scalar, an intensity map
+ * or a plane instead of two full 3-D fields.  Arrays are (nx, ny, nz, 3)
+ * complex column-major like every field here. */
+typedef struct cbs_field cbs_field;
+/* Upload E and H (H may be NULL: no H-based analysis then). */
+cbs_status cbs_field_create(cbs_ctx *ctx, const int64_t n[3], const double res[3],
+                            double wavelength, const double *E, const double *H,
+                            cbs_field **out);
+cbs_status cbs_field_destroy(cbs_field *f);
+/* Copy the fields back (either pointer may be NULL). */
+cbs_status cbs_field_get(cbs_field *f, double *E, double *H);
+/* field_energy (:112-127): (eps0 sum|E|^2 + mu0 sum|H|^2) / 2 * voxel volume. */
+cbs_status cbs_field_energy(cbs_field *f, double *energy);
+/* field_intensity (:162-180): component 0 = total, 1/2/3 = x/y/z; out: nx*ny*nz doubles. */
+cbs_status cbs_field_intensity(cbs_field *f, int32_t component, double *out);
+/* poynting_vector (:135-154): S = E x conj(H); out: nx*ny*nz*3 complex. */
+cbs_status cbs_field_poynting(cbs_field *f, double *S);
+/* extract_plane (:192-235): plane_axis 1..3, plane_index 1-based; E_plane and
+ * H_plane (either may be NULL) have the field's shape with that axis of length 1. */
+cbs_status cbs_field_extract_plane(cbs_field *f, int32_t plane_axis, int64_t plane_index,
+                                   double *E_plane, double *H_plane);
+/* cbs_solve_scene leaving the cropped total fields on the device. */
+cbs_status cbs_solve_scene_field(cbs_ctx *ctx, const cbs_scene_desc *scene, const double *eps,
+                                 int32_t nsrc, const cbs_plane_wave *srcs,
+                                 const cbs_solve_opts *opts, cbs_field **out,
+                                 cbs_solve_stats *stats, cbs_scene_info *info);
+
 /* ---- serving: continuous batching with HBM-footprint admission ----------
  * No counterpart in the reference (it solves one problem per call).  A
  * server thread owns a HIP stream; admitted requests share every sweep

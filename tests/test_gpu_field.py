@@ -1,0 +1,83 @@
+"""GPU field analysis on resident fields (cbs_field_*, cbs_solve_scene_field)
+against the host mirror of src/fields/electromagnetic_field.jl:97-237:
+intensity and planes bit for bit (same operations), Poynting vector to
+1e-14 (fused complex products), energy to 1e-12 (summation order); a scene
+solve left on the device equals the downloaded one bit for bit.
+"""
+
+import numpy as np
+import pytest
+
+from cases import RES, rel, scene_bcs, scene_medium, scene_sources
+
+pytestmark = pytest.mark.gpu
+
+
+def _random_field(fm, shape, seed, with_h=True):
+    rng = np.random.default_rng(seed)
+    E = np.asfortranarray(rng.standard_normal(shape + (3,)) + 1j * rng.standard_normal(shape + (3,)))
+    H = np.asfortranarray(rng.standard_normal(shape + (3,)) + 1j * rng.standard_normal(shape + (3,)))
+    return fm.ElectromagneticField(E, H if with_h else np.zeros_like(E), (30e-9, 40e-9, 50e-9), 532e-9)
+
+
+@pytest.mark.parametrize("shape", [(7, 5, 3), (32, 24, 16)])
+def test_device_field_analysis_matches_host(fm, shape):
+    print(f"case device field analysis {shape}", flush=True)
+    f = _random_field(fm, shape, 5)
+    d = fm.DeviceField.from_fields(f)
+    try:
+        assert abs(d.energy() - fm.field_energy(f)) <= 1e-12 * fm.field_energy(f)
+        for comp in ("total", "x", "y", "z"):
+            assert np.array_equal(d.intensity(comp), fm.field_intensity(f, comp)), comp
+        assert rel(d.poynting(), fm.poynting_vector(f)) <= 1e-14
+        for axis in (1, 2, 3):
+            for idx in (1, shape[axis - 1]):
+                p_d, p_h = d.extract_plane(axis, idx), fm.extract_plane(f, axis, idx)
+                assert np.array_equal(p_d.E, p_h.E) and np.array_equal(p_d.H, p_h.H)
+        back = d.fields()
+        assert np.array_equal(back.E, f.E) and np.array_equal(back.H, f.H)
+        with pytest.raises(ValueError):
+            d.extract_plane(2, shape[1] + 1)
+    finally:
+        d.close()
+
+
+def test_device_field_without_h(fm):
+    f = _random_field(fm, (6, 6, 4), 6)
+    E_only = fm.ElectromagneticField(f.E, f.H, f.resolution, f.wavelength)
+    E_only.H = None
+    d = fm.DeviceField.from_fields(E_only)
+    try:
+        assert np.array_equal(d.intensity(), fm.field_intensity(f))
+        with pytest.raises(ValueError):
+            d.poynting()
+    finally:
+        d.close()
+
+
+def test_solve_scene_field_resident(fm, oracle):
+    """A scene solved with its totals kept on the device: the same fields as
+    cbs_solve_scene, and the analysis of them equals the host analysis."""
+    print("case solve scene field", flush=True)
+    from frequencymaxwell import engine
+    from frequencymaxwell import solver as S
+
+    grid = (16, 12, 10)
+    bf, _ = scene_bcs(fm, oracle, "absorbing")
+    srcf, _ = scene_sources(fm, oracle, True)
+    med = scene_medium(fm, grid, 7)
+    solver = fm.ConvergentBornSolver(resolution=RES, grid_size=grid, boundary_conditions=bf)
+    d, eps, waves, keep = S.scene(solver, srcf, med)
+    o = fm.make_opts(method="born", tol=0.0, maxiters=30)
+    E, H, st, info = engine.solve_scene(d, eps, waves, o)
+    fd, st2, info2 = engine.solve_scene_field(d, eps, waves, o)
+    try:
+        assert st2.iters == st.iters == 30
+        back = fd.fields()
+        assert np.array_equal(back.E, E) and np.array_equal(back.H, H)
+        host = fm.ElectromagneticField(E, H, RES, 500e-9)
+        assert abs(fd.energy() - fm.field_energy(host)) <= 1e-12 * fm.field_energy(host)
+        assert np.array_equal(fd.intensity(), fm.field_intensity(host))
+        assert np.array_equal(fd.extract_plane(3, 5).E, fm.extract_plane(host, 3, 5).E)
+    finally:
+        fd.close()

@@ -11,6 +11,7 @@
 """
 
 import ctypes
+import math
 import re
 import shutil
 import subprocess
@@ -241,3 +242,54 @@ def test_request_footprint_without_gpu(fm):
 def test_no_gpu_fails_loudly(fm):
     with pytest.raises((fm.CBSError, ValueError)):
         fm.Context(0)
+
+
+# ---------------------------------------------------------------------------
+# field analysis (src/fields/electromagnetic_field.jl:97-237), host mirror
+# ---------------------------------------------------------------------------
+def test_field_analysis_reference_cases(fm):
+    """test/fields/electromagnetic_field.jl "Field Utilities" and "Field Domain
+    Properties", with the closed forms of a uniform field."""
+    E = np.ones((16, 16, 8, 3), dtype=np.complex128) * 0.1
+    H = np.ones((16, 16, 8, 3), dtype=np.complex128) * 0.01
+    res = (50e-9, 50e-9, 100e-9)
+    f = fm.ElectromagneticField(E, H, res, 633e-9)
+    n = 16 * 16 * 8
+    want = (8.854187817e-12 * 3 * n * 0.01 + 4e-7 * math.pi * 3 * n * 1e-4) / 2 * (res[0] * res[1] * res[2])
+    assert abs(fm.field_energy(f) - want) <= 1e-14 * want
+    inten = fm.field_intensity(f)
+    assert inten.shape == (16, 16, 8) and np.allclose(inten, 0.03, rtol=1e-15, atol=0)
+    assert np.allclose(fm.field_intensity(f, "y"), 0.01, rtol=1e-15, atol=0)
+    assert fm.poynting_vector(f).shape == (16, 16, 8, 3)
+    assert np.all(fm.poynting_vector(f) == 0)  # parallel E and H
+    g = fm.ElectromagneticField(np.zeros((20, 30, 10, 3), complex), np.zeros((20, 30, 10, 3), complex),
+                                (25e-9, 40e-9, 80e-9), 500e-9)
+    assert np.allclose(fm.domain_size(g), (20 * 25e-9, 30 * 40e-9, 10 * 80e-9), rtol=1e-15)
+    with pytest.raises(ValueError):
+        fm.field_intensity(f, "w")
+    with pytest.raises(ValueError):
+        fm.extract_plane(f, 4, 1)
+    with pytest.raises(ValueError):
+        fm.extract_plane(f, 3, 9)
+
+
+def test_field_analysis_plane_wave(fm):
+    """x-polarised plane wave along z: S = z_hat |A|^2 / Z, |E|^2 = |A|^2 in x only,
+    and a plane keeps the field's values."""
+    n = (6, 5, 8)
+    z = np.arange(n[2]) * 50e-9
+    A = 0.7 * np.exp(1j * 2 * np.pi * z / 500e-9)
+    Z = 377.0 / 1.33
+    E = np.zeros(n + (3,), complex)
+    H = np.zeros(n + (3,), complex)
+    E[..., 0] = A[None, None, :]
+    H[..., 1] = A[None, None, :] / Z
+    f = fm.ElectromagneticField(E, H, (50e-9,) * 3, 500e-9)
+    S = fm.poynting_vector(f)
+    assert np.allclose(S[..., 2], 0.49 / Z, rtol=1e-14, atol=0)
+    assert np.all(S[..., 0] == 0) and np.all(S[..., 1] == 0)
+    assert np.allclose(fm.field_intensity(f, "x"), 0.49, rtol=1e-14)
+    assert np.all(fm.field_intensity(f, "z") == 0)
+    p = fm.extract_plane(f, 3, 4)
+    assert p.E.shape == (6, 5, 1, 3) and np.array_equal(p.E[:, :, 0], E[:, :, 3])
+    assert np.array_equal(fm.extract_plane(f, 1, 2).H[0], H[1])
