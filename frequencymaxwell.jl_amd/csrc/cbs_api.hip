@@ -547,6 +547,77 @@ cbs_status cbs_solve_scene(cbs_ctx* ctx, const cbs_scene_desc* scene, const doub
   });
 }
 
+cbs_status cbs_solve_scene_multi(cbs_ctx* ctx, const cbs_scene_desc* scene, const double* eps,
+                                 int32_t nrhs, const int32_t* nsrc, const cbs_plane_wave* srcs,
+                                 const cbs_solve_opts* opts, double** E_tot, double** H_tot,
+                                 cbs_solve_stats* stats, cbs_scene_info* info) {
+  return guarded([&] {
+    CBS_REQUIRE(ctx && scene && eps && nsrc && srcs && opts, "NULL argument");
+    CBS_REQUIRE(nrhs >= 1, "nrhs must be >= 1");
+    set_device(ctx->impl);
+    Ctx& c = ctx->impl;
+    hipStream_t s = c.stream;
+    int total = 0;
+    for (int i = 0; i < nrhs; ++i) {
+      CBS_REQUIRE(nsrc[i] >= 1, "every right-hand side needs a plane wave");
+      total += nsrc[i];
+    }
+    Scene sc;
+    cbs_problem_desc pd;
+    scene_setup(sc, &c, *scene, eps, total, srcs, pd, s);  // validates every wave
+    validate_desc(pd);
+    Problem p;
+    problem_alloc(p, &c, pd, false);
+    scene_potential(sc, p.V64.as<Cx<double>>(), s);
+    cbs_solve_opts o = *opts;
+    if (o.maxiters <= 0) o.maxiters = sc.maxiters;
+    o.h_scale_re = 0.0;
+    o.h_scale_im = scene_h_scale_im(*scene);
+    if (o.precision != CBS_FP64) {
+      fill_tables<float>(p);
+      ensure_v32(p, s);
+    }
+    const size_t fb = (size_t)sc.nvox * 3 * sizeof(Cx<double>);
+    std::vector<DevBuf> Sd(nrhs), Ed(nrhs);
+    std::vector<RhsIO> io;
+    const cbs_plane_wave* dw = sc.srcs.as<cbs_plane_wave>();
+    std::vector<int> first(nrhs);
+    for (int i = 0, acc = 0; i < nrhs; acc += nsrc[i], ++i) {
+      first[i] = acc;
+      Sd[i].alloc(&c, fb);
+      Ed[i].alloc(&c, fb);
+      scene_source(sc, p.V64.as<Cx<double>>(), Sd[i].as<Cx<double>>(), s, dw + acc, nsrc[i]);
+      io.push_back(RhsIO{&p, Sd[i].as<Cx<double>>(), Ed[i].as<Cx<double>>()});
+    }
+    std::vector<cbs_solve_stats> st;
+    solve_set(&c, io, o, st, s);
+    for (auto& b : Sd) b.release();
+    const size_t ob = (size_t)sc.nroi * 3 * sizeof(Cx<double>);
+    DevBuf Hd, Eo, Ho;
+    Eo.alloc(&c, ob);
+    for (int i = 0; i < nrhs; ++i) {
+      const bool want_e = E_tot && E_tot[i];
+      const bool want_h = H_tot && H_tot[i] && o.compute_h;
+      if (!want_e && !want_h) continue;
+      if (want_h) {
+        Hd.ensure(&c, fb);
+        Ho.ensure(&c, ob);
+        curl_apply_dev<double>(p, Ed[i].as<Cx<double>>(), Hd.as<Cx<double>>(), o.h_scale_re,
+                               o.h_scale_im, s);
+      }
+      scene_finish(sc, Ed[i].as<Cx<double>>(), want_h ? Hd.as<Cx<double>>() : nullptr,
+                   Eo.as<Cx<double>>(), want_h ? Ho.as<Cx<double>>() : nullptr, s,
+                   dw + first[i], nsrc[i]);
+      if (want_e) CBS_HIP(hipMemcpyAsync(E_tot[i], Eo.p, ob, hipMemcpyDeviceToHost, s));
+      if (want_h) CBS_HIP(hipMemcpyAsync(H_tot[i], Ho.p, ob, hipMemcpyDeviceToHost, s));
+      CBS_HIP(hipStreamSynchronize(s));
+    }
+    if (stats)
+      for (int i = 0; i < nrhs; ++i) stats[i] = st[i];
+    scene_info(sc, info);
+  });
+}
+
 cbs_status cbs_solve_scene_field(cbs_ctx* ctx, const cbs_scene_desc* scene, const double* eps,
                                  int32_t nsrc, const cbs_plane_wave* srcs,
                                  const cbs_solve_opts* opts, cbs_field** out,

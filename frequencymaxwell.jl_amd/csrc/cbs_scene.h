@@ -39,12 +39,15 @@ void scene_setup(Scene& sc, Ctx* c, const cbs_scene_desc& d, const double* eps, 
                  const cbs_plane_wave* srcs, cbs_problem_desc& pd, hipStream_t s);
 // V = (pad(k_bg^2 (eps/eps_bg - 1)) - i eps_imag) * attenuation filters
 void scene_potential(const Scene& sc, Cx<double>* V, hipStream_t s);
-// S = V E_inc + i eps_imag E_inc on the padded grid
-void scene_source(const Scene& sc, const Cx<double>* V, Cx<double>* S, hipStream_t s);
+// S = V E_inc + i eps_imag E_inc on the padded grid (waves: a device list of
+// nw plane waves in place of the scene's own)
+void scene_source(const Scene& sc, const Cx<double>* V, Cx<double>* S, hipStream_t s,
+                  const cbs_plane_wave* waves = nullptr, int nw = 0);
 // E_out = crop(E_scat + E_inc), H_out = crop(H_scat + H_inc) (ROI, (gx,gy,gz,3));
 // H_scat may be null (H_out then untouched)
 void scene_finish(const Scene& sc, const Cx<double>* E_scat, const Cx<double>* H_scat,
-                  Cx<double>* E_out, Cx<double>* H_out, hipStream_t s);
+                  Cx<double>* E_out, Cx<double>* H_out, hipStream_t s,
+                  const cbs_plane_wave* waves = nullptr, int nw = 0);
 void scene_info(const Scene& sc, cbs_scene_info* info);
 void validate_scene(const cbs_scene_desc& d, int nsrc, const cbs_plane_wave* srcs);
 // -i wavelength / (2 pi 377)  (convergent_born.jl:797)

@@ -290,16 +290,27 @@ void scene_potential(const Scene& sc, Cx<double>* V, hipStream_t s) {
   CBS_HIP(hipGetLastError());
 }
 
-void scene_source(const Scene& sc, const Cx<double>* V, Cx<double>* S, hipStream_t s) {
-  hipLaunchKernelGGL(k_source, dim3(nblk(sc.nvox)), dim3(kThreads), 0, s, scene_k(sc), sc.nvox,
-                     V, S);
+void scene_source(const Scene& sc, const Cx<double>* V, Cx<double>* S, hipStream_t s,
+                  const cbs_plane_wave* waves, int nw) {
+  SceneK k = scene_k(sc);
+  if (waves) {
+    k.srcs = waves;
+    k.nsrc = nw;
+  }
+  hipLaunchKernelGGL(k_source, dim3(nblk(sc.nvox)), dim3(kThreads), 0, s, k, sc.nvox, V, S);
   CBS_HIP(hipGetLastError());
 }
 
 void scene_finish(const Scene& sc, const Cx<double>* E_scat, const Cx<double>* H_scat,
-                  Cx<double>* E_out, Cx<double>* H_out, hipStream_t s) {
-  hipLaunchKernelGGL(k_finish, dim3(nblk(sc.nroi)), dim3(kThreads), 0, s, scene_k(sc), sc.nvox,
-                     sc.nroi, E_scat, H_scat, E_out, H_out);
+                  Cx<double>* E_out, Cx<double>* H_out, hipStream_t s,
+                  const cbs_plane_wave* waves, int nw) {
+  SceneK k = scene_k(sc);
+  if (waves) {
+    k.srcs = waves;
+    k.nsrc = nw;
+  }
+  hipLaunchKernelGGL(k_finish, dim3(nblk(sc.nroi)), dim3(kThreads), 0, s, k, sc.nvox, sc.nroi,
+                     E_scat, H_scat, E_out, H_out);
   CBS_HIP(hipGetLastError());
 }
 

@@ -35,7 +35,7 @@ EXPORTED = (
     "cbs_wait", "cbs_cancel", "cbs_time_sweeps", "cbs_stream_copy", "cbs_scene_build",
     "cbs_solve_scene", "cbs_submit_scene", "cbs_field_create", "cbs_field_destroy", "cbs_field_get",
     "cbs_field_energy", "cbs_field_intensity", "cbs_field_poynting", "cbs_field_extract_plane",
-    "cbs_solve_scene_field",
+    "cbs_solve_scene_field", "cbs_solve_scene_multi",
 )
 
 
@@ -196,6 +196,9 @@ def _declare(lib):
         "cbs_field_intensity": (s, [_P, C.c_int32, _D]),
         "cbs_field_poynting": (s, [_P, _D]),
         "cbs_field_extract_plane": (s, [_P, C.c_int32, C.c_int64, _D, _D]),
+        "cbs_solve_scene_multi": (s, [_P, C.POINTER(SceneDesc), _D, C.c_int32, C.POINTER(C.c_int32),
+                                      C.POINTER(PlaneWave), C.POINTER(SolveOpts), C.POINTER(_D),
+                                      C.POINTER(_D), C.POINTER(SolveStats), C.POINTER(SceneInfo)]),
         "cbs_solve_scene_field": (s, [_P, C.POINTER(SceneDesc), _D, C.c_int32, C.POINTER(PlaneWave),
                                       C.POINTER(SolveOpts), C.POINTER(_P), C.POINTER(SolveStats),
                                       C.POINTER(SceneInfo)]),

@@ -451,3 +451,27 @@ def solve_scene_field(desc: L.SceneDesc, eps, waves, opts: L.SolveOpts, ctx=None
                                            C.byref(opts), C.byref(h), C.byref(st), C.byref(info)))
     f = DeviceField(h, g, tuple(desc.res), desc.wavelength, bool(opts.compute_h), ctx)
     return f, st, info
+
+
+def solve_scene_multi(desc: L.SceneDesc, eps, waves_per_rhs, opts: L.SolveOpts, want_e=True,
+                      want_h=True, ctx=None):
+    """Many right-hand sides of one scene in shared launches: RHS i is the
+    coherent sum of the plane waves waves_per_rhs[i].  Returns (Es, Hs, stats,
+    info); Es / Hs are lists of cropped totals (None entries when not wanted)."""
+    ctx = ctx or L.default_context()
+    g = tuple(int(x) for x in desc.grid)
+    eps = L.as_field(eps, g)
+    n = len(waves_per_rhs)
+    flat = [w for ws in waves_per_rhs for w in ws]
+    counts = (C.c_int32 * n)(*[len(ws) for ws in waves_per_rhs])
+    arr = (L.PlaneWave * len(flat))(*flat)
+    Es = [np.empty(g + (3,), dtype=np.complex128, order="F") if want_e else None for _ in range(n)]
+    Hs = [np.empty(g + (3,), dtype=np.complex128, order="F") if want_h else None for _ in range(n)]
+    ArrD = L._D * n
+    pe = ArrD(*[L.dptr(a) if a is not None else None for a in Es])
+    ph = ArrD(*[L.dptr(a) if a is not None else None for a in Hs])
+    sts = (L.SolveStats * n)()
+    info = L.SceneInfo()
+    L.check(L.load().cbs_solve_scene_multi(ctx.handle, C.byref(desc), L.dptr(eps), n, counts, arr,
+                                           C.byref(opts), pe, ph, sts, C.byref(info)))
+    return Es, Hs, list(sts), info

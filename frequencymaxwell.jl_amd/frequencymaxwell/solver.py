@@ -390,6 +390,31 @@ def solve_batch(solver: ConvergentBornSolver, sources_list, medium: Medium):
     for s in sources_list[1:]:
         if source_wavelength(s) != lam:
             raise ValueError("solve_batch needs one wavelength for all right-hand sides")
+    if solver.device_setup:
+        from .engine import solve_scene_multi
+        from .sources import AbstractCurrentSource
+
+        groups = [[s] if isinstance(s, AbstractCurrentSource) else list(s) for s in sources_list]
+        flat = [s for g in groups for s in g]
+        d, eps, waves, keep = scene(solver, flat, medium)
+        per, i = [], 0
+        for g in groups:
+            per.append(waves[i:i + len(g)])
+            i += len(g)
+        opts = _solve_opts(solver, 0, lam)
+        if solver.maxiters is None:
+            opts.maxiters = 0  # the library applies max(2 Bornmax, 100)
+        Es, Hs, sts, info = solve_scene_multi(d, eps, per, opts)
+        del keep
+        solver.Green_function = None
+        solver.Bornmax = int(info.bornmax)
+        solver.iteration_count = 0
+        solver.residual_history = []
+        out = []
+        for E, H, st in zip(Es, Hs, sts):
+            _record(solver, st)
+            out.append(ElectromagneticField(E, H, solver.resolution, lam))
+        return out
     lam, pot, eps_imag, bornmax, problem = _prepare(solver, sources_list[0], medium)
     incs = [generate_incident_field(s, solver, permittivity_bg=permittivity_bg(medium))
             for s in sources_list]

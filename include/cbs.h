@@ -249,6 +249,17 @@ cbs_status cbs_solve_scene(cbs_ctx *ctx, const cbs_scene_desc *scene, const doub
                            double *E_tot, double *H_tot, cbs_solve_stats *stats,
                            cbs_scene_info *info);
 
+/* Many right-hand sides of one scene (illumination angles x polarisations):
+ * RHS i is the coherent sum of nsrc[i] plane waves, taken in order from
+ * srcs.  One potential, one source term per RHS built on the device, every
+ * sweep one launch over all of them; each RHS stops on its own.  E_tot[i] and
+ * H_tot[i] are ROI-cropped totals; H_tot (or any entry, or E_tot[i]) may be
+ * NULL to skip that download.  stats may be NULL or hold nrhs entries. */
+cbs_status cbs_solve_scene_multi(cbs_ctx *ctx, const cbs_scene_desc *scene, const double *eps,
+                                 int32_t nrhs, const int32_t *nsrc, const cbs_plane_wave *srcs,
+                                 const cbs_solve_opts *opts, double **E_tot, double **H_tot,
+                                 cbs_solve_stats *stats, cbs_scene_info *info);
+
 /* ---- fields kept on the device ---------------------------------------
  * The reference's field analysis (src/fields/electromagnetic_field.jl:97-237)
  * on fields resident in HBM, so a caller can take a scalar, an intensity map

@@ -106,3 +106,29 @@ def test_solve_scene_fp32_refine(fm, oracle):
                        maxiters=2000)
     assert ref.stats.converged
     assert rel(out.E, ref.E) <= 1e-5
+
+
+def test_solve_batch_device_equals_single(fm, oracle):
+    """solve_batch on the device (cbs_solve_scene_multi): every right-hand side
+    (two single plane waves and one coherent pair) equals its own solve bit for
+    bit, and the oracle's whole solve to 1e-10."""
+    print("case solve batch device", flush=True)
+    grid = (16, 12, 10)
+    bf, bo = _bcs(fm, oracle, "absorbing")
+    srcf, srco = _sources(fm, oracle, True)
+    single = fm.PlaneWaveSource(wavelength=500e-9, polarization=[0, 1, 0], k_vector=[0, 0, 1])
+    med = _medium(fm, grid, 9)
+    kw = dict(resolution=RES, grid_size=grid, boundary_conditions=bf, tolerance=0.0,
+              linear_solver="born", stop_rule="rhs", maxiters=25)
+    batch = [srcf[0], single, srcf]
+    solver = fm.ConvergentBornSolver(**kw)
+    with pytest.warns(UserWarning):
+        outs = fm.solve_batch(solver, batch, med)
+    for src, out in zip(batch, outs):
+        one = fm.ConvergentBornSolver(**kw)
+        with pytest.warns(UserWarning):
+            ref = fm.solve(one, src, med)
+        assert np.array_equal(out.E, ref.E) and np.array_equal(out.H, ref.H)
+    s = oracle.Solver(resolution=RES, grid_size=grid, boundary_conditions=bo, tolerance=0.0)
+    ref = oracle.solve(s, srco, med.permittivity, N_BG ** 2, method="born", rule="rhs", maxiters=25)
+    assert rel(outs[2].E, ref.E) <= 1e-10 and rel(outs[2].H, ref.H) <= 1e-10
