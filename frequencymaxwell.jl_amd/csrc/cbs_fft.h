@@ -125,17 +125,17 @@ __device__ __forceinline__ void dft(Cx<T>* v, const Cx<T>* __restrict__ tw, int 
 // ---------------------------------------------------------------------------
 // compile-time transform: in place, registers hold one stage's inputs
 // ---------------------------------------------------------------------------
-template <class T, int N, int Q, int QS, int DIR, int R, int NS>
+template <class T, int N, int Q, int QS, int DIR, int R, int NS, int NT>
 __device__ __forceinline__ void stage_ct(Cx<T>* __restrict__ s, const Cx<T>* __restrict__ tw,
                                          int tid) {
   constexpr int NB = N / R;
   constexpr int TOTAL = Q * NB;
-  constexpr int RD = (TOTAL + kThreads - 1) / kThreads;
-  constexpr bool EXACT = (TOTAL % kThreads) == 0;
+  constexpr int RD = (TOTAL + NT - 1) / NT;
+  constexpr bool EXACT = (TOTAL % NT) == 0;
   Cx<T> v[RD][R];
 #pragma unroll
   for (int k = 0; k < RD; ++k) {
-    const int b = tid + k * kThreads;
+    const int b = tid + k * NT;
     if (EXACT || b < TOTAL) {
       const int q = b % Q, t = b / Q;
 #pragma unroll
@@ -145,7 +145,7 @@ __device__ __forceinline__ void stage_ct(Cx<T>* __restrict__ s, const Cx<T>* __r
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < RD; ++k) {
-    const int b = tid + k * kThreads;
+    const int b = tid + k * NT;
     if (EXACT || b < TOTAL) {
       const int q = b % Q, t = b / Q;
       const int j = t % NS;
@@ -166,13 +166,13 @@ __device__ __forceinline__ void stage_ct(Cx<T>* __restrict__ s, const Cx<T>* __r
   __syncthreads();
 }
 
-template <class T, int N, int Q, int QS, int DIR, int S = 0, int NS = 1>
+template <class T, int N, int Q, int QS, int DIR, int NT = kThreads, int S = 0, int NS = 1>
 __device__ __forceinline__ void fft_ct(Cx<T>* __restrict__ s, const Cx<T>* __restrict__ tw,
                                        int tid) {
   if constexpr (S < plan_nst(N)) {
     constexpr int R = plan_radix(N, S);
-    stage_ct<T, N, Q, QS, DIR, R, NS>(s, tw, tid);
-    fft_ct<T, N, Q, QS, DIR, S + 1, NS * R>(s, tw, tid);
+    stage_ct<T, N, Q, QS, DIR, R, NS, NT>(s, tw, tid);
+    fft_ct<T, N, Q, QS, DIR, NT, S + 1, NS * R>(s, tw, tid);
   }
 }
 
@@ -227,7 +227,7 @@ __device__ __forceinline__ void bfly_rt_generic(const Cx<T>* __restrict__ a,
   }
 }
 
-template <class T, int DIR>
+template <class T, int DIR, int NT = kThreads>
 __device__ Cx<T>* fft_rt(Cx<T>* a, Cx<T>* b, int n, int Q, int QS, const int* __restrict__ plan,
                          const Cx<T>* __restrict__ tw, int tid) {
   const int nst = plan[0];
@@ -235,7 +235,7 @@ __device__ Cx<T>* fft_rt(Cx<T>* a, Cx<T>* b, int n, int Q, int QS, const int* __
   for (int st = 0; st < nst; ++st) {
     const int R = plan[1 + st];
     const int total = Q * (n / R);
-    for (int bi = tid; bi < total; bi += kThreads) {
+    for (int bi = tid; bi < total; bi += NT) {
       const int q = bi % Q, t = bi / Q;
       switch (R) {
         case 2: bfly_rt<T, DIR, 2>(a, b, tw, n, QS, q, t, NS); break;

@@ -7,7 +7,13 @@
 
 namespace cbs {
 
-constexpr int kThreads = 256;     // threads per block of every pass kernel
+constexpr int kThreads = 256;     // threads per block of the x and y pass kernels
+#ifndef CBS_ZTHREADS
+#define CBS_ZTHREADS 512
+#endif
+constexpr int kZThreads = CBS_ZTHREADS;  // threads per block of the z pass (its tile holds
+                                         // three components: more waves per tile, fewer
+                                         // registers per thread)
 constexpr int kMaxStages = 12;    // radix stages of one axis transform
 constexpr int kLdsBudget = 65536; // LDS bytes a pass tile may use (two blocks per CU)
 constexpr int kNumPart = 5;       // doubles of partial sums written per xinv tile

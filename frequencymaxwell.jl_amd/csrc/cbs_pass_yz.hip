@@ -27,7 +27,7 @@ k_ypass(const DevProb<T>* __restrict__ probs, const int* __restrict__ bstart, in
 }
 
 template <class T, int NC>
-__global__ void __launch_bounds__(kThreads)
+__global__ void __launch_bounds__(kZThreads)
 k_zpass(const DevProb<T>* __restrict__ probs, const int* __restrict__ bstart, int nprob) {
   const int pi = find_prob(bstart, nprob, blockIdx.x);
   const DevProb<T>& pr = probs[pi];
@@ -35,14 +35,14 @@ k_zpass(const DevProb<T>* __restrict__ probs, const int* __restrict__ bstart, in
   const int tile = blockIdx.x - bstart[pi];
   const int tid = threadIdx.x;
   if constexpr (NC != 0) {
-    zpass_tile<T, NC>(pr, tile, tid);
+    zpass_tile<T, NC, kZThreads>(pr, tile, tid);
   } else {
     switch (pr.rt[2] ? 0 : pr.n[2]) {
 #define CBS_CASE_(L) \
-      case L: zpass_tile<T, L>(pr, tile, tid); break;
+      case L: zpass_tile<T, L, kZThreads>(pr, tile, tid); break;
       CBS_CT_LENGTHS(CBS_CASE_)
 #undef CBS_CASE_
-      default: zpass_tile<T, 0>(pr, tile, tid); break;
+      default: zpass_tile<T, 0, kZThreads>(pr, tile, tid); break;
     }
   }
 }
@@ -54,7 +54,7 @@ static void go_y(const PassLaunch<T>& G, hipStream_t s) {
 }
 template <class T, int NC>
 static void go_z(const PassLaunch<T>& G, hipStream_t s) {
-  hipLaunchKernelGGL((k_zpass<T, NC>), dim3(G.blocks), dim3(kThreads), G.lds, s, G.probs,
+  hipLaunchKernelGGL((k_zpass<T, NC>), dim3(G.blocks), dim3(kZThreads), G.lds, s, G.probs,
                      G.bstart, G.nprob);
 }
 

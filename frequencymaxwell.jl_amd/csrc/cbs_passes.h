@@ -17,10 +17,10 @@ extern __shared__ __align__(16) unsigned char g_smem[];
 
 // twiddles of one axis into the head of the tile LDS (read after the tile's
 // first barrier); returns the start of the tile data
-template <class T>
+template <class T, int NT = kThreads>
 __device__ __forceinline__ Cx<T>* stage_twiddles(const Cx<T>* __restrict__ tw, int n, int tid) {
   Cx<T>* t = reinterpret_cast<Cx<T>*>(g_smem);
-  for (int i = tid; i < n; i += kThreads) t[i] = tw[i];
+  for (int i = tid; i < n; i += NT) t[i] = tw[i];
   return reinterpret_cast<Cx<T>*>(g_smem + tw_bytes(n, (int)sizeof(Cx<T>)));
 }
 template <class T>
@@ -176,14 +176,14 @@ __device__ __forceinline__ void xfwd_tile(const DevProb<T>& pr, int tile, int ti
 // row stride `rs` (elements), LDS row stride `ls` with the P columns at
 // offset `lo`.  FP32 moves two complex values per lane (16 bytes).
 // ---------------------------------------------------------------------------
-template <class T>
+template <class T, int NT = kThreads>
 __device__ __forceinline__ void tile_load(Cx<T>* __restrict__ lds, const Cx<T>* __restrict__ src,
                                           int P, int npx, int rows, long long rs, int ls, int lo,
                                           int tid) {
   if constexpr (sizeof(Cx<T>) == 8) {
     if ((P & 1) == 0) {
       const int P2 = P >> 1;
-      for (int e = tid; e < P2 * rows; e += kThreads) {
+      for (int e = tid; e < P2 * rows; e += NT) {
         const int p2 = e % P2, m = e / P2;
         const int p = 2 * p2;
         C2f v;
@@ -198,20 +198,20 @@ __device__ __forceinline__ void tile_load(Cx<T>* __restrict__ lds, const Cx<T>* 
       return;
     }
   }
-  for (int e = tid; e < P * rows; e += kThreads) {
+  for (int e = tid; e < P * rows; e += NT) {
     const int p = e % P, m = e / P;
     lds[m * ls + lo + p] = p < npx ? src[m * rs + p] : mk<T>(T(0), T(0));
   }
 }
 
-template <class T>
+template <class T, int NT = kThreads>
 __device__ __forceinline__ void tile_store(const Cx<T>* __restrict__ lds, Cx<T>* __restrict__ dst,
                                            int P, int npx, int rows, long long rs, int ls, int lo,
                                            int tid) {
   if constexpr (sizeof(Cx<T>) == 8) {
     if ((P & 1) == 0) {
       const int P2 = P >> 1;
-      for (int e = tid; e < P2 * rows; e += kThreads) {
+      for (int e = tid; e < P2 * rows; e += NT) {
         const int p2 = e % P2, m = e / P2;
         const int p = 2 * p2;
         const C2f v = *reinterpret_cast<const C2f*>(lds + m * ls + lo + p);
@@ -224,7 +224,7 @@ __device__ __forceinline__ void tile_store(const Cx<T>* __restrict__ lds, Cx<T>*
       return;
     }
   }
-  for (int e = tid; e < P * rows; e += kThreads) {
+  for (int e = tid; e < P * rows; e += NT) {
     const int p = e % P, m = e / P;
     if (p < npx) dst[m * rs + p] = lds[m * ls + lo + p];
   }
@@ -325,14 +325,14 @@ __device__ __forceinline__ void kspace_op(const DevProb<T>& pr, double kxd, doub
   }
 }
 
-template <class T, int NC>
+template <class T, int NC, int NT = kThreads>
 __device__ __forceinline__ void zpass_tile(const DevProb<T>& pr, int tile, int tid) {
   constexpr bool CT = NC > 0;
   constexpr int eb = (int)sizeof(Cx<T>);
   const int n = CT ? NC : pr.n[2];
   const int P = zcols(n, eb, !CT);
   const int Q = 3 * P;
-  Cx<T>* lds = stage_twiddles<T>(pr.tw[2], n, tid);
+  Cx<T>* lds = stage_twiddles<T, NT>(pr.tw[2], n, tid);
   const int nx = pr.n[0], ny = pr.n[1];
   const int nxb = (nx + P - 1) / P;
   const int xb = tile % nxb;
@@ -344,7 +344,7 @@ __device__ __forceinline__ void zpass_tile(const DevProb<T>& pr, int tile, int t
   Cx<T>* base = pr.w + 3 * s * pr.ws[2] + (long long)y * pr.ws[0] + x0;
 #pragma unroll
   for (int c = 0; c < 3; ++c)
-    tile_load(lds, base + c * pr.ws[2], P, npx, n, pr.ws[1], Q, c * P, tid);
+    tile_load<T, NT>(lds, base + c * pr.ws[2], P, npx, n, pr.ws[1], Q, c * P, tid);
   // k values of this tile: kt[m] along z, kt[n + p] along x, kt[n + P] at y
   double* kt = reinterpret_cast<double*>(reinterpret_cast<unsigned char*>(lds) + zk_offset(n, P, eb, !CT));
   {
@@ -352,7 +352,7 @@ __device__ __forceinline__ void zpass_tile(const DevProb<T>& pr, int tile, int t
     const double* tz = green ? pr.kk[2][s] : pr.kc[2];
     const double* tx = green ? pr.kk[0][s] : pr.kc[0];
     const double* ty = green ? pr.kk[1][s] : pr.kc[1];
-    for (int i = tid; i < zk_count(n, P); i += kThreads) {
+    for (int i = tid; i < zk_count(n, P); i += NT) {
       double v = 0.0;
       if (i < n) v = tz[i];
       else if (i < n + P) v = (i - n < npx) ? tx[x0 + i - n] : 0.0;
@@ -364,11 +364,11 @@ __device__ __forceinline__ void zpass_tile(const DevProb<T>& pr, int tile, int t
   Cx<T>* res = lds;
   if constexpr (CT) {
     constexpr int Pc = zcols(CT ? NC : 8, eb, false);
-    fft_ct<T, NC, 3 * Pc, 3 * Pc, -1>(lds, lds_twiddles<T>(), tid);
+    fft_ct<T, NC, 3 * Pc, 3 * Pc, -1, NT>(lds, lds_twiddles<T>(), tid);
   } else {
-    res = fft_rt<T, -1>(lds, lds + n * Q, n, Q, Q, pr.plan[2], lds_twiddles<T>(), tid);
+    res = fft_rt<T, -1, NT>(lds, lds + n * Q, n, Q, Q, pr.plan[2], lds_twiddles<T>(), tid);
   }
-  for (int e = tid; e < P * n; e += kThreads) {
+  for (int e = tid; e < P * n; e += NT) {
     const int p = e % P, m = e / P;
     if (p < npx) {
       Cx<T>* r = res + m * Q + p;
@@ -382,14 +382,14 @@ __device__ __forceinline__ void zpass_tile(const DevProb<T>& pr, int tile, int t
   __syncthreads();
   if constexpr (CT) {
     constexpr int Pc = zcols(CT ? NC : 8, eb, false);
-    fft_ct<T, NC, 3 * Pc, 3 * Pc, 1>(lds, lds_twiddles<T>(), tid);
+    fft_ct<T, NC, 3 * Pc, 3 * Pc, 1, NT>(lds, lds_twiddles<T>(), tid);
   } else {
     Cx<T>* other = res == lds ? lds + n * Q : lds;
-    res = fft_rt<T, 1>(res, other, n, Q, Q, pr.plan[2], lds_twiddles<T>(), tid);
+    res = fft_rt<T, 1, NT>(res, other, n, Q, Q, pr.plan[2], lds_twiddles<T>(), tid);
   }
 #pragma unroll
   for (int c = 0; c < 3; ++c)
-    tile_store(res, base + c * pr.ws[2], P, npx, n, pr.ws[1], Q, c * P, tid);
+    tile_store<T, NT>(res, base + c * pr.ws[2], P, npx, n, pr.ws[1], Q, c * P, tid);
 }
 
 // ---------------------------------------------------------------------------
