@@ -108,6 +108,7 @@ void cbs_default_opts(cbs_solve_opts* o) {
   o->use_graph = 0;
   o->x0_zero = 0;
   o->compute_h = 1;
+  o->fuse_sweeps = 1;
 }
 
 cbs_status cbs_ctx_create(cbs_ctx** out, int32_t device, uint64_t hbm_budget_bytes) {

@@ -197,8 +197,13 @@ void Pipe<T>::build(Ctx* c, hipStream_t s) {
   CBS_REQUIRE(np > 0, "empty pipeline");
   L = PipeLaunch<T>{};
   L.S = h[0].S;
-  for (int i = 0; i < np; ++i)
+  L.fused = h[0].fuse_next;
+  for (int i = 0; i < np; ++i) {
     CBS_REQUIRE(h[i].S == L.S, "pipeline mixes averaging and non-averaging problems");
+    CBS_REQUIRE(h[i].fuse_next == L.fused, "pipeline mixes fused and separate x sweeps");
+    CBS_REQUIRE(!h[i].fuse_next || (h[i].post == POST_BORN && h[i].pre == PRE_V),
+                "fused x sweeps need the Born update");
+  }
   struct Grp {
     int pass, off_p, off_b, n, blocks, len;
     size_t lds;
@@ -239,7 +244,7 @@ void Pipe<T>::build(Ctx* c, hipStream_t s) {
       if (g.n == 0) continue;
       hb.push_back(g.blocks);
       // the xinv epilogue reuses the tile LDS for its block reduction
-      g.lds = std::max(g.lds, (size_t)((kThreads / 64) * kNumPart * sizeof(double)));
+      g.lds = std::max(g.lds, (size_t)((kMaxXThreads / 64) * kNumPart * sizeof(double)));
       if (g.lds > 160 * 1024)
         throw Error(CBS_EINVAL, "grid axis too long for an LDS tile (" + std::to_string(g.lds) +
                                     " bytes)");

@@ -208,6 +208,21 @@ __host__ __device__ constexpr int width_z(int nz, int eb, bool rt) {
 __host__ __device__ constexpr int xlines(int nx, int eb, int S, bool rt) {
   return tile_p(nx, eb, 3 * (rt ? S : 1), width_x(nx, eb, S, rt), rt, 1);
 }
+// Threads per block of the specialised x-pass kernels: P n / E for the
+// smallest E >= CBS_XE that gives whole waves within 1024 threads, so every
+// thread owns exactly E elements of the tile (the loads of a tile are then
+// all issued before its first transform); kThreads where none fits.
+#ifndef CBS_XE
+#define CBS_XE 0
+#endif
+__host__ __device__ constexpr int xthreads(int nx, int eb, int S) {
+  if (CBS_XE == 0 || !is_ct_length(nx)) return kThreads;
+  const int e = xlines(nx, eb, S, false) * nx;
+  for (int E = CBS_XE; E <= 8; ++E)
+    if (e % E == 0 && (e / E) % 64 == 0 && e / E <= 1024) return e / E;
+  return kThreads;
+}
+constexpr int kMaxXThreads = 1024;
 // y pass: P columns of one component
 __host__ __device__ constexpr int ycols(int ny, int eb, bool rt) {
   return tile_p(ny, eb, 1, width_y(ny, eb, rt), rt, 0);
@@ -271,6 +286,8 @@ struct DevProb {
   int S;         // 1 or 2 pipelines (averaging over +/- shift)
   int has_ramp;  // any axis shift non-zero
   int pre, zop, post;
+  int fuse_next; // POST_BORN + PRE_V only: the xinv pass also writes the x-forward
+                 // spectra of V out0 into w (the next Born sweep starts at y)
   int rt[3];     // axis uses the runtime-plan transform
   long long nvox;
   const Cx<T>* tw[3];   // twiddles exp(-2 pi i q / n_d), q < n_d

@@ -186,7 +186,7 @@ void launch_pass(const PipeLaunch<T>& L, int k, hipStream_t s) {
 
 template <class T>
 void launch_pipeline(const PipeLaunch<T>& L, hipStream_t s) {
-  launch_xfwd<T>(L, s);
+  if (!L.fused) launch_xfwd<T>(L, s);
   launch_ypass<T>(L, -1, s);
   launch_zpass<T>(L, s);
   launch_ypass<T>(L, 1, s);

@@ -5,8 +5,14 @@
 
 namespace cbs {
 
+// block size of the x-pass kernels of length NC (0: runtime-plan lengths)
 template <class T, int S, int NC>
-__global__ void __launch_bounds__(kThreads)
+constexpr int x_nt() {
+  return NC > 0 ? xthreads(NC, (int)sizeof(Cx<T>), S) : kThreads;
+}
+
+template <class T, int S, int NC>
+__global__ void __launch_bounds__((x_nt<T, S, NC>()))
 k_xfwd(const DevProb<T>* __restrict__ probs, const int* __restrict__ bstart, int nprob) {
   const int pi = find_prob(bstart, nprob, blockIdx.x);
   const DevProb<T>& pr = probs[pi];
@@ -26,8 +32,9 @@ k_xfwd(const DevProb<T>* __restrict__ probs, const int* __restrict__ bstart, int
   }
 }
 
-template <class T, int S, int NC>
-__global__ void __launch_bounds__(kThreads)
+// F: the fused x pass of a Born sweep (x inverse + update + next x forward)
+template <class T, int S, int NC, bool F>
+__global__ void __launch_bounds__((x_nt<T, S, NC>()))
 k_xinv(const DevProb<T>* __restrict__ probs, const int* __restrict__ bstart, int nprob) {
   const int pi = find_prob(bstart, nprob, blockIdx.x);
   const DevProb<T>& pr = probs[pi];
@@ -35,27 +42,31 @@ k_xinv(const DevProb<T>* __restrict__ probs, const int* __restrict__ bstart, int
   const int tile = blockIdx.x - bstart[pi];
   const int tid = threadIdx.x;
   if constexpr (NC != 0) {
-    xinv_tile<T, NC, S>(pr, tile, tid);
+    xinv_tile<T, NC, S, F>(pr, tile, tid);
   } else {
     switch (pr.rt[0] ? 0 : pr.n[0]) {
 #define CBS_CASE_(L) \
-      case L: xinv_tile<T, L, S>(pr, tile, tid); break;
+      case L: xinv_tile<T, L, S, F>(pr, tile, tid); break;
       CBS_CT_LENGTHS(CBS_CASE_)
 #undef CBS_CASE_
-      default: xinv_tile<T, 0, S>(pr, tile, tid); break;
+      default: xinv_tile<T, 0, S, F>(pr, tile, tid); break;
     }
   }
 }
 
 template <class T, int S, int NC>
 static void go_xfwd(const PassLaunch<T>& G, hipStream_t s) {
-  hipLaunchKernelGGL((k_xfwd<T, S, NC>), dim3(G.blocks), dim3(kThreads), G.lds, s, G.probs,
+  hipLaunchKernelGGL((k_xfwd<T, S, NC>), dim3(G.blocks), dim3(x_nt<T, S, NC>()), G.lds, s, G.probs,
                      G.bstart, G.nprob);
 }
 template <class T, int S, int NC>
-static void go_xinv(const PassLaunch<T>& G, hipStream_t s) {
-  hipLaunchKernelGGL((k_xinv<T, S, NC>), dim3(G.blocks), dim3(kThreads), G.lds, s, G.probs,
-                     G.bstart, G.nprob);
+static void go_xinv(const PassLaunch<T>& G, bool fused, hipStream_t s) {
+  if (fused)
+    hipLaunchKernelGGL((k_xinv<T, S, NC, true>), dim3(G.blocks), dim3(x_nt<T, S, NC>()), G.lds, s,
+                       G.probs, G.bstart, G.nprob);
+  else
+    hipLaunchKernelGGL((k_xinv<T, S, NC, false>), dim3(G.blocks), dim3(x_nt<T, S, NC>()), G.lds, s,
+                       G.probs, G.bstart, G.nprob);
 }
 
 template <class T, int S>
@@ -69,13 +80,13 @@ static void xfwd_len(const PassLaunch<T>& G, hipStream_t s) {
   }
 }
 template <class T, int S>
-static void xinv_len(const PassLaunch<T>& G, hipStream_t s) {
+static void xinv_len(const PassLaunch<T>& G, bool fused, hipStream_t s) {
   switch (G.len) {
 #define CBS_CASE_(L) \
-    case L: go_xinv<T, S, L>(G, s); break;
+    case L: go_xinv<T, S, L>(G, fused, s); break;
       CBS_CT_LENGTHS(CBS_CASE_)
 #undef CBS_CASE_
-    default: go_xinv<T, S, 0>(G, s); break;
+    default: go_xinv<T, S, 0>(G, fused, s); break;
   }
 }
 
@@ -89,8 +100,8 @@ void launch_xfwd(const PipeLaunch<T>& L, hipStream_t s) {
 template <class T>
 void launch_xinv(const PipeLaunch<T>& L, hipStream_t s) {
   for (int i = 0; i < L.ng[3]; ++i) {
-    if (L.S == 2) xinv_len<T, 2>(L.g[3][i], s);
-    else xinv_len<T, 1>(L.g[3][i], s);
+    if (L.S == 2) xinv_len<T, 2>(L.g[3][i], L.fused != 0, s);
+    else xinv_len<T, 1>(L.g[3][i], L.fused != 0, s);
   }
 }
 

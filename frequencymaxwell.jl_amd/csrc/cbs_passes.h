@@ -38,9 +38,24 @@ __device__ __forceinline__ int find_prob(const int* __restrict__ bstart, int npr
   return lo;
 }
 
+// exp(-2 pi i sum_d idx_d s_d / n_d) as the product of the three axis tables,
+// rounded the same way in every kernel (no contraction into FMAs), so the
+// x forward of a fused x pass and of sweep A apply the same ramp
 template <class T>
 __device__ __forceinline__ Cx<T> ramp_at(const DevProb<T>& pr, int x, int y, int z) {
-  return pr.ramp[0][x] * pr.ramp[1][y] * pr.ramp[2][z];
+#pragma clang fp contract(off)
+  const Cx<T> a = pr.ramp[0][x], b = pr.ramp[1][y], c = pr.ramp[2][z];
+  const Cx<T> ab = mk<T>(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
+  return mk<T>(ab.re * c.re - ab.im * c.im, ab.re * c.im + ab.im * c.re);
+}
+
+// complex a b without FMA contraction: every kernel rounds it the same way,
+// whatever else the kernel does with a and b (the per-voxel arithmetic of the
+// x passes uses it, so sweep A and the fused x pass agree to the bit)
+template <class T>
+__device__ __forceinline__ Cx<T> cmul_nc(const Cx<T>& a, const Cx<T>& b) {
+#pragma clang fp contract(off)
+  return mk<T>(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
 }
 
 template <class T>
@@ -59,13 +74,91 @@ struct alignas(16) C2f {
 // 3P-pencil LDS tile (u and the ramp stay in registers); the runtime-plan path
 // keeps all 3 S P pencils in LDS.
 // ---------------------------------------------------------------------------
+// Forward x transform of the S ramped pipelines of one specialised tile and
+// the store of their spectra into pr.w.  u holds the pre-op values (V x or x)
+// of the E elements of this thread (zero outside the tile), rp their ramps.
+// Shared by sweep A and the fused tail of sweep E, so both give the same bits.
+template <class T, int NC, int S, int NT, int E>
+__device__ __forceinline__ void xfwd_ct_store(const DevProb<T>& pr, Cx<T>* __restrict__ lds,
+                                              const Cx<T> (&u)[E][3], const Cx<T> (&rp)[E],
+                                              int l0, int np, int tid) {
+  constexpr int eb = (int)sizeof(Cx<T>);
+  constexpr int Pc = xlines(NC, eb, S, false);
+  constexpr int Qc = 3 * Pc;
+  constexpr int QSc = xpad_stride(Qc);
+  const int ny = pr.n[1];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      const int e = tid + k * NT;
+      if (e < Pc * NC) {
+        const int m = e % NC, p = e / NC;
+        const Cx<T> r = s == 0 ? rp[k] : conj(rp[k]);
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          lds[m * QSc + c * Pc + p] = pr.has_ramp ? cmul_nc(r, u[k][c]) : u[k][c];
+      }
+    }
+    __syncthreads();
+    fft_ct<T, NC, Qc, QSc, -1, NT>(lds, lds_twiddles<T>(), tid);
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      const int e = tid + k * NT;
+      const int m = e % NC, p = e / NC;
+      if (e < Pc * NC && p < np) {
+        const int l = l0 + p;
+        const long long wo = (long long)(l / ny) * pr.ws[1] + (long long)(l % ny) * pr.ws[0] + m;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) pr.w[(s * 3 + c) * pr.ws[2] + wo] = lds[m * QSc + c * Pc + p];
+      }
+    }
+    if (S == 2 && s == 0) __syncthreads();  // the next pipeline reuses the tile
+  }
+}
+
+// Runtime-plan sweep A: the S ramped copies of the pre-op values u of
+// element (m, p) into the tile buf (row stride QS, P lines per pencil group).
+template <class T, int S>
+__device__ __forceinline__ void xfwd_rt_put(const DevProb<T>& pr, Cx<T>* __restrict__ buf, int QS,
+                                            int P, int m, int p, const Cx<T> (&u)[3],
+                                            const Cx<T>& rp) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    buf[m * QS + c * P + p] = pr.has_ramp ? cmul_nc(rp, u[c]) : u[c];
+    if (S == 2) buf[m * QS + (3 + c) * P + p] = pr.has_ramp ? cmul_nc(conj(rp), u[c]) : u[c];
+  }
+}
+
+// Runtime-plan sweep A: transform the filled tile a (b: the second buffer)
+// and store the spectra of its np lines.
+template <class T, int S>
+__device__ __forceinline__ void xfwd_rt_store(const DevProb<T>& pr, Cx<T>* a, Cx<T>* b, int n, int P,
+                                              int l0, int np, int tid) {
+  const int Q = 3 * S * P;
+  const int QS = xpad_stride(Q);
+  const int ny = pr.n[1];
+  __syncthreads();
+  Cx<T>* res = fft_rt<T, -1>(a, b, n, Q, QS, pr.plan[0], lds_twiddles<T>(), tid);
+  for (int e = tid; e < P * n; e += kThreads) {
+    const int m = e % n, p = e / n;
+    if (p < np) {
+      const int l = l0 + p;
+      const long long wo = (long long)(l / ny) * pr.ws[1] + (long long)(l % ny) * pr.ws[0] + m;
+#pragma unroll
+      for (int q = 0; q < 3 * S; ++q) pr.w[q * pr.ws[2] + wo] = res[m * QS + q * P + p];
+    }
+  }
+}
+
 template <class T, int NC, int S>
 __device__ __forceinline__ void xfwd_tile(const DevProb<T>& pr, int tile, int tid) {
   constexpr bool CT = NC > 0;
   constexpr int eb = (int)sizeof(Cx<T>);
+  constexpr int NT = CT ? xthreads(NC, eb, S) : kThreads;
   const int n = CT ? NC : pr.n[0];
   const int P = xlines(n, eb, S, !CT);
-  Cx<T>* lds = stage_twiddles<T>(pr.tw[0], n, tid);
+  Cx<T>* lds = stage_twiddles<T, NT>(pr.tw[0], n, tid);
   const int ny = pr.n[1];
   const int nlines = ny * pr.n[2];
   const int l0 = tile * P;
@@ -74,14 +167,12 @@ __device__ __forceinline__ void xfwd_tile(const DevProb<T>& pr, int tile, int ti
   const bool useV = pr.pre == PRE_V;
   if constexpr (CT) {
     constexpr int Pc = xlines(CT ? NC : 8, eb, S, false);
-    constexpr int Qc = 3 * Pc;
-    constexpr int QSc = xpad_stride(Qc);
-    constexpr int E = (Pc * NC + kThreads - 1) / kThreads;
+    constexpr int E = (Pc * NC + NT - 1) / NT;
     Cx<T> u[E][3];
     Cx<T> rp[E];
 #pragma unroll
     for (int k = 0; k < E; ++k) {
-      const int e = tid + k * kThreads;
+      const int e = tid + k * NT;
       const int m = e % NC, p = e / NC;
       rp[k] = mk<T>(T(1), T(0));
       if (e < Pc * NC && p < np) {
@@ -90,7 +181,7 @@ __device__ __forceinline__ void xfwd_tile(const DevProb<T>& pr, int tile, int ti
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
           const Cx<T> xv = pr.in0[c * nv + g];
-          u[k][c] = useV ? vv * xv : xv;
+          u[k][c] = useV ? cmul_nc(vv, xv) : xv;
         }
         if (pr.has_ramp) {
           const int l = l0 + p;
@@ -101,36 +192,9 @@ __device__ __forceinline__ void xfwd_tile(const DevProb<T>& pr, int tile, int ti
         for (int c = 0; c < 3; ++c) u[k][c] = mk<T>(T(0), T(0));
       }
     }
-#pragma unroll
-    for (int s = 0; s < S; ++s) {
-#pragma unroll
-      for (int k = 0; k < E; ++k) {
-        const int e = tid + k * kThreads;
-        if (e < Pc * NC) {
-          const int m = e % NC, p = e / NC;
-          const Cx<T> r = s == 0 ? rp[k] : conj(rp[k]);
-#pragma unroll
-          for (int c = 0; c < 3; ++c) lds[m * QSc + c * Pc + p] = pr.has_ramp ? r * u[k][c] : u[k][c];
-        }
-      }
-      __syncthreads();
-      fft_ct<T, NC, Qc, QSc, -1>(lds, lds_twiddles<T>(), tid);
-#pragma unroll
-      for (int k = 0; k < E; ++k) {
-        const int e = tid + k * kThreads;
-        const int m = e % NC, p = e / NC;
-        if (e < Pc * NC && p < np) {
-          const int l = l0 + p;
-          const long long wo = (long long)(l / ny) * pr.ws[1] + (long long)(l % ny) * pr.ws[0] + m;
-#pragma unroll
-          for (int c = 0; c < 3; ++c) pr.w[(s * 3 + c) * pr.ws[2] + wo] = lds[m * QSc + c * Pc + p];
-        }
-      }
-      if (S == 2 && s == 0) __syncthreads();  // the next pipeline reuses the tile
-    }
+    xfwd_ct_store<T, NC, S, NT, E>(pr, lds, u, rp, l0, np, tid);
   } else {
-    const int Q = 3 * S * P;
-    const int QS = xpad_stride(Q);
+    const int QS = xpad_stride(3 * S * P);
     for (int e = tid; e < P * n; e += kThreads) {
       const int m = e % n, p = e / n;
       Cx<T> u[3];
@@ -141,7 +205,7 @@ __device__ __forceinline__ void xfwd_tile(const DevProb<T>& pr, int tile, int ti
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
           const Cx<T> xv = pr.in0[c * nv + g];
-          u[c] = useV ? vv * xv : xv;
+          u[c] = useV ? cmul_nc(vv, xv) : xv;
         }
         if (pr.has_ramp) {
           const int l = l0 + p;
@@ -151,23 +215,9 @@ __device__ __forceinline__ void xfwd_tile(const DevProb<T>& pr, int tile, int ti
 #pragma unroll
         for (int c = 0; c < 3; ++c) u[c] = mk<T>(T(0), T(0));
       }
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        lds[m * QS + c * P + p] = pr.has_ramp ? rp * u[c] : u[c];
-        if (S == 2) lds[m * QS + (3 + c) * P + p] = pr.has_ramp ? conj(rp) * u[c] : u[c];
-      }
+      xfwd_rt_put<T, S>(pr, lds, QS, P, m, p, u, rp);
     }
-    __syncthreads();
-    Cx<T>* res = fft_rt<T, -1>(lds, lds + n * QS, n, Q, QS, pr.plan[0], lds_twiddles<T>(), tid);
-    for (int e = tid; e < P * n; e += kThreads) {
-      const int m = e % n, p = e / n;
-      if (p < np) {
-        const int l = l0 + p;
-        const long long wo = (long long)(l / ny) * pr.ws[1] + (long long)(l % ny) * pr.ws[0] + m;
-#pragma unroll
-        for (int q = 0; q < 3 * S; ++q) pr.w[q * pr.ws[2] + wo] = res[m * QS + q * P + p];
-      }
-    }
+    xfwd_rt_store<T, S>(pr, lds, lds + n * QS, n, P, l0, np, tid);
   }
 }
 
@@ -397,9 +447,37 @@ __device__ __forceinline__ void zpass_tile(const DevProb<T>& pr, int tile, int t
 // Specialised lengths transform the S pipelines one after the other through
 // one 3P-pencil LDS tile and average them in registers.
 // ---------------------------------------------------------------------------
+// The epilogue operands of one voxel g: vv = V (gamma post-ops), a0 = in0,
+// a1 = in1 (or out0 for the beta term of POST_OPERATOR); loaded before the
+// transforms so their latency hides behind them.
 template <class T>
-__device__ __forceinline__ void xinv_epilogue(const DevProb<T>& pr, long long g, const Cx<T> (&w)[3],
-                                              double (&acc)[kNumPart]) {
+__device__ __forceinline__ void xinv_operands(const DevProb<T>& pr, long long g, bool valid,
+                                              Cx<T>& vv, Cx<T> (&a0)[3], Cx<T> (&a1)[3]) {
+  const long long nv = pr.nvox;
+  const int post = pr.post;
+  const bool use_b = post == POST_OPERATOR && (pr.b_re != 0.0 || pr.b_im != 0.0);
+  const bool need0 = valid && post != POST_STORE;
+  const bool need1 = valid && (post == POST_BORN || post == POST_RESID || use_b ||
+                               (post == POST_PMATVEC && pr.in1 != nullptr));
+  const Cx<T>* src1 = post == POST_OPERATOR ? pr.out0 : pr.in1;
+  vv = valid && post >= POST_BORN ? pr.V[g] : mk<T>(T(0), T(0));
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    a0[c] = need0 ? pr.in0[c * nv + g] : mk<T>(T(0), T(0));
+    a1[c] = need1 ? src1[c * nv + g] : mk<T>(T(0), T(0));
+  }
+}
+
+// The fused epilogue of voxel g from w = G u and the operands above;
+// acc collects the FP64 partial sums of the post-op.  For POST_BORN w
+// returns the new iterate written to out0.
+template <class T>
+__device__ __forceinline__ void xinv_finish(const DevProb<T>& pr, long long g, Cx<T> (&w)[3],
+                                            const Cx<T>& vv, const Cx<T> (&a0)[3],
+                                            const Cx<T> (&a1)[3], double (&acc)[kNumPart]) {
+  // written out without FMA contraction, so the plain and the fused x pass
+  // round the update identically
+#pragma clang fp contract(off)
   const long long nv = pr.nvox;
   const int post = pr.post;
   if (post == POST_STORE) {
@@ -410,55 +488,61 @@ __device__ __forceinline__ void xinv_epilogue(const DevProb<T>& pr, long long g,
     const bool use_b = pr.b_re != 0.0 || pr.b_im != 0.0;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      Cx<T> y = a * (pr.in0[c * nv + g] - w[c]);
-      if (use_b) y = y + b * pr.out0[c * nv + g];
+      Cx<T> y = cmul_nc(a, mk<T>(a0[c].re - w[c].re, a0[c].im - w[c].im));
+      if (use_b) {
+        const Cx<T> by = cmul_nc(b, a1[c]);
+        y = mk<T>(y.re + by.re, y.im + by.im);
+      }
       pr.out0[c * nv + g] = y;
     }
   } else {
     // gamma = i V / eps
-    const Cx<T> gam = mul_i((T)pr.eps_inv * pr.V[g]);
+    const T ei = (T)pr.eps_inv;
+    const Cx<T> gam = mk<T>(-(ei * vv.im), ei * vv.re);
     if (post == POST_BORN) {
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        const Cx<T> xv = pr.in0[c * nv + g];
-        const Cx<T> r = gam * (pr.in1[c * nv + g] - xv + w[c]);
-        pr.out0[c * nv + g] = xv + r;
-        acc[0] += (double)abs2(r);
+        const Cx<T> t = mk<T>(a1[c].re - a0[c].re + w[c].re, a1[c].im - a0[c].im + w[c].im);
+        const Cx<T> r = cmul_nc(gam, t);
+        w[c] = mk<T>(a0[c].re + r.re, a0[c].im + r.im);
+        pr.out0[c * nv + g] = w[c];
+        acc[0] += (double)(r.re * r.re + r.im * r.im);
       }
     } else if (post == POST_RESID) {
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        const Cx<T> r = pr.in1[c * nv + g] - pr.in0[c * nv + g] + w[c];
+        const Cx<T> r = mk<T>(a1[c].re - a0[c].re + w[c].re, a1[c].im - a0[c].im + w[c].im);
         pr.out0[c * nv + g] = r;
-        acc[0] += (double)abs2(gam * r);
+        const Cx<T> gr = cmul_nc(gam, r);
+        acc[0] += (double)(gr.re * gr.re + gr.im * gr.im);
       }
     } else {  // POST_PMATVEC
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        const Cx<T> xv = pr.in0[c * nv + g];
-        const Cx<T> o = gam * (xv - w[c]);
+        const Cx<T> o = cmul_nc(gam, mk<T>(a0[c].re - w[c].re, a0[c].im - w[c].im));
         pr.out0[c * nv + g] = o;
         if (pr.in1) {
-          const Cx<T> d = cdot(pr.in1[c * nv + g], o);
-          acc[0] += (double)d.re;
-          acc[1] += (double)d.im;
+          // <in1, o> = conj(in1) o
+          acc[0] += (double)(a1[c].re * o.re + a1[c].im * o.im);
+          acc[1] += (double)(a1[c].re * o.im - a1[c].im * o.re);
         }
-        const Cx<T> d2 = cdot(o, xv);
-        acc[2] += (double)d2.re;
-        acc[3] += (double)d2.im;
-        acc[4] += (double)abs2(o);
+        // <o, in0>
+        acc[2] += (double)(o.re * a0[c].re + o.im * a0[c].im);
+        acc[3] += (double)(o.re * a0[c].im - o.im * a0[c].re);
+        acc[4] += (double)(o.re * o.re + o.im * o.im);
       }
     }
   }
 }
 
-template <class T, int NC, int S>
+template <class T, int NC, int S, bool FUSE>
 __device__ __forceinline__ void xinv_tile(const DevProb<T>& pr, int tile, int tid) {
   constexpr bool CT = NC > 0;
   constexpr int eb = (int)sizeof(Cx<T>);
+  constexpr int NT = CT ? xthreads(NC, eb, S) : kThreads;
   const int n = CT ? NC : pr.n[0];
   const int P = xlines(n, eb, S, !CT);
-  Cx<T>* lds = stage_twiddles<T>(pr.tw[0], n, tid);
+  Cx<T>* lds = stage_twiddles<T, NT>(pr.tw[0], n, tid);
   const int ny = pr.n[1];
   const int nlines = ny * pr.n[2];
   const int l0 = tile * P;
@@ -468,39 +552,44 @@ __device__ __forceinline__ void xinv_tile(const DevProb<T>& pr, int tile, int ti
     constexpr int Pc = xlines(CT ? NC : 8, eb, S, false);
     constexpr int Qc = 3 * Pc;
     constexpr int QSc = xpad_stride(Qc);
-    constexpr int E = (Pc * NC + kThreads - 1) / kThreads;
-    Cx<T> w[E][3];
+    constexpr int E = (Pc * NC + NT - 1) / NT;
+    // every global load of the tile first: both spectra, ramps, operands
+    Cx<T> w[S][E][3];
     Cx<T> rp[E];
+    Cx<T> vv[E], a0[E][3], a1[E][3];
 #pragma unroll
     for (int k = 0; k < E; ++k) {
-      const int e = tid + k * kThreads;
+      const int e = tid + k * NT;
       const int m = e % NC, p = e / NC;
-      rp[k] = mk<T>(T(1), T(0));
-      if (pr.has_ramp && e < Pc * NC && p < np) {
-        const int l = l0 + p;
-        rp[k] = ramp_at(pr, m, l % ny, l / ny);
+      const bool valid = e < Pc * NC && p < np;
+      const int l = l0 + p;
+      const long long wo = (long long)(l / ny) * pr.ws[1] + (long long)(l % ny) * pr.ws[0] + m;
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          w[s][k][c] = valid ? pr.w[(s * 3 + c) * pr.ws[2] + wo] : mk<T>(T(0), T(0));
       }
+      rp[k] = pr.has_ramp && valid ? ramp_at(pr, m, l % ny, l / ny) : mk<T>(T(1), T(0));
+      xinv_operands(pr, (long long)l * NC + m, valid, vv[k], a0[k], a1[k]);
     }
 #pragma unroll
     for (int s = 0; s < S; ++s) {
+      if (S == 2 && s == 1) __syncthreads();  // the second pipeline reuses the tile
 #pragma unroll
       for (int k = 0; k < E; ++k) {
-        const int e = tid + k * kThreads;
+        const int e = tid + k * NT;
         if (e < Pc * NC) {
           const int m = e % NC, p = e / NC;
-          const int l = l0 + p;
-          const long long wo = (long long)(l / ny) * pr.ws[1] + (long long)(l % ny) * pr.ws[0] + m;
 #pragma unroll
-          for (int c = 0; c < 3; ++c)
-            lds[m * QSc + c * Pc + p] =
-                p < np ? pr.w[(s * 3 + c) * pr.ws[2] + wo] : mk<T>(T(0), T(0));
+          for (int c = 0; c < 3; ++c) lds[m * QSc + c * Pc + p] = w[s][k][c];
         }
       }
       __syncthreads();
-      fft_ct<T, NC, Qc, QSc, 1>(lds, lds_twiddles<T>(), tid);
+      fft_ct<T, NC, Qc, QSc, 1, NT>(lds, lds_twiddles<T>(), tid);
 #pragma unroll
       for (int k = 0; k < E; ++k) {
-        const int e = tid + k * kThreads;
+        const int e = tid + k * NT;
         if (e < Pc * NC) {
           const int m = e % NC, p = e / NC;
           // + pipeline: exp(+2 pi i phi) = conj(ramp);  - pipeline: exp(-2 pi i phi) = ramp
@@ -508,24 +597,36 @@ __device__ __forceinline__ void xinv_tile(const DevProb<T>& pr, int tile, int ti
 #pragma unroll
           for (int c = 0; c < 3; ++c) {
             const Cx<T> v = lds[m * QSc + c * Pc + p];
-            const Cx<T> t = pr.has_ramp ? r * v : v;
-            w[k][c] = s == 0 ? t : w[k][c] + t;
+            const Cx<T> t = pr.has_ramp ? cmul_nc(r, v) : v;
+            w[0][k][c] = s == 0 ? t : w[0][k][c] + t;
           }
         }
       }
-      if (S == 2 && s == 0) __syncthreads();  // the next pipeline reuses the tile
     }
 #pragma unroll
     for (int k = 0; k < E; ++k) {
-      const int e = tid + k * kThreads;
+      const int e = tid + k * NT;
       const int m = e % NC, p = e / NC;
       if (e < Pc * NC && p < np) {
         if (S == 2) {
 #pragma unroll
-          for (int c = 0; c < 3; ++c) w[k][c] = T(0.5) * w[k][c];
+          for (int c = 0; c < 3; ++c) w[0][k][c] = T(0.5) * w[0][k][c];
         }
-        xinv_epilogue(pr, (long long)(l0 + p) * NC + m, w[k], acc);
+        xinv_finish(pr, (long long)(l0 + p) * NC + m, w[0][k], vv[k], a0[k], a1[k], acc);
       }
+    }
+    if constexpr (FUSE) {
+      // sweep A of the next Born sweep on the new iterate, still in registers
+      Cx<T> u[E][3];
+#pragma unroll
+      for (int k = 0; k < E; ++k) {
+        const int e = tid + k * NT;
+        const bool valid = e < Pc * NC && (e / NC) < np;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) u[k][c] = valid ? cmul_nc(vv[k], w[0][k][c]) : mk<T>(T(0), T(0));
+      }
+      __syncthreads();  // other waves may still read the inverse tile
+      xfwd_ct_store<T, NC, S, NT, E>(pr, lds, u, rp, l0, np, tid);
     }
   } else {
     const int Q = 3 * S * P;
@@ -540,27 +641,46 @@ __device__ __forceinline__ void xinv_tile(const DevProb<T>& pr, int tile, int ti
     }
     __syncthreads();
     Cx<T>* res = fft_rt<T, 1>(lds, lds + n * QS, n, Q, QS, pr.plan[0], lds_twiddles<T>(), tid);
+    // the fused sweep A of the next Born sweep fills the other buffer
+    Cx<T>* nxt = res == lds ? lds + n * QS : lds;
+    constexpr bool fuse = FUSE;
     for (int e = tid; e < P * n; e += kThreads) {
       const int m = e % n, p = e / n;
-      if (p >= np) continue;
+      if (p >= np) {
+        if constexpr (fuse) {
+          const Cx<T> z3[3] = {mk<T>(T(0), T(0)), mk<T>(T(0), T(0)), mk<T>(T(0), T(0))};
+          xfwd_rt_put<T, S>(pr, nxt, QS, P, m, p, z3, mk<T>(T(1), T(0)));
+        }
+        continue;
+      }
       Cx<T> w[3];
       const int l = l0 + p;
       const Cx<T> rp = pr.has_ramp ? ramp_at(pr, m, l % ny, l / ny) : mk<T>(T(1), T(0));
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        Cx<T> v = pr.has_ramp ? conj(rp) * res[m * QS + c * P + p] : res[m * QS + c * P + p];
+        Cx<T> v = pr.has_ramp ? cmul_nc(conj(rp), res[m * QS + c * P + p]) : res[m * QS + c * P + p];
         if (S == 2) {
           const Cx<T> v2 = res[m * QS + (3 + c) * P + p];
-          v = T(0.5) * (v + (pr.has_ramp ? rp * v2 : v2));
+          v = T(0.5) * (v + (pr.has_ramp ? cmul_nc(rp, v2) : v2));
         }
         w[c] = v;
       }
-      xinv_epilogue(pr, (long long)(l0 + p) * n + m, w, acc);
+      const long long g = (long long)l * n + m;
+      Cx<T> vv, a0[3], a1[3];
+      xinv_operands(pr, g, true, vv, a0, a1);
+      xinv_finish(pr, g, w, vv, a0, a1, acc);
+      if constexpr (fuse) {
+        Cx<T> u[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) u[c] = cmul_nc(vv, w[c]);
+        xfwd_rt_put<T, S>(pr, nxt, QS, P, m, p, u, rp);
+      }
     }
+    if constexpr (fuse) xfwd_rt_store<T, S>(pr, nxt, res, n, P, l0, np, tid);
   }
   if (pr.post >= POST_BORN) {
     double* scratch = reinterpret_cast<double*>(g_smem);
-    block_reduce_store(acc, pr.part + (long long)tile * kNumPart, scratch, tid);
+    block_reduce_store<NT>(acc, pr.part + (long long)tile * kNumPart, scratch, tid);
   }
 }
 }  // namespace cbs

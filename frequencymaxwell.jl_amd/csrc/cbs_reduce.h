@@ -7,8 +7,9 @@
 
 namespace cbs {
 
-// Reduce kNumPart doubles over the block; thread 0 writes them to dst.
-// `scratch` holds (kThreads/64)*kNumPart doubles of LDS.
+// Reduce kNumPart doubles over a block of NT threads; thread 0 writes them to
+// dst.  `scratch` holds (NT/64)*kNumPart doubles of LDS.
+template <int NT = kThreads>
 __device__ __forceinline__ void block_reduce_store(double (&acc)[kNumPart], double* dst,
                                                    double* scratch, int tid) {
 #pragma unroll
@@ -29,7 +30,7 @@ __device__ __forceinline__ void block_reduce_store(double (&acc)[kNumPart], doub
 #pragma unroll
     for (int k = 0; k < kNumPart; ++k) {
       double s = 0.0;
-      for (int w = 0; w < kThreads / 64; ++w) s += scratch[w * kNumPart + k];
+      for (int w = 0; w < NT / 64; ++w) s += scratch[w * kNumPart + k];
       dst[k] = s;
     }
   }

@@ -225,7 +225,10 @@ void Server::end_job(const std::shared_ptr<Job>& j, cbs_status st, const std::st
     }
 }
 
-void Server::join_set(Job& j, bool fp32, const IterOpts& io) {
+void Server::join_set(Job& j, bool fp32, const IterOpts& io_in) {
+  // served Born requests always run the fused sweeps (one set per precision)
+  IterOpts io = io_in;
+  io.fuse = 1;
   const int idx = set_index(fp32, j.o.method, j.p->S);
   const std::vector<long long> key{(long long)j.ticket};
   const int s = j.p->S - 1;

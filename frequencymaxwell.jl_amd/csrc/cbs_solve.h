@@ -12,6 +12,9 @@ struct SolveItem {
   const Cx<T>* b = nullptr;  // right-hand side (G_avg S, or the refinement residual)
 };
 
+// kernels timed by time_sweeps: the five sweeps and the fused x pass
+constexpr int kSweepKernels = 6;
+
 struct IterOpts {
   double tol = 1e-6, abstol = 0.0;
   int rule = CBS_STOP_RHS;
@@ -20,6 +23,7 @@ struct IterOpts {
   int use_graph = 0;
   int x0_zero = 0;
   int restart = 0;
+  int fuse = 1;  // Born: fused x sweeps (xinv + next xfwd in one pass)
 };
 
 template <class T>
@@ -133,6 +137,12 @@ template <class T>
 struct BornRunner : RunnerBase<T> {
   Pipe<T> pipe[2];
   StateSet fin;
+  // fused sweeps: each sweep's x inverse also runs the next sweep's x
+  // forward, so a sweep is four launches; a member's first x forward runs
+  // once when it is added (prime), and every later sweep of that member is
+  // the same sequence of operations whatever the chunking or the mix
+  bool fuse = true;
+  Pipe<T> prime;
   void add(const std::vector<long long>& keys, std::vector<SolveItem<T>>& xs,
            const IterOpts& io);
   // k iterations of every member (k even; 0: the runner's own k), no wait

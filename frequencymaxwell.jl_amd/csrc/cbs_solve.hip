@@ -204,6 +204,20 @@ void BornRunner<T>::add(const std::vector<long long>& keys, std::vector<SolveIte
       copy_dev<T>(X0, x.b, 3 * x.p->nvox, this->s);
   }
   init_states<T>(this->ctx, xs, io.tol, io.abstol, io.rule, io.maxit, this->s, gn);
+  if (this->items.empty()) fuse = io.fuse != 0;
+  CBS_REQUIRE(fuse == (io.fuse != 0), "a Born set mixes fused and separate x sweeps");
+  if (fuse) {
+    // the first x forward of the new members: spectra of V x0 into W
+    prime.h.clear();
+    for (auto& x : xs)
+      prime.h.push_back(make_desc<T>(*x.p, PRE_V, ZOP_GREEN, POST_BORN,
+                                     x.w->X[0].template as<Cx<T>>(), x.b,
+                                     x.w->X[1].template as<Cx<T>>(), x.w->W.template as<Cx<T>>(),
+                                     x.w->part.template as<double>(), nullptr));
+    prime.build(this->ctx, this->s);
+    launch_xfwd<T>(prime.L, this->s);
+    CBS_HIP(hipGetLastError());
+  }
   for (size_t i = 0; i < xs.size(); ++i)
     this->items.push_back({keys[i], xs[i], 0, (long long)io.maxit + 2 * this->k + 2});
   this->dirty = true;
@@ -228,6 +242,7 @@ void BornRunner<T>::rebuild() {
                                          x.w->W.template as<Cx<T>>(),
                                          x.w->part.template as<double>(),
                                          x.w->st.template as<RState>()));
+      pipe[par].h.back().fuse_next = fuse ? 1 : 0;
     }
     pipe[par].build(this->ctx, this->s);
   }
@@ -426,6 +441,7 @@ IterOpts iter_opts(const cbs_solve_opts& o, int maxit) {
   io.use_graph = o.use_graph;
   io.x0_zero = o.x0_zero;
   io.restart = o.gmres_restart;
+  io.fuse = o.fuse_sweeps != 0;
   return io;
 }
 
@@ -683,16 +699,25 @@ void time_sweeps(Problem& p, int reps, double* ms, double* bytes) {
                                 w.W.template as<Cx<T>>(), w.part.template as<double>(),
                                 nullptr));
   pipe.build(p.ctx, s);
+  // the fused x pass: x inverse + Born update + the next sweep's x forward
+  Pipe<T> fpipe;
+  fpipe.h = pipe.h;
+  fpipe.h[0].fuse_next = 1;
+  fpipe.build(p.ctx, s);
+  auto run = [&](int k) {
+    if (k < 5) launch_pass<T>(pipe.L, k, s);
+    else launch_xinv<T>(fpipe.L, s);
+  };
   // one warm-up sweep, then each kernel `reps` times back to back between two
   // events (the kernels are timed one at a time; their data are left as
   // they end up, which changes nothing in the timing of these streaming passes)
   hipEvent_t ev[2];
   for (auto& e : ev) CBS_HIP(hipEventCreate(&e));
-  for (int k = 0; k < 5; ++k) launch_pass<T>(pipe.L, k, s);
-  double acc[5] = {0, 0, 0, 0, 0};
-  for (int k = 0; k < 5; ++k) {
+  for (int k = 0; k < kSweepKernels; ++k) run(k);
+  double acc[kSweepKernels] = {0, 0, 0, 0, 0, 0};
+  for (int k = 0; k < kSweepKernels; ++k) {
     CBS_HIP(hipEventRecord(ev[0], s));
-    for (int r = 0; r < reps; ++r) launch_pass<T>(pipe.L, k, s);
+    for (int r = 0; r < reps; ++r) run(k);
     CBS_HIP(hipEventRecord(ev[1], s));
     CBS_HIP(hipEventSynchronize(ev[1]));
     CBS_HIP(hipGetLastError());
@@ -702,8 +727,8 @@ void time_sweeps(Problem& p, int reps, double* ms, double* bytes) {
   }
   for (auto& e : ev) (void)hipEventDestroy(e);
   const double sz = (double)sizeof(Cx<T>), S = (double)p.S, nv = (double)p.nvox;
-  const double per[5] = {4 + 3 * S, 6 * S, 6 * S, 6 * S, 3 * S + 10};
-  for (int k = 0; k < 5; ++k) {
+  const double per[kSweepKernels] = {4 + 3 * S, 6 * S, 6 * S, 6 * S, 3 * S + 10, 6 * S + 10};
+  for (int k = 0; k < kSweepKernels; ++k) {
     ms[k] = acc[k] / reps;
     bytes[k] = per[k] * sz * nv;
   }

@@ -69,6 +69,7 @@ class SolveOpts(C.Structure):
         ("compute_h", C.c_int32),
         ("h_scale_re", C.c_double),
         ("h_scale_im", C.c_double),
+        ("fuse_sweeps", C.c_int32),
     ]
 
 

@@ -38,7 +38,8 @@ def method_code(m) -> int:
 
 def make_opts(*, method="born", precision="fp64", tol=1e-6, abstol=0.0, stop_rule="rhs",
               maxiters=1000, gmres_restart=0, max_refine=20, inner_tol=0.0, check_every=1,
-              use_graph=False, x0_zero=False, h_scale=0j, compute_h=True) -> L.SolveOpts:
+              use_graph=False, x0_zero=False, h_scale=0j, compute_h=True,
+              fuse_sweeps=True) -> L.SolveOpts:
     o = L.default_opts()
     o.method = method_code(method)
     o.precision = precision_code(precision)
@@ -55,6 +56,7 @@ def make_opts(*, method="born", precision="fp64", tol=1e-6, abstol=0.0, stop_rul
     o.compute_h = int(bool(compute_h))
     o.h_scale_re = float(complex(h_scale).real)
     o.h_scale_im = float(complex(h_scale).imag)
+    o.fuse_sweeps = int(bool(fuse_sweeps))
     return o
 
 
@@ -182,10 +184,12 @@ class CBSProblem:
         return E, H
 
     def time_sweeps(self, precision="fp32", reps: int = 20):
-        """Average duration (ms) and algorithmic bytes of each of the five sweep
-        kernels, timed with HIP events on the library stream."""
-        ms = (C.c_double * 5)()
-        by = (C.c_double * 5)()
+        """Average duration (ms) and algorithmic bytes of each sweep kernel,
+        timed with HIP events on the library stream: x forward, y forward,
+        z + Green, y inverse, x inverse + Born update, and the fused x pass
+        (x inverse + Born update + next x forward) of the Born sweeps."""
+        ms = (C.c_double * 6)()
+        by = (C.c_double * 6)()
         L.check(self._lib.cbs_time_sweeps(self.handle, precision_code(precision), int(reps), ms, by))
         return list(ms), list(by)
 

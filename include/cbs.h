@@ -100,6 +100,10 @@ typedef struct {
   int32_t compute_h;    /* cbs_solve: also return H_scat                   */
   double h_scale_re;    /* H = h_scale * curl(E): -i lambda/(2 pi 377)     */
   double h_scale_im;
+  int32_t fuse_sweeps;  /* Born: 1 (default) runs each sweep's x inverse and
+                         * the next sweep's x forward as one pass (same
+                         * results, fewer HBM bytes); 0 runs the five
+                         * separate sweeps.  Served requests always fuse. */
 } cbs_solve_opts;
 
 typedef struct {
@@ -177,12 +181,14 @@ cbs_status cbs_solve_batch(cbs_ctx *ctx, int32_t n, cbs_problem *const *probs,
 /* Measurement aid -- no counterpart in the reference; for benchmarks that
  * need per-kernel durations on the library's own stream.  Runs one Born
  * sweep of the resident source term (cbs_problem_set_source first) in the
- * given precision, then launches each of the five sweep kernels `reps` times
+ * given precision, then launches each of the six sweep kernels `reps` times
  * back to back between two HIP events (the problem's work buffers are left
  * overwritten; a later solve starts afresh).
- * ms[k]: average duration of one launch of kernel k (0 x forward + V x,
- * 1 y forward, 2 z + Green, 3 y inverse, 4 x inverse + Born update);
- * bytes[k]: algorithmic HBM bytes of one launch of kernel k. */
+ * ms[k] (6 entries): average duration of one launch of kernel k (0 x forward
+ * + V x, 1 y forward, 2 z + Green, 3 y inverse, 4 x inverse + Born update,
+ * 5 the fused x pass of a Born sweep: x inverse + Born update + the next
+ * sweep's V x and x forward); a fused Born sweep is kernels 1, 2, 3, 5.
+ * bytes[k] (6 entries): algorithmic HBM bytes of one launch of kernel k. */
 cbs_status cbs_time_sweeps(cbs_problem *p, int32_t precision, int32_t reps, double *ms,
                            double *bytes);
 

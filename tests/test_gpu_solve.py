@@ -43,6 +43,36 @@ def test_born_fixed_iterations_fp64(fm, oracle, absorb, graph):
     assert abs(st.resid - st_o.resid) <= 1e-10 * st_o.resid
 
 
+@pytest.mark.parametrize("grid", [(16, 12, 10), (12, 16, 10)])  # x: specialised, runtime plan
+@pytest.mark.parametrize("absorb", [False, True])
+def test_born_fused_sweeps_same_as_separate(fm, oracle, grid, absorb):
+    """Fusing each sweep's x inverse with the next sweep's x forward changes
+    no bit of the iterates, whatever the chunking; FP64 matches the oracle."""
+    print(f"case born fused sweeps grid={grid} absorb={absorb}", flush=True)
+    bc = oracle.AbsorbingBC(thickness=300e-9, attenuation_thickness=200e-9) if absorb else oracle.PeriodicBC()
+    init, S, rhs, prob = _problem_and_rhs(fm, oracle, grid, bc, seed=23)
+    k = 11
+    x_o, _ = oracle.solve_born(rhs, rhs, init.potential, init.eps_imag, init.green, 0.0, k)
+    for precision in ("fp64", "fp32"):
+        p = prob if precision == "fp64" else fm.CBSProblem(init.potential, spec_from(fm, init),
+                                                           precision="fp32")
+        res = []
+        for fuse, every, graph in ((False, 1, False), (True, 1, False), (True, 4, True),
+                                   (True, 3, False)):
+            E, _, st = p.solve(S, fm.make_opts(method="born", precision=precision, tol=0.0,
+                                               maxiters=k, check_every=every, use_graph=graph,
+                                               fuse_sweeps=fuse), want_h=False)
+            assert st.iters == k
+            res.append(E)
+        for i, E in enumerate(res[1:], 1):
+            assert np.array_equal(E, res[0]), f"{precision} variant {i}: rel {rel(E, res[0]):.3e}"
+        if precision == "fp64":
+            assert rel(res[0], x_o) <= 1e-10
+        else:
+            assert rel(res[0], x_o) <= 1e-5
+            p.close()
+
+
 def test_born_converged_fp64_rhs_rule(fm, oracle):
     print("case born converged fp64", flush=True)
     init, S, rhs, prob = _problem_and_rhs(fm, oracle, (16, 12, 8), oracle.PeriodicBC(), seed=22)
