@@ -215,6 +215,9 @@ __host__ __device__ constexpr int xlines(int nx, int eb, int S, bool rt) {
 #ifndef CBS_XE
 #define CBS_XE 0
 #endif
+#ifndef CBS_XOPS_LATE
+#define CBS_XOPS_LATE 1
+#endif
 __host__ __device__ constexpr int xthreads(int nx, int eb, int S) {
   if (CBS_XE == 0 || !is_ct_length(nx)) return kThreads;
   const int e = xlines(nx, eb, S, false) * nx;

@@ -553,7 +553,10 @@ __device__ __forceinline__ void xinv_tile(const DevProb<T>& pr, int tile, int ti
     constexpr int Qc = 3 * Pc;
     constexpr int QSc = xpad_stride(Qc);
     constexpr int E = (Pc * NC + NT - 1) / NT;
-    // every global load of the tile first: both spectra, ramps, operands
+    // every global load of the tile first: both spectra, ramps and (LATE
+    // false) the epilogue operands; LATE loads the operands after the
+    // transforms, for tiles whose registers cannot hold them all
+    constexpr bool LATE = CBS_XOPS_LATE && eb == 16;
     Cx<T> w[S][E][3];
     Cx<T> rp[E];
     Cx<T> vv[E], a0[E][3], a1[E][3];
@@ -571,7 +574,7 @@ __device__ __forceinline__ void xinv_tile(const DevProb<T>& pr, int tile, int ti
           w[s][k][c] = valid ? pr.w[(s * 3 + c) * pr.ws[2] + wo] : mk<T>(T(0), T(0));
       }
       rp[k] = pr.has_ramp && valid ? ramp_at(pr, m, l % ny, l / ny) : mk<T>(T(1), T(0));
-      xinv_operands(pr, (long long)l * NC + m, valid, vv[k], a0[k], a1[k]);
+      if constexpr (!LATE) xinv_operands(pr, (long long)l * NC + m, valid, vv[k], a0[k], a1[k]);
     }
 #pragma unroll
     for (int s = 0; s < S; ++s) {
@@ -607,6 +610,9 @@ __device__ __forceinline__ void xinv_tile(const DevProb<T>& pr, int tile, int ti
     for (int k = 0; k < E; ++k) {
       const int e = tid + k * NT;
       const int m = e % NC, p = e / NC;
+      if constexpr (LATE) {
+        xinv_operands(pr, (long long)(l0 + p) * NC + m, e < Pc * NC && p < np, vv[k], a0[k], a1[k]);
+      }
       if (e < Pc * NC && p < np) {
         if (S == 2) {
 #pragma unroll
