@@ -16,6 +16,20 @@ constexpr int kZThreads = CBS_ZTHREADS;  // threads per block of the z pass (its
                                          // registers per thread)
 constexpr int kMaxStages = 12;    // radix stages of one axis transform
 constexpr int kLdsBudget = 65536; // LDS bytes a pass tile may use (two blocks per CU)
+// x-pass tile shape switches (defaults are the measured best on MI355X):
+// CBS_XE: elements per thread that set the x-pass block size (0: kThreads);
+// CBS_XOPS_LATE: FP64 x inverse loads its epilogue operands after the
+// transforms; CBS_XJOINT: specialised x tiles transform the S pipelines of a
+// line together (3 S P pencils) rather than one after the other
+#ifndef CBS_XE
+#define CBS_XE 0
+#endif
+#ifndef CBS_XOPS_LATE
+#define CBS_XOPS_LATE 1
+#endif
+#ifndef CBS_XJOINT
+#define CBS_XJOINT 1
+#endif
 constexpr int kNumPart = 5;       // doubles of partial sums written per xinv tile
 constexpr int kXLines = 4;        // x lines per x-pass tile where no tuned width exists
 
@@ -205,19 +219,16 @@ __host__ __device__ constexpr int width_z(int nz, int eb, bool rt) {
 }
 // x-axis passes: 3 components * P lines per tile; specialised lengths run the
 // S pipelines one after the other through the tile, runtime plans hold all S.
+// pipelines transformed together in an x tile
+__host__ __device__ constexpr int xjoint(int S, bool rt) { return (rt || CBS_XJOINT) ? S : 1; }
 __host__ __device__ constexpr int xlines(int nx, int eb, int S, bool rt) {
-  return tile_p(nx, eb, 3 * (rt ? S : 1), width_x(nx, eb, S, rt), rt, 1);
+  return tile_p(nx, eb, 3 * xjoint(S, rt), width_x(nx, eb, S, rt), rt, 1);
 }
 // Threads per block of the specialised x-pass kernels: P n / E for the
 // smallest E >= CBS_XE that gives whole waves within 1024 threads, so every
 // thread owns exactly E elements of the tile (the loads of a tile are then
 // all issued before its first transform); kThreads where none fits.
-#ifndef CBS_XE
-#define CBS_XE 0
-#endif
-#ifndef CBS_XOPS_LATE
-#define CBS_XOPS_LATE 1
-#endif
+
 __host__ __device__ constexpr int xthreads(int nx, int eb, int S) {
   if (CBS_XE == 0 || !is_ct_length(nx)) return kThreads;
   const int e = xlines(nx, eb, S, false) * nx;
@@ -355,7 +366,7 @@ inline int tile_lds_bytes(const DevProb<T>& p, int pass) {
     case 3: {
       const bool rt = p.rt[0] != 0;
       const int P = xlines(nx, eb, p.S, rt);
-      return tw_bytes(nx, eb) + (rt ? 2 : 1) * nx * xpad_stride(3 * (rt ? p.S : 1) * P) * eb;
+      return tw_bytes(nx, eb) + (rt ? 2 : 1) * nx * xpad_stride(3 * xjoint(p.S, rt) * P) * eb;
     }
     case 1: {
       const bool rt = p.rt[1] != 0;

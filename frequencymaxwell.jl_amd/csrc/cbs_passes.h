@@ -84,20 +84,24 @@ __device__ __forceinline__ void xfwd_ct_store(const DevProb<T>& pr, Cx<T>* __res
                                               int l0, int np, int tid) {
   constexpr int eb = (int)sizeof(Cx<T>);
   constexpr int Pc = xlines(NC, eb, S, false);
-  constexpr int Qc = 3 * Pc;
+  constexpr int J = xjoint(S, false);  // pipelines per transform
+  constexpr int Qc = 3 * J * Pc;
   constexpr int QSc = xpad_stride(Qc);
   const int ny = pr.n[1];
 #pragma unroll
-  for (int s = 0; s < S; ++s) {
+  for (int s0 = 0; s0 < S; s0 += J) {
 #pragma unroll
     for (int k = 0; k < E; ++k) {
       const int e = tid + k * NT;
       if (e < Pc * NC) {
         const int m = e % NC, p = e / NC;
-        const Cx<T> r = s == 0 ? rp[k] : conj(rp[k]);
 #pragma unroll
-        for (int c = 0; c < 3; ++c)
-          lds[m * QSc + c * Pc + p] = pr.has_ramp ? cmul_nc(r, u[k][c]) : u[k][c];
+        for (int j = 0; j < J; ++j) {
+          const Cx<T> r = s0 + j == 0 ? rp[k] : conj(rp[k]);
+#pragma unroll
+          for (int c = 0; c < 3; ++c)
+            lds[m * QSc + (3 * j + c) * Pc + p] = pr.has_ramp ? cmul_nc(r, u[k][c]) : u[k][c];
+        }
       }
     }
     __syncthreads();
@@ -110,10 +114,14 @@ __device__ __forceinline__ void xfwd_ct_store(const DevProb<T>& pr, Cx<T>* __res
         const int l = l0 + p;
         const long long wo = (long long)(l / ny) * pr.ws[1] + (long long)(l % ny) * pr.ws[0] + m;
 #pragma unroll
-        for (int c = 0; c < 3; ++c) pr.w[(s * 3 + c) * pr.ws[2] + wo] = lds[m * QSc + c * Pc + p];
+        for (int j = 0; j < J; ++j) {
+#pragma unroll
+          for (int c = 0; c < 3; ++c)
+            pr.w[((s0 + j) * 3 + c) * pr.ws[2] + wo] = lds[m * QSc + (3 * j + c) * Pc + p];
+        }
       }
     }
-    if (S == 2 && s == 0) __syncthreads();  // the next pipeline reuses the tile
+    if (s0 + J < S) __syncthreads();  // the next pipeline reuses the tile
   }
 }
 
@@ -550,8 +558,6 @@ __device__ __forceinline__ void xinv_tile(const DevProb<T>& pr, int tile, int ti
   double acc[kNumPart] = {0.0, 0.0, 0.0, 0.0, 0.0};
   if constexpr (CT) {
     constexpr int Pc = xlines(CT ? NC : 8, eb, S, false);
-    constexpr int Qc = 3 * Pc;
-    constexpr int QSc = xpad_stride(Qc);
     constexpr int E = (Pc * NC + NT - 1) / NT;
     // every global load of the tile first: both spectra, ramps and (LATE
     // false) the epilogue operands; LATE loads the operands after the
@@ -576,16 +582,22 @@ __device__ __forceinline__ void xinv_tile(const DevProb<T>& pr, int tile, int ti
       rp[k] = pr.has_ramp && valid ? ramp_at(pr, m, l % ny, l / ny) : mk<T>(T(1), T(0));
       if constexpr (!LATE) xinv_operands(pr, (long long)l * NC + m, valid, vv[k], a0[k], a1[k]);
     }
+    constexpr int J = xjoint(S, false);  // pipelines per transform
+    constexpr int Qc = 3 * J * Pc;
+    constexpr int QSc = xpad_stride(Qc);
 #pragma unroll
-    for (int s = 0; s < S; ++s) {
-      if (S == 2 && s == 1) __syncthreads();  // the second pipeline reuses the tile
+    for (int s0 = 0; s0 < S; s0 += J) {
+      if (s0 > 0) __syncthreads();  // the next pipeline reuses the tile
 #pragma unroll
       for (int k = 0; k < E; ++k) {
         const int e = tid + k * NT;
         if (e < Pc * NC) {
           const int m = e % NC, p = e / NC;
 #pragma unroll
-          for (int c = 0; c < 3; ++c) lds[m * QSc + c * Pc + p] = w[s][k][c];
+          for (int j = 0; j < J; ++j) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) lds[m * QSc + (3 * j + c) * Pc + p] = w[s0 + j][k][c];
+          }
         }
       }
       __syncthreads();
@@ -595,13 +607,17 @@ __device__ __forceinline__ void xinv_tile(const DevProb<T>& pr, int tile, int ti
         const int e = tid + k * NT;
         if (e < Pc * NC) {
           const int m = e % NC, p = e / NC;
-          // + pipeline: exp(+2 pi i phi) = conj(ramp);  - pipeline: exp(-2 pi i phi) = ramp
-          const Cx<T> r = s == 0 ? conj(rp[k]) : rp[k];
 #pragma unroll
-          for (int c = 0; c < 3; ++c) {
-            const Cx<T> v = lds[m * QSc + c * Pc + p];
-            const Cx<T> t = pr.has_ramp ? cmul_nc(r, v) : v;
-            w[0][k][c] = s == 0 ? t : w[0][k][c] + t;
+          for (int j = 0; j < J; ++j) {
+            const int s = s0 + j;
+            // + pipeline: exp(+2 pi i phi) = conj(ramp);  - pipeline: exp(-2 pi i phi) = ramp
+            const Cx<T> r = s == 0 ? conj(rp[k]) : rp[k];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+              const Cx<T> v = lds[m * QSc + (3 * j + c) * Pc + p];
+              const Cx<T> t = pr.has_ramp ? cmul_nc(r, v) : v;
+              w[0][k][c] = s == 0 ? t : w[0][k][c] + t;
+            }
           }
         }
       }
