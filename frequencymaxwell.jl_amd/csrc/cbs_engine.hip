@@ -195,6 +195,7 @@ template <class T>
 void Pipe<T>::build(Ctx* c, hipStream_t s) {
   const int np = (int)h.size();
   CBS_REQUIRE(np > 0, "empty pipeline");
+  for (auto& d : h) count_tiles(d);  // the fused x pass has tiles of its own
   L = PipeLaunch<T>{};
   L.S = h[0].S;
   L.fused = h[0].fuse_next;

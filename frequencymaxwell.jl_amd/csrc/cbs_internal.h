@@ -176,7 +176,8 @@ __host__ __device__ constexpr int tuned_width(int pass, int n, int eb, int S) {
 #endif
   for (int i = 0; i < kTileTuneN; ++i) {
     const TileTune& t = kTileTune[i];
-    if (t.pass == pass && t.n == n && t.eb == eb && (pass != 0 || t.S == S)) return t.p;
+    if (t.pass == pass && t.n == n && t.eb == eb && ((pass != 0 && pass != 3) || t.S == S))
+      return t.p;
   }
   return 0;
 }
@@ -203,6 +204,17 @@ __host__ __device__ constexpr int width_x(int nx, int eb, int S, bool rt) {
   return (!rt && tuned_width(0, nx, eb, S)) ? tuned_width(0, nx, eb, S) : kXLines;
 #endif
 }
+// the fused x pass of the Born sweeps has widths of its own (table pass 3),
+// falling back to those of the x sweeps
+__host__ __device__ constexpr int width_xf(int nx, int eb, int S, bool rt) {
+#if defined(CBS_TUNE_XFP)
+  return CBS_TUNE_XFP;
+#elif defined(CBS_TUNE_XP)
+  return CBS_TUNE_XP;
+#else
+  return (!rt && tuned_width(3, nx, eb, S)) ? tuned_width(3, nx, eb, S) : width_x(nx, eb, S, rt);
+#endif
+}
 __host__ __device__ constexpr int width_y(int ny, int eb, bool rt) {
 #ifdef CBS_TUNE_YP
   return CBS_TUNE_YP;
@@ -223,6 +235,10 @@ __host__ __device__ constexpr int width_z(int nz, int eb, bool rt) {
 __host__ __device__ constexpr int xjoint(int S, bool rt) { return (rt || CBS_XJOINT) ? S : 1; }
 __host__ __device__ constexpr int xlines(int nx, int eb, int S, bool rt) {
   return tile_p(nx, eb, 3 * xjoint(S, rt), width_x(nx, eb, S, rt), rt, 1);
+}
+// x lines per tile of the fused x pass
+__host__ __device__ constexpr int xlines_f(int nx, int eb, int S, bool rt) {
+  return tile_p(nx, eb, 3 * xjoint(S, rt), width_xf(nx, eb, S, rt), rt, 1);
 }
 // Threads per block of the specialised x-pass kernels: P n / E for the
 // smallest E >= CBS_XE that gives whole waves within 1024 threads, so every
@@ -341,19 +357,27 @@ inline void w_strides(int nx, int ny, int nz, int elem_bytes, long long ws[3]) {
   ws[2] = comp;
 }
 
+// tiles of the x sweeps (fused: of the fused x pass)
+template <class T>
+inline int x_tiles(const DevProb<T>& p, bool fused) {
+  const int eb = (int)sizeof(Cx<T>);
+  const bool rt = p.rt[0] != 0;
+  const int px = fused ? xlines_f(p.n[0], eb, p.S, rt) : xlines(p.n[0], eb, p.S, rt);
+  const int lines = p.n[1] * p.n[2];
+  return (lines + px - 1) / px;
+}
+
 // host-side tile counts (must match the kernels)
 template <class T>
 inline void count_tiles(DevProb<T>& p) {
   const int eb = (int)sizeof(Cx<T>);
   const int nx = p.n[0], ny = p.n[1], nz = p.n[2];
-  const int px = xlines(nx, eb, p.S, p.rt[0] != 0);
   const int py = ycols(ny, eb, p.rt[1] != 0);
   const int pz = zcols(nz, eb, p.rt[2] != 0);
-  const int lines = ny * nz;
-  p.tiles[0] = (lines + px - 1) / px;
+  p.tiles[0] = x_tiles(p, false);
   p.tiles[1] = p.S * 3 * nz * ((nx + py - 1) / py);
   p.tiles[2] = p.S * ny * ((nx + pz - 1) / pz);
-  p.tiles[3] = p.tiles[0];
+  p.tiles[3] = x_tiles(p, p.fuse_next != 0);
 }
 
 // LDS bytes a pass tile of problem p needs
@@ -365,7 +389,7 @@ inline int tile_lds_bytes(const DevProb<T>& p, int pass) {
     case 0:
     case 3: {
       const bool rt = p.rt[0] != 0;
-      const int P = xlines(nx, eb, p.S, rt);
+      const int P = (pass == 3 && p.fuse_next) ? xlines_f(nx, eb, p.S, rt) : xlines(nx, eb, p.S, rt);
       return tw_bytes(nx, eb) + (rt ? 2 : 1) * nx * xpad_stride(3 * xjoint(p.S, rt) * P) * eb;
     }
     case 1: {

@@ -78,12 +78,10 @@ struct alignas(16) C2f {
 // the store of their spectra into pr.w.  u holds the pre-op values (V x or x)
 // of the E elements of this thread (zero outside the tile), rp their ramps.
 // Shared by sweep A and the fused tail of sweep E, so both give the same bits.
-template <class T, int NC, int S, int NT, int E>
+template <class T, int NC, int S, int NT, int E, int Pc>
 __device__ __forceinline__ void xfwd_ct_store(const DevProb<T>& pr, Cx<T>* __restrict__ lds,
                                               const Cx<T> (&u)[E][3], const Cx<T> (&rp)[E],
                                               int l0, int np, int tid) {
-  constexpr int eb = (int)sizeof(Cx<T>);
-  constexpr int Pc = xlines(NC, eb, S, false);
   constexpr int J = xjoint(S, false);  // pipelines per transform
   constexpr int Qc = 3 * J * Pc;
   constexpr int QSc = xpad_stride(Qc);
@@ -200,7 +198,7 @@ __device__ __forceinline__ void xfwd_tile(const DevProb<T>& pr, int tile, int ti
         for (int c = 0; c < 3; ++c) u[k][c] = mk<T>(T(0), T(0));
       }
     }
-    xfwd_ct_store<T, NC, S, NT, E>(pr, lds, u, rp, l0, np, tid);
+    xfwd_ct_store<T, NC, S, NT, E, Pc>(pr, lds, u, rp, l0, np, tid);
   } else {
     const int QS = xpad_stride(3 * S * P);
     for (int e = tid; e < P * n; e += kThreads) {
@@ -549,7 +547,7 @@ __device__ __forceinline__ void xinv_tile(const DevProb<T>& pr, int tile, int ti
   constexpr int eb = (int)sizeof(Cx<T>);
   constexpr int NT = CT ? xthreads(NC, eb, S) : kThreads;
   const int n = CT ? NC : pr.n[0];
-  const int P = xlines(n, eb, S, !CT);
+  const int P = FUSE ? xlines_f(n, eb, S, !CT) : xlines(n, eb, S, !CT);
   Cx<T>* lds = stage_twiddles<T, NT>(pr.tw[0], n, tid);
   const int ny = pr.n[1];
   const int nlines = ny * pr.n[2];
@@ -557,7 +555,7 @@ __device__ __forceinline__ void xinv_tile(const DevProb<T>& pr, int tile, int ti
   const int np = min(P, nlines - l0);
   double acc[kNumPart] = {0.0, 0.0, 0.0, 0.0, 0.0};
   if constexpr (CT) {
-    constexpr int Pc = xlines(CT ? NC : 8, eb, S, false);
+    constexpr int Pc = FUSE ? xlines_f(CT ? NC : 8, eb, S, false) : xlines(CT ? NC : 8, eb, S, false);
     constexpr int E = (Pc * NC + NT - 1) / NT;
     // every global load of the tile first: both spectra, ramps and (LATE
     // false) the epilogue operands; LATE loads the operands after the
@@ -648,7 +646,7 @@ __device__ __forceinline__ void xinv_tile(const DevProb<T>& pr, int tile, int ti
         for (int c = 0; c < 3; ++c) u[k][c] = valid ? cmul_nc(vv[k], w[0][k][c]) : mk<T>(T(0), T(0));
       }
       __syncthreads();  // other waves may still read the inverse tile
-      xfwd_ct_store<T, NC, S, NT, E>(pr, lds, u, rp, l0, np, tid);
+      xfwd_ct_store<T, NC, S, NT, E, Pc>(pr, lds, u, rp, l0, np, tid);
     }
   } else {
     const int Q = 3 * S * P;

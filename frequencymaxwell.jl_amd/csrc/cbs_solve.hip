@@ -403,7 +403,8 @@ Work<T>& prepare_work(Problem& p, int slot, bool need_r) {
   Work<T>& w = work_slot<T>(p, slot);
   DevProb<T> d = make_desc<T>(p, PRE_V, ZOP_GREEN, POST_BORN, nullptr, nullptr, nullptr, nullptr,
                               nullptr, nullptr);
-  w.ensure(p.ctx, p.nvox, p.S, d.tiles[3], need_r, d.ws[2]);
+  // partial sums for the plain and the fused x inverse
+  w.ensure(p.ctx, p.nvox, p.S, std::max(d.tiles[3], x_tiles(d, true)), need_r, d.ws[2]);
   return w;
 }
 template Work<double>& prepare_work<double>(Problem&, int, bool);
