@@ -125,9 +125,19 @@ __device__ __forceinline__ void dft(Cx<T>* v, const Cx<T>* __restrict__ tw, int 
 // ---------------------------------------------------------------------------
 // compile-time transform: in place, registers hold one stage's inputs
 // ---------------------------------------------------------------------------
-template <class T, int N, int Q, int QS, int DIR, int R, int NS, int NT>
-__device__ __forceinline__ void stage_ct(Cx<T>* __restrict__ s, const Cx<T>* __restrict__ tw,
-                                         int tid) {
+// Placeholder for the global-memory accessors of a stage that reads and
+// writes only the LDS tile.
+struct NoTileIO {};
+
+// One radix-R stage over Q pencils of length N in the LDS tile s (row stride
+// QS).  GIN: the stage reads its inputs through ld(q, m) (pencil q, element m)
+// instead of the tile (the first stage of a transform taking its data straight
+// from HBM); GOUT: it writes its outputs through st(q, m, v) instead of the tile
+// (the last stage writing straight to HBM; no barrier needed then).
+template <class T, int N, int Q, int QS, int DIR, int R, int NS, int NT, bool GIN, bool GOUT,
+          class LD, class ST>
+__device__ __forceinline__ void stage_ct_io(Cx<T>* __restrict__ s, const Cx<T>* __restrict__ tw,
+                                            int tid, const LD& ld, const ST& st) {
   constexpr int NB = N / R;
   constexpr int TOTAL = Q * NB;
   constexpr int RD = (TOTAL + NT - 1) / NT;
@@ -139,10 +149,15 @@ __device__ __forceinline__ void stage_ct(Cx<T>* __restrict__ s, const Cx<T>* __r
     if (EXACT || b < TOTAL) {
       const int q = b % Q, t = b / Q;
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[k][r] = s[(t + r * NB) * QS + q];
+      for (int r = 0; r < R; ++r) {
+        if constexpr (GIN) v[k][r] = ld(q, t + r * NB);
+        else v[k][r] = s[(t + r * NB) * QS + q];
+      }
     }
   }
-  __syncthreads();
+  // orders the tile (and, after GIN, the staged twiddles and tables) for the
+  // writes and butterflies below; a GOUT stage reading the tile writes none of it
+  if constexpr (GIN || !GOUT) __syncthreads();
 #pragma unroll
   for (int k = 0; k < RD; ++k) {
     const int b = tid + k * NT;
@@ -160,10 +175,19 @@ __device__ __forceinline__ void stage_ct(Cx<T>* __restrict__ s, const Cx<T>* __r
       dft<DIR, R>(v[k], tw, N);
       const int d = (t / NS) * NS * R + j;
 #pragma unroll
-      for (int r = 0; r < R; ++r) s[(d + r * NS) * QS + q] = v[k][r];
+      for (int r = 0; r < R; ++r) {
+        if constexpr (GOUT) st(q, d + r * NS, v[k][r]);
+        else s[(d + r * NS) * QS + q] = v[k][r];
+      }
     }
   }
-  __syncthreads();
+  if constexpr (!GOUT) __syncthreads();
+}
+
+template <class T, int N, int Q, int QS, int DIR, int R, int NS, int NT>
+__device__ __forceinline__ void stage_ct(Cx<T>* __restrict__ s, const Cx<T>* __restrict__ tw,
+                                         int tid) {
+  stage_ct_io<T, N, Q, QS, DIR, R, NS, NT, false, false>(s, tw, tid, NoTileIO{}, NoTileIO{});
 }
 
 template <class T, int N, int Q, int QS, int DIR, int NT = kThreads, int S = 0, int NS = 1>
@@ -173,6 +197,21 @@ __device__ __forceinline__ void fft_ct(Cx<T>* __restrict__ s, const Cx<T>* __res
     constexpr int R = plan_radix(N, S);
     stage_ct<T, N, Q, QS, DIR, R, NS, NT>(s, tw, tid);
     fft_ct<T, N, Q, QS, DIR, NT, S + 1, NS * R>(s, tw, tid);
+  }
+}
+
+// The compile-time transform with its first stage reading through ld (GIN)
+// and/or its last stage writing through st (GOUT); the tile s carries the
+// stages in between.  The per-pencil arithmetic is that of fft_ct.
+template <class T, int N, int Q, int QS, int DIR, int NT, bool GIN, bool GOUT, int S = 0,
+          int NS = 1, class LD, class ST>
+__device__ __forceinline__ void fft_ct_io(Cx<T>* __restrict__ s, const Cx<T>* __restrict__ tw,
+                                          int tid, const LD& ld, const ST& st) {
+  if constexpr (S < plan_nst(N)) {
+    constexpr int R = plan_radix(N, S);
+    constexpr bool first = S == 0, last = S + 1 == plan_nst(N);
+    stage_ct_io<T, N, Q, QS, DIR, R, NS, NT, GIN && first, GOUT && last>(s, tw, tid, ld, st);
+    fft_ct_io<T, N, Q, QS, DIR, NT, GIN, GOUT, S + 1, NS * R>(s, tw, tid, ld, st);
   }
 }
 

@@ -305,16 +305,21 @@ __device__ __forceinline__ void ypass_tile(const DevProb<T>& pr, int tile, int t
   const int x0 = xb * P;
   const int npx = min(P, nx - x0);
   Cx<T>* base = pr.w + sc * pr.ws[2] + (long long)z * pr.ws[1] + x0;
-  tile_load(lds, base, P, npx, n, pr.ws[0], P, 0, tid);
-  __syncthreads();
-  Cx<T>* res = lds;
   if constexpr (CT) {
+    // first stage straight from HBM, last stage straight to HBM
     constexpr int Pc = ycols(CT ? NC : 8, eb, false);
-    fft_ct<T, NC, Pc, Pc, DIR>(lds, lds_twiddles<T>(), tid);
+    const long long rs = pr.ws[0];
+    auto ld = [&](int q, int m) { return q < npx ? base[m * rs + q] : mk<T>(T(0), T(0)); };
+    auto st = [&](int q, int m, const Cx<T>& v) {
+      if (q < npx) base[m * rs + q] = v;
+    };
+    fft_ct_io<T, NC, Pc, Pc, DIR, kThreads, true, true>(lds, lds_twiddles<T>(), tid, ld, st);
   } else {
-    res = fft_rt<T, DIR>(lds, lds + n * P, n, P, P, pr.plan[1], lds_twiddles<T>(), tid);
+    tile_load(lds, base, P, npx, n, pr.ws[0], P, 0, tid);
+    __syncthreads();
+    Cx<T>* res = fft_rt<T, DIR>(lds, lds + n * P, n, P, P, pr.plan[1], lds_twiddles<T>(), tid);
+    tile_store(res, base, P, npx, n, pr.ws[0], P, 0, tid);
   }
-  tile_store(res, base, P, npx, n, pr.ws[0], P, 0, tid);
 }
 
 // ---------------------------------------------------------------------------
@@ -398,9 +403,11 @@ __device__ __forceinline__ void zpass_tile(const DevProb<T>& pr, int tile, int t
   const int x0 = xb * P;
   const int npx = min(P, nx - x0);
   Cx<T>* base = pr.w + 3 * s * pr.ws[2] + (long long)y * pr.ws[0] + x0;
+  if constexpr (!CT) {
 #pragma unroll
-  for (int c = 0; c < 3; ++c)
-    tile_load<T, NT>(lds, base + c * pr.ws[2], P, npx, n, pr.ws[1], Q, c * P, tid);
+    for (int c = 0; c < 3; ++c)
+      tile_load<T, NT>(lds, base + c * pr.ws[2], P, npx, n, pr.ws[1], Q, c * P, tid);
+  }
   // k values of this tile: kt[m] along z, kt[n + p] along x, kt[n + P] at y
   double* kt = reinterpret_cast<double*>(reinterpret_cast<unsigned char*>(lds) + zk_offset(n, P, eb, !CT));
   {
@@ -416,12 +423,23 @@ __device__ __forceinline__ void zpass_tile(const DevProb<T>& pr, int tile, int t
       kt[i] = v;
     }
   }
-  __syncthreads();
+  // specialised lengths: the forward transform's first stage reads HBM and the
+  // inverse transform's last stage writes it (pencil q = component c, column p)
+  constexpr int Pz = CT ? zcols(CT ? NC : 8, eb, false) : 1;
+  const long long cs = pr.ws[2], zs = pr.ws[1];
+  auto ld = [&](int q, int m) {
+    const int c = q / Pz, p = q % Pz;
+    return p < npx ? base[c * cs + m * zs + p] : mk<T>(T(0), T(0));
+  };
+  auto st = [&](int q, int m, const Cx<T>& v) {
+    const int c = q / Pz, p = q % Pz;
+    if (p < npx) base[c * cs + m * zs + p] = v;
+  };
   Cx<T>* res = lds;
   if constexpr (CT) {
-    constexpr int Pc = zcols(CT ? NC : 8, eb, false);
-    fft_ct<T, NC, 3 * Pc, 3 * Pc, -1, NT>(lds, lds_twiddles<T>(), tid);
+    fft_ct_io<T, NC, 3 * Pz, 3 * Pz, -1, NT, true, false>(lds, lds_twiddles<T>(), tid, ld, st);
   } else {
+    __syncthreads();
     res = fft_rt<T, -1, NT>(lds, lds + n * Q, n, Q, Q, pr.plan[2], lds_twiddles<T>(), tid);
   }
   for (int e = tid; e < P * n; e += NT) {
@@ -437,15 +455,14 @@ __device__ __forceinline__ void zpass_tile(const DevProb<T>& pr, int tile, int t
   }
   __syncthreads();
   if constexpr (CT) {
-    constexpr int Pc = zcols(CT ? NC : 8, eb, false);
-    fft_ct<T, NC, 3 * Pc, 3 * Pc, 1, NT>(lds, lds_twiddles<T>(), tid);
+    fft_ct_io<T, NC, 3 * Pz, 3 * Pz, 1, NT, false, true>(lds, lds_twiddles<T>(), tid, ld, st);
   } else {
     Cx<T>* other = res == lds ? lds + n * Q : lds;
     res = fft_rt<T, 1, NT>(res, other, n, Q, Q, pr.plan[2], lds_twiddles<T>(), tid);
-  }
 #pragma unroll
-  for (int c = 0; c < 3; ++c)
-    tile_store<T, NT>(res, base + c * pr.ws[2], P, npx, n, pr.ws[1], Q, c * P, tid);
+    for (int c = 0; c < 3; ++c)
+      tile_store<T, NT>(res, base + c * pr.ws[2], P, npx, n, pr.ws[1], Q, c * P, tid);
+  }
 }
 
 // ---------------------------------------------------------------------------
