@@ -265,8 +265,7 @@ void BicgRunner<T>::add(const std::vector<long long>& keys, std::vector<SolveIte
   CBS_HIP(hipStreamSynchronize(s));  // dvi, fv and m0 go out of scope
   for (int i = 0; i < n; ++i)
     this->items.push_back({keys[i], xs[i], 0, (long long)io.maxit / 2 + this->k + 2});
-  this->dirty = true;
-  this->hg.clear();
+  this->changed();
 }
 
 template <class T>
@@ -338,7 +337,10 @@ void BicgRunner<T>::launch(int kk) {
   const int n = (int)this->items.size();
   if (n == 0) return;
   if (kk <= 0) kk = this->k;
-  if (this->dirty) rebuild();
+  if (this->dirty) {
+    this->drain();  // the chunks in flight still use the old graph and tables
+    rebuild();
+  }
   const std::vector<int> bud = this->budgets(kk, false);
   const int K = bud[0];
   if (this->gexec && K == this->k && bud[n - 1] == K) {

@@ -523,9 +523,9 @@ void Server::fail_active(cbs_status st, const std::string& err) {
   for (auto& j : all) end_job(j, st, err);
   each_set([&](auto& R, int) {
     R.items.clear();
-    R.dirty = true;
-    R.hg.clear();
-    R.pending = false;
+    R.changed();
+    R.inflight = 0;
+    R.head = 0;
     R.drop_graph();
   });
 }

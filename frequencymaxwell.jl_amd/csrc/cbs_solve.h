@@ -106,10 +106,23 @@ struct RunnerBase {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   DevBuf gath;
-  double* hgp = nullptr;   // pinned landing buffer of the state gather
+  // pinned landing slots of the state gathers of up to two chunks in flight
+  // (a chunk may be launched before the previous one has been checked)
+  double* hgp[2] = {nullptr, nullptr};
   size_t hgp_n = 0;
-  bool pending = false;    // a chunk has been launched and not waited for
-  std::vector<double> hg;  // (done, iters, r2, ans) per item after the last chunk
+  hipEvent_t hev[2] = {nullptr, nullptr};
+  int inflight = 0;        // chunks launched and not waited for
+  int head = 0;            // slot of the oldest of them
+  unsigned gen = 0;        // membership generation (a gather of an older one is stale)
+  unsigned hgen[2] = {0, 0};
+  size_t hgn[2] = {0, 0};  // items gathered into each slot
+  // the membership changed: tables and graph are rebuilt at the next launch
+  void changed() {
+    dirty = true;
+    ++gen;
+    hg.clear();
+  }
+  std::vector<double> hg;  // (done, iters, r2, ans) per item after the last waited chunk
   RunnerBase() = default;
   RunnerBase(const RunnerBase&) = delete;
   RunnerBase& operator=(const RunnerBase&) = delete;
@@ -118,8 +131,12 @@ struct RunnerBase {
   bool empty() const { return items.empty(); }
   // after the chunk's launches: gather (done, iters, r2, ans) of every item
   void launch_gather(RState* const* sts);
-  // block until the launched chunk and its gather are complete
+  // block until the oldest chunk in flight and its gather are complete
   void wait();
+  // wait for every chunk in flight (hg then holds the latest gather)
+  void drain() {
+    while (inflight > 0) wait();
+  }
   // finished items (device done flag set, or cap reached) leave the set
   void collect(std::vector<Finished>& out, bool born);
   void remove(long long key);

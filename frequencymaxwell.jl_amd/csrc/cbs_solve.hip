@@ -103,31 +103,50 @@ void RunnerBase<T>::drop_graph() {
 
 template <class T>
 RunnerBase<T>::~RunnerBase() {
+  if (inflight > 0 && s) (void)hipStreamSynchronize(s);
   drop_graph();
-  if (hgp) (void)hipHostFree(hgp);
+  for (int k = 0; k < 2; ++k) {
+    if (hgp[k]) (void)hipHostFree(hgp[k]);
+    if (hev[k]) (void)hipEventDestroy(hev[k]);
+  }
 }
 
 template <class T>
 void RunnerBase<T>::launch_gather(RState* const* sts) {
+  CBS_REQUIRE(inflight < 2, "more than two chunks in flight");
   const size_t n = items.size();
   if (hgp_n < 4 * n) {
-    if (hgp) CBS_HIP(hipHostFree(hgp));
-    hgp = nullptr;
+    drain();  // the slots are reallocated
+    for (int k = 0; k < 2; ++k) {
+      if (hgp[k]) CBS_HIP(hipHostFree(hgp[k]));
+      hgp[k] = nullptr;
+    }
     hgp_n = 0;
-    CBS_HIP(hipHostMalloc((void**)&hgp, 4 * n * sizeof(double), hipHostMallocDefault));
+    for (int k = 0; k < 2; ++k)
+      CBS_HIP(hipHostMalloc((void**)&hgp[k], 4 * n * sizeof(double), hipHostMallocDefault));
     hgp_n = 4 * n;
   }
+  for (int k = 0; k < 2; ++k)
+    if (!hev[k]) CBS_HIP(hipEventCreateWithFlags(&hev[k], hipEventDisableTiming));
+  const int slot = (head + inflight) & 1;
+  hgen[slot] = gen;
+  hgn[slot] = n;
   launch_gather_state(sts, (int)n, gath.template as<double>(), s);
-  CBS_HIP(hipMemcpyAsync(hgp, gath.p, 4 * n * sizeof(double), hipMemcpyDeviceToHost, s));
-  pending = true;
+  CBS_HIP(hipMemcpyAsync(hgp[slot], gath.p, 4 * n * sizeof(double), hipMemcpyDeviceToHost, s));
+  CBS_HIP(hipEventRecord(hev[slot], s));
+  ++inflight;
 }
 
 template <class T>
 void RunnerBase<T>::wait() {
-  if (!pending) return;
-  CBS_HIP(hipStreamSynchronize(s));
-  pending = false;
-  hg.assign(hgp, hgp + 4 * items.size());
+  if (inflight == 0) return;
+  CBS_HIP(hipEventSynchronize(hev[head]));
+  if (hgen[head] == gen && hgn[head] == items.size())
+    hg.assign(hgp[head], hgp[head] + 4 * hgn[head]);
+  else
+    hg.clear();  // gathered before the membership changed
+  head ^= 1;
+  --inflight;
 }
 
 template <class T>
@@ -149,11 +168,11 @@ void RunnerBase<T>::collect(std::vector<Finished>& out, bool born) {
       keep.push_back(m);
     }
   }
+  hg.clear();
   if (keep.size() != items.size()) {
     items.swap(keep);
-    dirty = true;
+    changed();
   }
-  hg.clear();
 }
 
 template <class T>
@@ -181,8 +200,7 @@ void RunnerBase<T>::remove(long long key) {
   for (size_t i = 0; i < items.size(); ++i)
     if (items[i].key == key) {
       items.erase(items.begin() + i);
-      dirty = true;
-      hg.clear();
+      changed();
       return;
     }
 }
@@ -220,8 +238,7 @@ void BornRunner<T>::add(const std::vector<long long>& keys, std::vector<SolveIte
   }
   for (size_t i = 0; i < xs.size(); ++i)
     this->items.push_back({keys[i], xs[i], 0, (long long)io.maxit + 2 * this->k + 2});
-  this->dirty = true;
-  this->hg.clear();
+  this->changed();
 }
 
 template <class T>
@@ -271,7 +288,10 @@ void BornRunner<T>::launch(int kk) {
   const int n = (int)this->items.size();
   if (n == 0) return;
   if (kk <= 0) kk = this->k;
-  if (this->dirty) rebuild();
+  if (this->dirty) {
+    this->drain();  // the chunks in flight still use the old graph and tables
+    rebuild();
+  }
   const std::vector<int> bud = this->budgets(kk, true);
   const int K = bud[0];
   if (this->gexec && K == this->k && bud[n - 1] == K) {
@@ -317,10 +337,16 @@ void born_solve(Ctx* ctx, std::vector<SolveItem<T>>& it, const IterOpts& io, hip
   for (int i = 0; i < n; ++i) keys[i] = i;
   R.add(keys, it, io);
   std::vector<Finished> done;
+  // the next chunk is launched before the previous one is checked, so the GPU
+  // never waits for the host between chunks; a right-hand side that has
+  // converged skips the sweeps of the extra chunk on the device
+  R.launch();
   while (!R.empty()) {
-    R.chunk();
+    if (R.inflight == 0 || (!R.dirty && R.inflight < 2)) R.launch();
+    R.wait();
     R.collect(done);
   }
+  R.drain();
   states.resize(n);
   result.resize(n);
   for (auto& f : done) {
