@@ -303,8 +303,7 @@ void BicgRunner<T>::rebuild() {
   m2.build(ctx, s);
   fin_mv.build(ctx, sts, parts, nt_mv, s);
   fin_vec.build(ctx, sts, parts, nt_vec, s);
-  this->gath.ensure(ctx, (size_t)n * 4 * sizeof(double));
-  if (this->use_graph) {
+  if (this->want_graph()) {
     CBS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     for (int j = 0; j < this->k; ++j) iteration(n);
     CBS_HIP(hipStreamEndCapture(s, &this->graph));

@@ -87,6 +87,11 @@ struct Finished {
   const void* x;  // device result (3 nvox, precision of the runner)
 };
 
+// Chunks of sets with fewer voxels than this run as one hipGraph; larger sets
+// launch their kernels directly (the host enqueues a chunk in a small part of
+// its GPU time, and a graph launch stalls the queue for 60-100 us per chunk).
+constexpr long long kGraphMaxVoxels = 1LL << 20;
+
 // state shared by both runners: membership, chunk graph, state gather
 template <class T>
 struct RunnerBase {
@@ -105,10 +110,11 @@ struct RunnerBase {
   bool dirty = true;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
-  DevBuf gath;
-  // pinned landing slots of the state gathers of up to two chunks in flight
-  // (a chunk may be launched before the previous one has been checked)
+  // pinned, device-mapped landing slots of the state gathers of up to two
+  // chunks in flight (a chunk may be launched before the previous one has been
+  // checked); the gather kernel writes them directly
   double* hgp[2] = {nullptr, nullptr};
+  double* hgd[2] = {nullptr, nullptr};
   size_t hgp_n = 0;
   hipEvent_t hev[2] = {nullptr, nullptr};
   int inflight = 0;        // chunks launched and not waited for
@@ -129,6 +135,13 @@ struct RunnerBase {
   ~RunnerBase();
   void drop_graph();
   bool empty() const { return items.empty(); }
+  // capture the chunk as a graph (use_graph and a small set)
+  bool want_graph() const {
+    if (!use_graph) return false;
+    long long v = 0;
+    for (const Item& m : items) v += m.it.p->nvox;
+    return v < kGraphMaxVoxels;
+  }
   // after the chunk's launches: gather (done, iters, r2, ans) of every item
   void launch_gather(RState* const* sts);
   // block until the oldest chunk in flight and its gather are complete

@@ -119,11 +119,14 @@ void RunnerBase<T>::launch_gather(RState* const* sts) {
     drain();  // the slots are reallocated
     for (int k = 0; k < 2; ++k) {
       if (hgp[k]) CBS_HIP(hipHostFree(hgp[k]));
-      hgp[k] = nullptr;
+      hgp[k] = hgd[k] = nullptr;
     }
     hgp_n = 0;
-    for (int k = 0; k < 2; ++k)
-      CBS_HIP(hipHostMalloc((void**)&hgp[k], 4 * n * sizeof(double), hipHostMallocDefault));
+    for (int k = 0; k < 2; ++k) {
+      CBS_HIP(hipHostMalloc((void**)&hgp[k], 4 * n * sizeof(double),
+                            hipHostMallocMapped | hipHostMallocCoherent));
+      CBS_HIP(hipHostGetDevicePointer((void**)&hgd[k], hgp[k], 0));
+    }
     hgp_n = 4 * n;
   }
   for (int k = 0; k < 2; ++k)
@@ -131,8 +134,7 @@ void RunnerBase<T>::launch_gather(RState* const* sts) {
   const int slot = (head + inflight) & 1;
   hgen[slot] = gen;
   hgn[slot] = n;
-  launch_gather_state(sts, (int)n, gath.template as<double>(), s);
-  CBS_HIP(hipMemcpyAsync(hgp[slot], gath.p, 4 * n * sizeof(double), hipMemcpyDeviceToHost, s));
+  launch_gather_state(sts, (int)n, hgd[slot], s);
   CBS_HIP(hipEventRecord(hev[slot], s));
   ++inflight;
 }
@@ -269,8 +271,7 @@ void BornRunner<T>::rebuild() {
     nt.push_back(pipe[0].h[i].tiles[3]);
   }
   fin.build(this->ctx, sts, parts, nt, this->s);
-  this->gath.ensure(this->ctx, (size_t)n * 4 * sizeof(double));
-  if (this->use_graph) {
+  if (this->want_graph()) {
     CBS_HIP(hipStreamBeginCapture(this->s, hipStreamCaptureModeThreadLocal));
     for (int j = 0; j < this->k; ++j) {
       pipe[j & 1].launch(this->s);
