@@ -15,13 +15,16 @@ namespace cbs {
 // reductions and solver state
 // ---------------------------------------------------------------------------
 // Born: one block per RHS.  Sums |r|^2 of this iteration and advances the state.
-__global__ void __launch_bounds__(kThreads)
+// A wide block with several loads in flight per thread: this one-block kernel
+// runs after every sweep, so its latency is on the critical path.
+constexpr int kFinThreads = 1024;
+__global__ void __launch_bounds__(kFinThreads)
 k_born_finalize(RState* const* __restrict__ sts, double* const* __restrict__ parts,
                 const int* __restrict__ ntiles, int parity) {
-  __shared__ double red[kThreads / 64];
+  __shared__ double red[kFinThreads / 64];
   RState* st = sts[blockIdx.x];
   if (st->done) return;
-  const double r2 = block_sum_parts(parts[blockIdx.x], ntiles[blockIdx.x], 0, red);
+  const double r2 = block_sum_col<kFinThreads>(parts[blockIdx.x], ntiles[blockIdx.x], 0, red);
   if (threadIdx.x == 0) {
     st->r2 = r2;
     const int it = st->iters;
@@ -195,7 +198,7 @@ void launch_pipeline(const PipeLaunch<T>& L, hipStream_t s) {
 
 void launch_born_finalize(RState* const* sts, double* const* parts, const int* ntiles,
                           int nrhs, int parity, hipStream_t s) {
-  hipLaunchKernelGGL(k_born_finalize, dim3(nrhs), dim3(kThreads), 0, s, sts, parts, ntiles,
+  hipLaunchKernelGGL(k_born_finalize, dim3(nrhs), dim3(kFinThreads), 0, s, sts, parts, ntiles,
                      parity);
 }
 

@@ -52,4 +52,34 @@ __device__ __forceinline__ double block_sum_parts(const double* __restrict__ par
   return s;
 }
 
+// Column k of kNumPart-strided partials [0, ntile) summed by a block of NT
+// threads, four rows in flight per thread (each thread still adds its rows in
+// ascending order); every thread gets the total.
+template <int NT>
+__device__ __forceinline__ double block_sum_col(const double* __restrict__ part, int ntile, int k,
+                                                double* red) {
+  const int tid = threadIdx.x;
+  double v = 0.0;
+  int i = tid;
+  for (; i + 3 * NT < ntile; i += 4 * NT) {
+    const double a = part[(long long)i * kNumPart + k];
+    const double b = part[(long long)(i + NT) * kNumPart + k];
+    const double c = part[(long long)(i + 2 * NT) * kNumPart + k];
+    const double d = part[(long long)(i + 3 * NT) * kNumPart + k];
+    v += a;
+    v += b;
+    v += c;
+    v += d;
+  }
+  for (; i < ntile; i += NT) v += part[(long long)i * kNumPart + k];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  __syncthreads();
+  if ((tid & 63) == 0) red[tid >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int w = 0; w < NT / 64; ++w) s += red[w];
+  return s;
+}
+
 }  // namespace cbs
