@@ -215,6 +215,25 @@ __device__ __forceinline__ void fft_ct_io(Cx<T>* __restrict__ s, const Cx<T>* __
   }
 }
 
+// All stages but the last of the compile-time transform with the radix plan
+// run in reverse order (the Stockham stages accept any order of the factors);
+// the first of them reads through ld when GIN.  The remaining stage, of radix
+// plan_radix(N, 0) and span N / plan_radix(N, 0), is left to the caller: its
+// outputs m = t + r N/R are exactly the inputs of the first stage of the
+// forward-order transform, so a caller can fuse work between the two.
+template <class T, int N, int Q, int QS, int DIR, int NT, bool GIN, int S = 0, int NS = 1,
+          class LD, class ST>
+__device__ __forceinline__ void fft_ct_rev_head(Cx<T>* __restrict__ s,
+                                                const Cx<T>* __restrict__ tw, int tid,
+                                                const LD& ld, const ST& st) {
+  constexpr int nst = plan_nst(N);
+  if constexpr (S < nst - 1) {
+    constexpr int R = plan_radix(N, nst - 1 - S);
+    stage_ct_io<T, N, Q, QS, DIR, R, NS, NT, GIN && S == 0, false>(s, tw, tid, ld, st);
+    fft_ct_rev_head<T, N, Q, QS, DIR, NT, GIN, S + 1, NS * R>(s, tw, tid, ld, st);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // runtime-plan transform: ping-pong between two LDS buffers; returns the
 // buffer holding the result.  Used for lengths without a specialisation.

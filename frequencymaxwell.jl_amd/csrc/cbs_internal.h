@@ -30,6 +30,11 @@ constexpr int kLdsBudget = 65536; // LDS bytes a pass tile may use (two blocks p
 #ifndef CBS_XJOINT
 #define CBS_XJOINT 1
 #endif
+// z pass: the last forward stage, the k-space multiply and the first inverse
+// stage run in registers in one pass over the tile
+#ifndef CBS_ZMERGE
+#define CBS_ZMERGE 1
+#endif
 constexpr int kNumPart = 5;       // doubles of partial sums written per xinv tile
 constexpr int kXLines = 4;        // x lines per x-pass tile where no tuned width exists
 

@@ -386,6 +386,75 @@ __device__ __forceinline__ void kspace_op(const DevProb<T>& pr, double kxd, doub
   }
 }
 
+// The middle of a specialised z tile in one stage: the last stage of the
+// forward transform (reverse radix order, so its radix R = plan_radix(N, 0)),
+// the k-space multiply of the three components, and the first stage of the
+// inverse transform (forward radix order, radix R, span 1).  The forward stage
+// leaves element m = t + r N/R of each pencil in the thread that the inverse
+// stage reads it from, so one item (column p, index t) holds all 3 R values in
+// registers.  GIN / GOUT: the transform has a single stage, read / write HBM.
+template <class T, int N, int P, int NT, bool GIN, bool GOUT, class LD, class ST>
+__device__ __forceinline__ void zpass_mid(const DevProb<T>& pr, Cx<T>* __restrict__ s,
+                                          const Cx<T>* __restrict__ tw,
+                                          const double* __restrict__ kt, int npx, int tid,
+                                          const LD& ld, const ST& st) {
+  constexpr int R = plan_radix(N, 0);
+  constexpr int NB = N / R;
+  constexpr int QS = 3 * P;
+  constexpr int TOTAL = P * NB;
+  constexpr int RD = (TOTAL + NT - 1) / NT;
+  constexpr bool EXACT = (TOTAL % NT) == 0;
+  Cx<T> v[RD][3][R];
+#pragma unroll
+  for (int k = 0; k < RD; ++k) {
+    const int b = tid + k * NT;
+    if (EXACT || b < TOTAL) {
+      const int p = b % P, t = b / P;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          if constexpr (GIN) v[k][c][r] = ld(c * P + p, t + r * NB);
+          else v[k][c][r] = s[(t + r * NB) * QS + c * P + p];
+        }
+      }
+    }
+  }
+  if constexpr (GIN || !GOUT) __syncthreads();
+#pragma unroll
+  for (int k = 0; k < RD; ++k) {
+    const int b = tid + k * NT;
+    if (EXACT || b < TOTAL) {
+      const int p = b % P, t = b / P;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        // forward stage of span NB: twiddles exp(-2 pi i t r / N)
+        if constexpr (NB > 1) {
+#pragma unroll
+          for (int r = 1; r < R; ++r) v[k][c][r] = v[k][c][r] * tw[t * r];
+        }
+        dft<-1, R>(v[k][c], tw, N);
+      }
+      if (p < npx) {
+        const double kx = kt[N + p], ky = kt[N + P];
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+          kspace_op(pr, kx, ky, kt[t + r * NB], v[k][0][r], v[k][1][r], v[k][2][r]);
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        dft<1, R>(v[k][c], tw, N);  // inverse stage of span 1: no twiddles
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          if constexpr (GOUT) st(c * P + p, t * R + r, v[k][c][r]);
+          else s[(t * R + r) * QS + c * P + p] = v[k][c][r];
+        }
+      }
+    }
+  }
+  if constexpr (!GOUT) __syncthreads();
+}
+
 template <class T, int NC, int NT = kThreads>
 __device__ __forceinline__ void zpass_tile(const DevProb<T>& pr, int tile, int tid) {
   constexpr bool CT = NC > 0;
@@ -436,7 +505,16 @@ __device__ __forceinline__ void zpass_tile(const DevProb<T>& pr, int tile, int t
     if (p < npx) base[c * cs + m * zs + p] = v;
   };
   Cx<T>* res = lds;
-  if constexpr (CT) {
+  if constexpr (CT && CBS_ZMERGE) {
+    constexpr int nst = plan_nst(CT ? NC : 8);
+    const Cx<T>* tw = lds_twiddles<T>();
+    fft_ct_rev_head<T, NC, 3 * Pz, 3 * Pz, -1, NT, true>(lds, tw, tid, ld, st);
+    zpass_mid<T, NC, Pz, NT, nst == 1, nst == 1>(pr, lds, tw, kt, npx, tid, ld, st);
+    if constexpr (nst > 1)
+      fft_ct_io<T, NC, 3 * Pz, 3 * Pz, 1, NT, false, true, 1, plan_radix(CT ? NC : 8, 0)>(
+          lds, tw, tid, ld, st);
+    return;
+  } else if constexpr (CT) {
     fft_ct_io<T, NC, 3 * Pz, 3 * Pz, -1, NT, true, false>(lds, lds_twiddles<T>(), tid, ld, st);
   } else {
     __syncthreads();
