@@ -2,6 +2,7 @@
 #pragma once
 
 #include "cbs_internal.h"
+#include "cbs_reduce.h"
 
 namespace cbs {
 
@@ -37,12 +38,13 @@ void launch_xfwd(const PipeLaunch<T>& L, hipStream_t s);
 template <class T>
 void launch_xinv(const PipeLaunch<T>& L, hipStream_t s);
 template <class T>
-void launch_ypass(const PipeLaunch<T>& L, int dir, hipStream_t s);
+void launch_ypass(const PipeLaunch<T>& L, int dir, hipStream_t s, const BornFin& fin = BornFin{});
 template <class T>
 void launch_zpass(const PipeLaunch<T>& L, hipStream_t s);
 
+// fin: the Born finalize of the previous sweep, run by the y forward launch
 template <class T>
-void launch_pipeline(const PipeLaunch<T>& L, hipStream_t s);
+void launch_pipeline(const PipeLaunch<T>& L, hipStream_t s, const BornFin& fin = BornFin{});
 // one of the five sweeps: 0 xfwd, 1 y forward, 2 z (k-space), 3 y inverse, 4 xinv
 template <class T>
 void launch_pass(const PipeLaunch<T>& L, int k, hipStream_t s);

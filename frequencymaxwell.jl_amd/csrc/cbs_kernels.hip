@@ -15,36 +15,10 @@ namespace cbs {
 // reductions and solver state
 // ---------------------------------------------------------------------------
 // Born: one block per RHS.  Sums |r|^2 of this iteration and advances the state.
-// A wide block with several loads in flight per thread: this one-block kernel
-// runs after every sweep, so its latency is on the critical path.
-constexpr int kFinThreads = 1024;
-__global__ void __launch_bounds__(kFinThreads)
-k_born_finalize(RState* const* __restrict__ sts, double* const* __restrict__ parts,
-                const int* __restrict__ ntiles, int parity) {
-  __shared__ double red[kFinThreads / 64];
-  RState* st = sts[blockIdx.x];
-  if (st->done) return;
-  const double r2 = block_sum_col<kFinThreads>(parts[blockIdx.x], ntiles[blockIdx.x], 0, red);
-  if (threadIdx.x == 0) {
-    st->r2 = r2;
-    const int it = st->iters;
-    st->iters = it + 1;
-    if (st->thr2 < 0.0) {
-      const double t = st->abstol + st->tol * sqrt(r2);
-      st->thr2 = t * t;
-    }
-    if (it < 512) st->hist[it] = st->bnorm2 > 0 ? sqrt(r2 / st->bnorm2) : 0.0;
-    if (!(r2 == r2) || r2 > 1e300) {
-      st->done = 3;
-      st->ans = parity;
-    } else if (r2 <= st->thr2) {
-      st->done = 1;
-      st->ans = parity;  // the operand of this sweep met the tolerance
-    } else if (st->iters >= st->maxit) {
-      st->done = 2;
-      st->ans = 1 - parity;  // the freshly updated iterate
-    }
-  }
+__global__ void __launch_bounds__(kThreads)
+k_born_finalize(BornFin f) {
+  __shared__ double red[kThreads / 64];
+  born_finalize_rhs(f, blockIdx.x, red);
 }
 
 __global__ void k_gather_state(RState* const* __restrict__ sts, int n, double* __restrict__ out) {
@@ -188,9 +162,9 @@ void launch_pass(const PipeLaunch<T>& L, int k, hipStream_t s) {
 }
 
 template <class T>
-void launch_pipeline(const PipeLaunch<T>& L, hipStream_t s) {
+void launch_pipeline(const PipeLaunch<T>& L, hipStream_t s, const BornFin& fin) {
   if (!L.fused) launch_xfwd<T>(L, s);
-  launch_ypass<T>(L, -1, s);
+  launch_ypass<T>(L, -1, s, fin);
   launch_zpass<T>(L, s);
   launch_ypass<T>(L, 1, s);
   launch_xinv<T>(L, s);
@@ -198,8 +172,13 @@ void launch_pipeline(const PipeLaunch<T>& L, hipStream_t s) {
 
 void launch_born_finalize(RState* const* sts, double* const* parts, const int* ntiles,
                           int nrhs, int parity, hipStream_t s) {
-  hipLaunchKernelGGL(k_born_finalize, dim3(nrhs), dim3(kFinThreads), 0, s, sts, parts, ntiles,
-                     parity);
+  BornFin f;
+  f.sts = sts;
+  f.parts = parts;
+  f.ntiles = ntiles;
+  f.n = nrhs;
+  f.parity = parity;
+  hipLaunchKernelGGL(k_born_finalize, dim3(nrhs), dim3(kThreads), 0, s, f);
 }
 
 void launch_gather_state(RState* const* sts, int n, double* out, hipStream_t s) {
@@ -255,7 +234,7 @@ void launch_gamma_mul(Cx<T>* dst, const Cx<T>* src, const Cx<T>* V, long long nv
 }
 
 #define CBS_INST(T)                                                                         \
-  template void launch_pipeline<T>(const PipeLaunch<T>&, hipStream_t);                      \
+  template void launch_pipeline<T>(const PipeLaunch<T>&, hipStream_t, const BornFin&);      \
   template void launch_pass<T>(const PipeLaunch<T>&, int, hipStream_t);                     \
   template int launch_gnorm_parts<T>(const Cx<T>*, const Cx<T>*, long long, double, double*, \
                                      hipStream_t);                                          \

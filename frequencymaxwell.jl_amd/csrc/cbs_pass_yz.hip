@@ -7,11 +7,20 @@ namespace cbs {
 
 template <class T, int DIR, int NC>
 __global__ void __launch_bounds__(kThreads)
-k_ypass(const DevProb<T>* __restrict__ probs, const int* __restrict__ bstart, int nprob) {
-  const int pi = find_prob(bstart, nprob, blockIdx.x);
+k_ypass(const DevProb<T>* __restrict__ probs, const int* __restrict__ bstart, int nprob,
+        BornFin fin) {
+  // the first fin.n blocks finalize the previous Born sweep; a tile block that
+  // reads a done flag before it is set only transforms the spectra of a
+  // converged right-hand side, whose later kernels all skip
+  if ((int)blockIdx.x < fin.n) {
+    born_finalize_rhs(fin, blockIdx.x, reinterpret_cast<double*>(g_smem));
+    return;
+  }
+  const int blk = blockIdx.x - fin.n;
+  const int pi = find_prob(bstart, nprob, blk);
   const DevProb<T>& pr = probs[pi];
   if (skip_prob(pr)) return;
-  const int tile = blockIdx.x - bstart[pi];
+  const int tile = blk - bstart[pi];
   const int tid = threadIdx.x;
   if constexpr (NC != 0) {
     ypass_tile<T, NC, DIR>(pr, tile, tid);
@@ -48,9 +57,9 @@ k_zpass(const DevProb<T>* __restrict__ probs, const int* __restrict__ bstart, in
 }
 
 template <class T, int DIR, int NC>
-static void go_y(const PassLaunch<T>& G, hipStream_t s) {
-  hipLaunchKernelGGL((k_ypass<T, DIR, NC>), dim3(G.blocks), dim3(kThreads), G.lds, s, G.probs,
-                     G.bstart, G.nprob);
+static void go_y(const PassLaunch<T>& G, const BornFin& fin, hipStream_t s) {
+  hipLaunchKernelGGL((k_ypass<T, DIR, NC>), dim3(G.blocks + fin.n), dim3(kThreads), G.lds, s,
+                     G.probs, G.bstart, G.nprob, fin);
 }
 template <class T, int NC>
 static void go_z(const PassLaunch<T>& G, hipStream_t s) {
@@ -59,21 +68,22 @@ static void go_z(const PassLaunch<T>& G, hipStream_t s) {
 }
 
 template <class T, int DIR>
-static void y_len(const PassLaunch<T>& G, hipStream_t s) {
+static void y_len(const PassLaunch<T>& G, const BornFin& fin, hipStream_t s) {
   switch (G.len) {
 #define CBS_CASE_(L) \
-    case L: go_y<T, DIR, L>(G, s); break;
+    case L: go_y<T, DIR, L>(G, fin, s); break;
       CBS_CT_LENGTHS(CBS_CASE_)
 #undef CBS_CASE_
-    default: go_y<T, DIR, 0>(G, s); break;
+    default: go_y<T, DIR, 0>(G, fin, s); break;
   }
 }
 
 template <class T>
-void launch_ypass(const PipeLaunch<T>& L, int dir, hipStream_t s) {
+void launch_ypass(const PipeLaunch<T>& L, int dir, hipStream_t s, const BornFin& fin) {
   for (int i = 0; i < L.ng[1]; ++i) {
-    if (dir < 0) y_len<T, -1>(L.g[1][i], s);
-    else y_len<T, 1>(L.g[1][i], s);
+    const BornFin f = i == 0 ? fin : BornFin{};  // the first group's launch carries it
+    if (dir < 0) y_len<T, -1>(L.g[1][i], f, s);
+    else y_len<T, 1>(L.g[1][i], f, s);
   }
 }
 
@@ -91,8 +101,8 @@ void launch_zpass(const PipeLaunch<T>& L, hipStream_t s) {
   }
 }
 
-template void launch_ypass<float>(const PipeLaunch<float>&, int, hipStream_t);
-template void launch_ypass<double>(const PipeLaunch<double>&, int, hipStream_t);
+template void launch_ypass<float>(const PipeLaunch<float>&, int, hipStream_t, const BornFin&);
+template void launch_ypass<double>(const PipeLaunch<double>&, int, hipStream_t, const BornFin&);
 template void launch_zpass<float>(const PipeLaunch<float>&, hipStream_t);
 template void launch_zpass<double>(const PipeLaunch<double>&, hipStream_t);
 

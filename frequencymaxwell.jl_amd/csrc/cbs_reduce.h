@@ -82,4 +82,43 @@ __device__ __forceinline__ double block_sum_col(const double* __restrict__ part,
   return s;
 }
 
+// The Born finalize of right-hand sides (one block each): sums |r|^2 of the
+// sweep from the partials of its x inverse tiles and advances the state.
+// n = 0: no finalize.  Run by its own kernel or by the first n blocks of the
+// next sweep's y forward launch; both use blocks of kThreads and this code,
+// so a right-hand side's residuals do not depend on where it runs.
+struct BornFin {
+  RState* const* sts = nullptr;
+  double* const* parts = nullptr;
+  const int* ntiles = nullptr;
+  int n = 0;
+  int parity = 0;  // the sweep's double-buffer parity
+};
+
+__device__ __forceinline__ void born_finalize_rhs(const BornFin& f, int i, double* red) {
+  RState* st = f.sts[i];
+  if (st->done) return;
+  const double r2 = block_sum_col<kThreads>(f.parts[i], f.ntiles[i], 0, red);
+  if (threadIdx.x == 0) {
+    st->r2 = r2;
+    const int it = st->iters;
+    st->iters = it + 1;
+    if (st->thr2 < 0.0) {
+      const double t = st->abstol + st->tol * sqrt(r2);
+      st->thr2 = t * t;
+    }
+    if (it < 512) st->hist[it] = st->bnorm2 > 0 ? sqrt(r2 / st->bnorm2) : 0.0;
+    if (!(r2 == r2) || r2 > 1e300) {
+      st->done = 3;
+      st->ans = f.parity;
+    } else if (r2 <= st->thr2) {
+      st->done = 1;
+      st->ans = f.parity;  // the operand of this sweep met the tolerance
+    } else if (st->iters >= st->maxit) {
+      st->done = 2;
+      st->ans = 1 - f.parity;  // the freshly updated iterate
+    }
+  }
+}
+
 }  // namespace cbs

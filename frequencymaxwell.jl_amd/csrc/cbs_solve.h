@@ -173,6 +173,12 @@ struct BornRunner : RunnerBase<T> {
   // the same sequence of operations whatever the chunking or the mix
   bool fuse = true;
   Pipe<T> prime;
+  BornFin fin_args(int n, int parity) const;
+  // sweep j of a chunk over the first na members; fused sweeps also finalize
+  // sweep j - 1 of its first prev members (prev = 0 at the chunk's start)
+  void sweep(const PipeLaunch<T>& L, int j, int na, int prev);
+  // after the K sweeps of a chunk: the last sweep's finalize (fused sweeps)
+  void finish(int K, int prev);
   void add(const std::vector<long long>& keys, std::vector<SolveItem<T>>& xs,
            const IterOpts& io);
   // k iterations of every member (k even; 0: the runner's own k), no wait

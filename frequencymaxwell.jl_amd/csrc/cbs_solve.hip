@@ -244,6 +244,37 @@ void BornRunner<T>::add(const std::vector<long long>& keys, std::vector<SolveIte
 }
 
 template <class T>
+BornFin BornRunner<T>::fin_args(int n, int parity) const {
+  BornFin f;
+  f.sts = fin.sts.template as<RState*>();
+  f.parts = fin.parts.template as<double*>();
+  f.ntiles = fin.ntiles.template as<int>();
+  f.n = n;
+  f.parity = parity;
+  return f;
+}
+
+template <class T>
+void BornRunner<T>::sweep(const PipeLaunch<T>& L, int j, int na, int prev) {
+  if (fuse) {
+    // the finalize of sweep j - 1 (prev members) rides on this sweep's y forward
+    launch_pipeline<T>(L, this->s, fin_args(prev, (j - 1) & 1));
+  } else {
+    launch_pipeline<T>(L, this->s);
+    launch_born_finalize(fin.sts.template as<RState*>(), fin.parts.template as<double*>(),
+                         fin.ntiles.template as<int>(), na, j & 1, this->s);
+  }
+}
+
+template <class T>
+void BornRunner<T>::finish(int K, int prev) {
+  // fused sweeps: the last sweep's finalize, before the chunk's state gather
+  if (fuse && prev > 0)
+    launch_born_finalize(fin.sts.template as<RState*>(), fin.parts.template as<double*>(),
+                         fin.ntiles.template as<int>(), prev, (K - 1) & 1, this->s);
+}
+
+template <class T>
 void BornRunner<T>::rebuild() {
   this->drop_graph();
   this->sort_items();
@@ -273,11 +304,8 @@ void BornRunner<T>::rebuild() {
   fin.build(this->ctx, sts, parts, nt, this->s);
   if (this->want_graph()) {
     CBS_HIP(hipStreamBeginCapture(this->s, hipStreamCaptureModeThreadLocal));
-    for (int j = 0; j < this->k; ++j) {
-      pipe[j & 1].launch(this->s);
-      launch_born_finalize(fin.sts.as<RState*>(), fin.parts.as<double*>(), fin.ntiles.as<int>(),
-                           n, j & 1, this->s);
-    }
+    for (int j = 0; j < this->k; ++j) sweep(pipe[j & 1].L, j, n, j > 0 ? n : 0);
+    finish(this->k, n);
     CBS_HIP(hipStreamEndCapture(this->s, &this->graph));
     CBS_HIP(hipGraphInstantiate(&this->gexec, this->graph, nullptr, nullptr, 0));
   }
@@ -299,7 +327,7 @@ void BornRunner<T>::launch(int kk) {
     CBS_HIP(hipGraphLaunch(this->gexec, this->s));
   } else {
     // iteration j runs the members whose budget exceeds j: a prefix
-    int na = n;
+    int na = n, prev = 0;
     PipeLaunch<T> Lp[2] = {pipe[0].L, pipe[1].L};
     for (int j = 0; j < K; ++j) {
       int m = na;
@@ -309,10 +337,10 @@ void BornRunner<T>::launch(int kk) {
         Lp[0] = pipe[0].prefix(na);
         Lp[1] = pipe[1].prefix(na);
       }
-      launch_pipeline<T>(Lp[j & 1], this->s);
-      launch_born_finalize(fin.sts.as<RState*>(), fin.parts.as<double*>(), fin.ntiles.as<int>(),
-                           na, j & 1, this->s);
+      sweep(Lp[j & 1], j, na, prev);
+      prev = na;
     }
+    finish(K, prev);
   }
   CBS_HIP(hipGetLastError());
   for (int i = 0; i < n; ++i) this->items[i].launched += bud[i];
