@@ -13,9 +13,71 @@
 // the 1/n factor (the callers fold 1/N into the k-space multiply).
 #pragma once
 
+#include <type_traits>
+
 #include "cbs_internal.h"
 
 namespace cbs {
+
+// cos and sin of 2 pi m / R at compile time: quadrant reduction in integers,
+// then the Taylor series on [0, pi/2) (the constant twiddles of the composite
+// register butterflies below)
+constexpr double kHalfPi = 1.5707963267948966192313216916398;
+constexpr double ct_taylor_sin(double x) {
+  double t = x, s = x;
+  for (int k = 1; k < 14; ++k) {
+    t *= -x * x / double((2 * k) * (2 * k + 1));
+    s += t;
+  }
+  return s;
+}
+constexpr double ct_taylor_cos(double x) {
+  double t = 1.0, s = 1.0;
+  for (int k = 1; k < 14; ++k) {
+    t *= -x * x / double((2 * k - 1) * (2 * k));
+    s += t;
+  }
+  return s;
+}
+// angle 2 pi m / R = q pi/2 + t with 0 <= t < pi/2; t > pi/4 is taken as
+// pi/2 - x (cos and sin swap), so the series runs on 0 <= x <= pi/4
+constexpr double ct_qx(int m, int R, int& q, bool& sw) {
+  m %= R;
+  q = 4 * m / R;
+  const int y = 4 * m - q * R;
+  sw = 2 * y > R;
+  return kHalfPi * double(sw ? R - y : y) / double(R);
+}
+constexpr double ct_cos2pi(int m, int R) {
+  int q = 0;
+  bool sw = false;
+  const double x = ct_qx(m, R, q, sw);
+  const double c = sw ? ct_taylor_sin(x) : ct_taylor_cos(x);
+  const double sn = sw ? ct_taylor_cos(x) : ct_taylor_sin(x);
+  return q == 0 ? c : q == 1 ? -sn : q == 2 ? -c : sn;
+}
+constexpr double ct_sin2pi(int m, int R) {
+  int q = 0;
+  bool sw = false;
+  const double x = ct_qx(m, R, q, sw);
+  const double c = sw ? ct_taylor_sin(x) : ct_taylor_cos(x);
+  const double sn = sw ? ct_taylor_cos(x) : ct_taylor_sin(x);
+  return q == 0 ? sn : q == 1 ? c : q == 2 ? -sn : -c;
+}
+template <int M, int R>
+struct Wc {
+  static constexpr double c = ct_cos2pi(M, R);
+  static constexpr double s = ct_sin2pi(M, R);
+};
+
+// f(integral_constant<int, i>) for i = B .. E-1, unrolled at compile time
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
 
 template <class T>
 struct K {
@@ -113,12 +175,64 @@ __device__ __forceinline__ void dftg(Cx<T>* v, const Cx<T>* __restrict__ tw, int
 }
 
 template <int DIR, int R, class T>
+__device__ __forceinline__ void dft(Cx<T>* v, const Cx<T>* __restrict__ tw, int n);
+
+// composite radix R = R1 R2 in registers (Cooley-Tukey: R2 DFTs of R1 points
+// over v[R2 n1 + n2], twiddles w_R^(k1 n2), R1 DFTs of R2 points giving
+// X[k1 + R1 k2]); the twiddles are compile-time constants
+template <int DIR, int R1, int R2, class T>
+__device__ __forceinline__ void dftc(Cx<T>* v, const Cx<T>* __restrict__ tw, int n) {
+  constexpr int R = R1 * R2;
+  Cx<T> a[R];
+  static_for<0, R2>([&](auto n2c) {
+    constexpr int n2 = decltype(n2c)::value;
+    Cx<T> t[R1];
+#pragma unroll
+    for (int n1 = 0; n1 < R1; ++n1) t[n1] = v[R2 * n1 + n2];
+    dft<DIR, R1>(t, tw, n);
+    static_for<0, R1>([&](auto k1c) {
+      constexpr int k1 = decltype(k1c)::value;
+      constexpr int m = (k1 * n2) % R;
+      Cx<T> x = t[k1];
+      if constexpr (m == 0) {
+      } else if constexpr (4 * m == R) {
+        x = mul_di<DIR>(x);
+      } else if constexpr (2 * m == R) {
+        x = mk<T>(-x.re, -x.im);
+      } else if constexpr (4 * m == 3 * R) {
+        x = mul_di<-DIR>(x);
+      } else {
+        x = x * mk<T>(T(Wc<m, R>::c), T(DIR * Wc<m, R>::s));
+      }
+      a[k1 * R2 + n2] = x;
+    });
+  });
+#pragma unroll
+  for (int k1 = 0; k1 < R1; ++k1) {
+    Cx<T> t[R2];
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) t[n2] = a[k1 * R2 + n2];
+    dft<DIR, R2>(t, tw, n);
+#pragma unroll
+    for (int k2 = 0; k2 < R2; ++k2) v[k1 + R1 * k2] = t[k2];
+  }
+}
+
+template <int DIR, int R, class T>
 __device__ __forceinline__ void dft(Cx<T>* v, const Cx<T>* __restrict__ tw, int n) {
   if constexpr (R == 2) dft2<DIR>(v);
   else if constexpr (R == 3) dft3<DIR>(v);
   else if constexpr (R == 4) dft4<DIR>(v);
   else if constexpr (R == 5) dft5<DIR>(v);
   else if constexpr (R == 8) dft8<DIR>(v);
+  else if constexpr (R == 6) dftc<DIR, 3, 2>(v, tw, n);
+  else if constexpr (R == 9) dftc<DIR, 3, 3>(v, tw, n);
+  else if constexpr (R == 10) dftc<DIR, 5, 2>(v, tw, n);
+  else if constexpr (R == 12) dftc<DIR, 4, 3>(v, tw, n);
+  else if constexpr (R == 14) dftc<DIR, 7, 2>(v, tw, n);
+  else if constexpr (R == 16) dftc<DIR, 4, 4>(v, tw, n);
+  else if constexpr (R == 18) dftc<DIR, 9, 2>(v, tw, n);
+  else if constexpr (R == 20) dftc<DIR, 5, 4>(v, tw, n);
   else dftg<DIR, R>(v, tw, n);
 }
 
@@ -190,13 +304,14 @@ __device__ __forceinline__ void stage_ct(Cx<T>* __restrict__ s, const Cx<T>* __r
   stage_ct_io<T, N, Q, QS, DIR, R, NS, NT, false, false>(s, tw, tid, NoTileIO{}, NoTileIO{});
 }
 
-template <class T, int N, int Q, int QS, int DIR, int NT = kThreads, int S = 0, int NS = 1>
+template <class T, int N, int Q, int QS, int DIR, int NT = kThreads, int S = 0, int NS = 1,
+          int PK = 1>
 __device__ __forceinline__ void fft_ct(Cx<T>* __restrict__ s, const Cx<T>* __restrict__ tw,
                                        int tid) {
-  if constexpr (S < plan_nst(N)) {
-    constexpr int R = plan_radix(N, S);
+  if constexpr (S < plan_nst(N, PK)) {
+    constexpr int R = plan_radix(N, S, PK);
     stage_ct<T, N, Q, QS, DIR, R, NS, NT>(s, tw, tid);
-    fft_ct<T, N, Q, QS, DIR, NT, S + 1, NS * R>(s, tw, tid);
+    fft_ct<T, N, Q, QS, DIR, NT, S + 1, NS * R, PK>(s, tw, tid);
   }
 }
 
@@ -204,14 +319,14 @@ __device__ __forceinline__ void fft_ct(Cx<T>* __restrict__ s, const Cx<T>* __res
 // and/or its last stage writing through st (GOUT); the tile s carries the
 // stages in between.  The per-pencil arithmetic is that of fft_ct.
 template <class T, int N, int Q, int QS, int DIR, int NT, bool GIN, bool GOUT, int S = 0,
-          int NS = 1, class LD, class ST>
+          int NS = 1, int PK = 1, class LD, class ST>
 __device__ __forceinline__ void fft_ct_io(Cx<T>* __restrict__ s, const Cx<T>* __restrict__ tw,
                                           int tid, const LD& ld, const ST& st) {
-  if constexpr (S < plan_nst(N)) {
-    constexpr int R = plan_radix(N, S);
-    constexpr bool first = S == 0, last = S + 1 == plan_nst(N);
+  if constexpr (S < plan_nst(N, PK)) {
+    constexpr int R = plan_radix(N, S, PK);
+    constexpr bool first = S == 0, last = S + 1 == plan_nst(N, PK);
     stage_ct_io<T, N, Q, QS, DIR, R, NS, NT, GIN && first, GOUT && last>(s, tw, tid, ld, st);
-    fft_ct_io<T, N, Q, QS, DIR, NT, GIN, GOUT, S + 1, NS * R>(s, tw, tid, ld, st);
+    fft_ct_io<T, N, Q, QS, DIR, NT, GIN, GOUT, S + 1, NS * R, PK>(s, tw, tid, ld, st);
   }
 }
 
@@ -221,16 +336,16 @@ __device__ __forceinline__ void fft_ct_io(Cx<T>* __restrict__ s, const Cx<T>* __
 // plan_radix(N, 0) and span N / plan_radix(N, 0), is left to the caller: its
 // outputs m = t + r N/R are exactly the inputs of the first stage of the
 // forward-order transform, so a caller can fuse work between the two.
-template <class T, int N, int Q, int QS, int DIR, int NT, bool GIN, int S = 0, int NS = 1,
-          class LD, class ST>
+template <class T, int N, int Q, int QS, int DIR, int NT, bool GIN, int PK = 1, int S = 0,
+          int NS = 1, class LD, class ST>
 __device__ __forceinline__ void fft_ct_rev_head(Cx<T>* __restrict__ s,
                                                 const Cx<T>* __restrict__ tw, int tid,
                                                 const LD& ld, const ST& st) {
-  constexpr int nst = plan_nst(N);
+  constexpr int nst = plan_nst(N, PK);
   if constexpr (S < nst - 1) {
-    constexpr int R = plan_radix(N, nst - 1 - S);
+    constexpr int R = plan_radix(N, nst - 1 - S, PK);
     stage_ct_io<T, N, Q, QS, DIR, R, NS, NT, GIN && S == 0, false>(s, tw, tid, ld, st);
-    fft_ct_rev_head<T, N, Q, QS, DIR, NT, GIN, S + 1, NS * R>(s, tw, tid, ld, st);
+    fft_ct_rev_head<T, N, Q, QS, DIR, NT, GIN, PK, S + 1, NS * R>(s, tw, tid, ld, st);
   }
 }
 

@@ -146,10 +146,41 @@ __host__ __device__ constexpr RadixPlan make_plan(int n) {
   return p;
 }
 
-__host__ __device__ constexpr int plan_nst(int n) { return make_plan(n).nst; }
-__host__ __device__ constexpr int plan_radix(int n, int s) { return make_plan(n).r[s]; }
+// Plans of the compile-time transforms: lengths 16 m (m = 6, 8, 9, 10, 12, 14,
+// 16, 18, 20) run two stages, radix m (no twiddles) and radix 16, with
+// composite butterflies in registers: one LDS round trip per transform
+// instead of two or three.  Other lengths (and the runtime-plan path) use
+// make_plan.
+#ifndef CBS_PLAN16
+#define CBS_PLAN16 1
+#endif
+__host__ __device__ constexpr RadixPlan make_plan_ct(int n) {
+  const int m = n / 16;
+  if (CBS_PLAN16 && n % 16 == 0 &&
+      (m == 6 || m == 8 || m == 9 || m == 10 || m == 12 || m == 14 || m == 16 || m == 18 ||
+       m == 20)) {
+    RadixPlan p{2, {m, 16, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0}};
+    return p;
+  }
+  return make_plan(n);
+}
+// plan kind of a compile-time transform: 1 the two-stage plan where one exists,
+// 0 make_plan
+__host__ __device__ constexpr RadixPlan make_plan_k(int n, int pk) {
+  return pk ? make_plan_ct(n) : make_plan(n);
+}
+__host__ __device__ constexpr int plan_nst(int n, int pk = 1) { return make_plan_k(n, pk).nst; }
+__host__ __device__ constexpr int plan_radix(int n, int s, int pk = 1) {
+  return make_plan_k(n, pk).r[s];
+}
+// z tiles hold the first radix of all three components in registers in their
+// merged middle stage: the two-stage plan only while those 3 m complex values
+// stay few (m = n/16 <= 10, FP32 only; measured on MI355X)
+__host__ __device__ constexpr int zplan_kind(int n, int eb) {
+  return eb == 8 && make_plan_ct(n).r[0] <= 10 ? 1 : 0;
+}
 __host__ __device__ constexpr int plan_maxradix(int n) {
-  RadixPlan p = make_plan(n);
+  RadixPlan p = make_plan_ct(n);
   int m = 1;
   for (int s = 0; s < p.nst; ++s) m = p.r[s] > m ? p.r[s] : m;
   return m;

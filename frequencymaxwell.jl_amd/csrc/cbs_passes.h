@@ -308,12 +308,15 @@ __device__ __forceinline__ void ypass_tile(const DevProb<T>& pr, int tile, int t
   if constexpr (CT) {
     // first stage straight from HBM, last stage straight to HBM
     constexpr int Pc = ycols(CT ? NC : 8, eb, false);
+    // FP64: the three-stage plan (radix-16 butterflies cost FP64 y tiles occupancy)
+    constexpr int PK = eb == 8 ? 1 : 0;
     const long long rs = pr.ws[0];
     auto ld = [&](int q, int m) { return q < npx ? base[m * rs + q] : mk<T>(T(0), T(0)); };
     auto st = [&](int q, int m, const Cx<T>& v) {
       if (q < npx) base[m * rs + q] = v;
     };
-    fft_ct_io<T, NC, Pc, Pc, DIR, kThreads, true, true>(lds, lds_twiddles<T>(), tid, ld, st);
+    fft_ct_io<T, NC, Pc, Pc, DIR, kThreads, true, true, 0, 1, PK>(lds, lds_twiddles<T>(), tid, ld,
+                                                                   st);
   } else {
     tile_load(lds, base, P, npx, n, pr.ws[0], P, 0, tid);
     __syncthreads();
@@ -393,12 +396,12 @@ __device__ __forceinline__ void kspace_op(const DevProb<T>& pr, double kxd, doub
 // leaves element m = t + r N/R of each pencil in the thread that the inverse
 // stage reads it from, so one item (column p, index t) holds all 3 R values in
 // registers.  GIN / GOUT: the transform has a single stage, read / write HBM.
-template <class T, int N, int P, int NT, bool GIN, bool GOUT, class LD, class ST>
+template <class T, int N, int P, int NT, int PK, bool GIN, bool GOUT, class LD, class ST>
 __device__ __forceinline__ void zpass_mid(const DevProb<T>& pr, Cx<T>* __restrict__ s,
                                           const Cx<T>* __restrict__ tw,
                                           const double* __restrict__ kt, int npx, int tid,
                                           const LD& ld, const ST& st) {
-  constexpr int R = plan_radix(N, 0);
+  constexpr int R = plan_radix(N, 0, PK);
   constexpr int NB = N / R;
   constexpr int QS = 3 * P;
   constexpr int TOTAL = P * NB;
@@ -505,17 +508,19 @@ __device__ __forceinline__ void zpass_tile(const DevProb<T>& pr, int tile, int t
     if (p < npx) base[c * cs + m * zs + p] = v;
   };
   Cx<T>* res = lds;
+  constexpr int PK = zplan_kind(CT ? NC : 8, eb);
   if constexpr (CT && CBS_ZMERGE) {
-    constexpr int nst = plan_nst(CT ? NC : 8);
+    constexpr int nst = plan_nst(CT ? NC : 8, PK);
     const Cx<T>* tw = lds_twiddles<T>();
-    fft_ct_rev_head<T, NC, 3 * Pz, 3 * Pz, -1, NT, true>(lds, tw, tid, ld, st);
-    zpass_mid<T, NC, Pz, NT, nst == 1, nst == 1>(pr, lds, tw, kt, npx, tid, ld, st);
+    fft_ct_rev_head<T, NC, 3 * Pz, 3 * Pz, -1, NT, true, PK>(lds, tw, tid, ld, st);
+    zpass_mid<T, NC, Pz, NT, PK, nst == 1, nst == 1>(pr, lds, tw, kt, npx, tid, ld, st);
     if constexpr (nst > 1)
-      fft_ct_io<T, NC, 3 * Pz, 3 * Pz, 1, NT, false, true, 1, plan_radix(CT ? NC : 8, 0)>(
-          lds, tw, tid, ld, st);
+      fft_ct_io<T, NC, 3 * Pz, 3 * Pz, 1, NT, false, true, 1, plan_radix(CT ? NC : 8, 0, PK),
+                PK>(lds, tw, tid, ld, st);
     return;
   } else if constexpr (CT) {
-    fft_ct_io<T, NC, 3 * Pz, 3 * Pz, -1, NT, true, false>(lds, lds_twiddles<T>(), tid, ld, st);
+    fft_ct_io<T, NC, 3 * Pz, 3 * Pz, -1, NT, true, false, 0, 1, PK>(lds, lds_twiddles<T>(), tid,
+                                                                     ld, st);
   } else {
     __syncthreads();
     res = fft_rt<T, -1, NT>(lds, lds + n * Q, n, Q, Q, pr.plan[2], lds_twiddles<T>(), tid);
@@ -533,7 +538,8 @@ __device__ __forceinline__ void zpass_tile(const DevProb<T>& pr, int tile, int t
   }
   __syncthreads();
   if constexpr (CT) {
-    fft_ct_io<T, NC, 3 * Pz, 3 * Pz, 1, NT, false, true>(lds, lds_twiddles<T>(), tid, ld, st);
+    fft_ct_io<T, NC, 3 * Pz, 3 * Pz, 1, NT, false, true, 0, 1, PK>(lds, lds_twiddles<T>(), tid,
+                                                                    ld, st);
   } else {
     Cx<T>* other = res == lds ? lds + n * Q : lds;
     res = fft_rt<T, 1, NT>(res, other, n, Q, Q, pr.plan[2], lds_twiddles<T>(), tid);
