@@ -176,8 +176,11 @@ __host__ __device__ constexpr int plan_radix(int n, int s, int pk = 1) {
 // z tiles hold the first radix of all three components in registers in their
 // merged middle stage: the two-stage plan only while those 3 m complex values
 // stay few (m = n/16 <= 10, FP32 only; measured on MI355X)
+#ifndef CBS_ZPLAN_MAX
+#define CBS_ZPLAN_MAX 10
+#endif
 __host__ __device__ constexpr int zplan_kind(int n, int eb) {
-  return eb == 8 && make_plan_ct(n).r[0] <= 10 ? 1 : 0;
+  return eb == 8 && make_plan_ct(n).r[0] <= CBS_ZPLAN_MAX ? 1 : 0;
 }
 __host__ __device__ constexpr int plan_maxradix(int n) {
   RadixPlan p = make_plan_ct(n);
