@@ -18,8 +18,8 @@ constexpr int kMaxStages = 12;    // radix stages of one axis transform
 constexpr int kLdsBudget = 65536; // LDS bytes a pass tile may use (two blocks per CU)
 // x-pass tile shape switches (defaults are the measured best on MI355X):
 // CBS_XE: elements per thread that set the x-pass block size (0: kThreads);
-// CBS_XOPS_LATE: FP64 x inverse loads its epilogue operands after the
-// transforms; CBS_XJOINT: specialised x tiles transform the S pipelines of a
+// CBS_XOPS_LATE: the x inverse loads its epilogue operands after the
+// transforms (its spectra and ramps before them); CBS_XJOINT: specialised x tiles transform the S pipelines of a
 // line together (3 S P pencils) rather than one after the other
 #ifndef CBS_XE
 #define CBS_XE 0

@@ -658,10 +658,10 @@ __device__ __forceinline__ void xinv_tile(const DevProb<T>& pr, int tile, int ti
   if constexpr (CT) {
     constexpr int Pc = FUSE ? xlines_f(CT ? NC : 8, eb, S, false) : xlines(CT ? NC : 8, eb, S, false);
     constexpr int E = (Pc * NC + NT - 1) / NT;
-    // every global load of the tile first: both spectra, ramps and (LATE
-    // false) the epilogue operands; LATE loads the operands after the
-    // transforms, for tiles whose registers cannot hold them all
-    constexpr bool LATE = CBS_XOPS_LATE && eb == 16;
+    // the spectra and ramps of the tile are loaded before the transforms; the
+    // epilogue operands after them (LATE; with the two-stage transforms their
+    // registers would cost the tile occupancy) or, LATE false, before
+    constexpr bool LATE = CBS_XOPS_LATE;
     Cx<T> w[S][E][3];
     Cx<T> rp[E];
     Cx<T> vv[E], a0[E][3], a1[E][3];
