@@ -268,9 +268,9 @@ __host__ __device__ constexpr int width_z(int nz, int eb, bool rt) {
   return (!rt && tuned_width(2, nz, eb, 0)) ? tuned_width(2, nz, eb, 0) : (eb == 8 ? 16 : 8);
 #endif
 }
-// x-axis passes: 3 components * P lines per tile; specialised lengths run the
-// S pipelines one after the other through the tile, runtime plans hold all S.
-// pipelines transformed together in an x tile
+// x-axis passes: 3 components * S pipelines * P lines per tile (CBS_XJOINT 0:
+// specialised lengths run the S pipelines one after the other through a
+// 3 P-pencil tile); pipelines transformed together in an x tile:
 __host__ __device__ constexpr int xjoint(int S, bool rt) { return (rt || CBS_XJOINT) ? S : 1; }
 __host__ __device__ constexpr int xlines(int nx, int eb, int S, bool rt) {
   return tile_p(nx, eb, 3 * xjoint(S, rt), width_x(nx, eb, S, rt), rt, 1);

@@ -20,9 +20,10 @@ struct PassLaunch {
 
 constexpr int kMaxGroups = 8;
 
-// One five-sweep pipeline over a table of right-hand sides: each sweep is one
-// launch per group of equal axis length, so a mix of grids runs the same
-// specialised kernels (registers, LDS) as each grid alone.
+// One sweep pipeline (five kernels, or four when fused) over a table of
+// right-hand sides: each sweep is one launch per group of equal axis length, so
+// a mix of grids runs the same specialised kernels (registers, LDS) as each
+// grid alone.
 template <class T>
 struct PipeLaunch {
   int S;                            // pipelines per RHS (uniform over the table)

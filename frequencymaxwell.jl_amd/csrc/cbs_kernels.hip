@@ -1,8 +1,9 @@
 // Reductions, solver-state and elementwise kernels of the CBS engine, and the
-// launcher of one five-sweep pipeline (the sweep kernels themselves live in
+// launcher of one sweep pipeline (the sweep kernels themselves live in
 // cbs_pass_x.hip and cbs_pass_yz.hip):
 //   A  x forward + V x pre-op      B  y forward      C  z + k-space multiply + z inverse
 //   D  y inverse                   E  x inverse + fused epilogue with FP64 partial sums
+//   F  E followed by A of the next Born sweep (a fused pipeline is B C D F)
 // Every launch covers a table of right-hand sides (possibly of different grids).
 #include <algorithm>
 

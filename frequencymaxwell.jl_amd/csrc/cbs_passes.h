@@ -70,9 +70,9 @@ struct alignas(16) C2f {
 
 // ---------------------------------------------------------------------------
 // sweep A: x forward transform, pre-op u = V x (or x), shift ramps.
-// Specialised lengths run the S pipelines one after the other through one
-// 3P-pencil LDS tile (u and the ramp stay in registers); the runtime-plan path
-// keeps all 3 S P pencils in LDS.
+// Every path transforms the 3 S P pencils of its P lines (the S ramped copies
+// of u) together in one LDS tile; specialised lengths keep u and the ramp in
+// registers while the copies are written.
 // ---------------------------------------------------------------------------
 // Forward x transform of the S ramped pipelines of one specialised tile and
 // the store of their spectra into pr.w.  u holds the pre-op values (V x or x)
@@ -551,8 +551,9 @@ __device__ __forceinline__ void zpass_tile(const DevProb<T>& pr, int tile, int t
 
 // ---------------------------------------------------------------------------
 // sweep E: x inverse transform, inverse ramps, averaging, fused epilogue.
-// Specialised lengths transform the S pipelines one after the other through
-// one 3P-pencil LDS tile and average them in registers.
+// The S pipelines of a line are transformed together in one tile and averaged
+// in registers; the fused variant then runs sweep A of the next Born sweep on
+// the new iterate.
 // ---------------------------------------------------------------------------
 // The epilogue operands of one voxel g: vv = V (gamma post-ops), a0 = in0,
 // a1 = in1 (or out0 for the beta term of POST_OPERATOR); loaded before the
