@@ -188,6 +188,64 @@ def illuminations(fm, n_dir=64, seed=5):
     return out
 
 
+def large_phantom(fm, grid=(256, 256, 128), n_spheres=20, seed=3):
+    """20 seeded spheres in water: radii uniform in 4-16 px, centres uniform in
+    [r, g - r] per axis, n uniform in 1.36-1.60 (later spheres overwrite)."""
+    rng = np.random.default_rng(seed)
+    eps_bg = 1.33 ** 2
+    ph = np.full(grid, complex(eps_bg), dtype=np.complex128, order="F")
+    ax = [np.arange(1, g + 1, dtype=np.float64) for g in grid]
+    for _ in range(n_spheres):
+        r = float(rng.uniform(4.0, 16.0))
+        c = [float(rng.uniform(r, g - r)) for g in grid]
+        n = float(rng.uniform(1.36, 1.60))
+        lo = [max(0, int(math.floor(c[d] - r)) - 1) for d in range(3)]
+        hi = [min(grid[d], int(math.ceil(c[d] + r)) + 1) for d in range(3)]
+        d2 = ((ax[0][lo[0]:hi[0]] - c[0])[:, None, None] ** 2
+              + (ax[1][lo[1]:hi[1]] - c[1])[None, :, None] ** 2
+              + (ax[2][lo[2]:hi[2]] - c[2])[None, None, :] ** 2)
+        box = ph[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]]
+        box[d2 <= r * r] = n * n
+    return fm.Medium(ph, eps_bg)
+
+
+def large_leg(fm, args):
+    """The large heterogeneous phantom (256x256x128, 16 px absorbing padding every
+    axis -> 288x288x160, lambda 532 nm, x-polarised along +z): Born and BiCGStab
+    to 1e-6 in FP32+refine and in FP64, scene solves (potential, incident field,
+    source term, H, incident add and crop on the GPU)."""
+    from frequencymaxwell import engine
+    from frequencymaxwell import solver as S
+
+    print("[bench] large phantom ...", file=sys.stderr, flush=True)
+    grid = (256, 256, 128)
+    med = large_phantom(fm, grid)
+    sol = fm.ConvergentBornSolver(resolution=(50e-9,) * 3, grid_size=grid,
+                                  boundary_conditions=boundary(fm))
+    src = fm.PlaneWaveSource(wavelength=532e-9, polarization=[1, 0, 0], k_vector=[0, 0, 1])
+    d, eps, waves, keep = S.scene(sol, src, med)
+    out = {"workload": "256x256x128, 20 seeded spheres r 4-16 px n 1.36-1.60 in water, absorbing "
+                       "16 px every axis (padded 288x288x160), 532 nm, tol 1e-6 (FP64-verified)"}
+    for method in ("born", "bicgstab"):
+        for prec in ("fp32_refine", "fp64"):
+            o = fm.make_opts(method=method, precision=prec, tol=1e-6, maxiters=args.maxiters,
+                             check_every=args.check_every, use_graph=not args.no_graph)
+            t = time.perf_counter()
+            _, _, st, info = engine.solve_scene(d, eps, waves, o)
+            wall = time.perf_counter() - t
+            # FP32+refine reports the FP64 true residual; a plain FP64 solve's own
+            # residual is FP64
+            resid = st.resid_true_fp64 if prec == "fp32_refine" else st.resid
+            out[f"{method}_{prec}"] = {
+                "solve_ms": st.time_ms, "wall_ms": wall * 1e3, "iters": st.iters,
+                "fp64_iters": st.iters_fp64, "resid_fp64": resid,
+                "converged": bool(st.converged),
+                "voxel_iters_per_s": st.voxel_iters / (st.time_ms * 1e-3)}
+            print(f"[bench] large {method} {prec}: {st.time_ms:.0f} ms, {st.iters} iters, "
+                  f"resid {resid:.2e}", file=sys.stderr, flush=True)
+    return out
+
+
 def batched_leg(fm, args):
     """Many illuminations of one cell phantom in shared launches
     (cbs_solve_scene_multi, FP32+refine, tol 1e-5) against one illumination
@@ -322,6 +380,9 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-illumination leg")
     ap.add_argument("--batch-dirs", type=int, default=16, help="illumination directions (x 2 pol)")
     ap.add_argument("--batch-only", action="store_true")
+    ap.add_argument("--no-large", action="store_true",
+                    help="skip the large-phantom leg (256x256x128, 20 spheres)")
+    ap.add_argument("--large-only", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build measures the single-GPU path only "
@@ -336,6 +397,9 @@ def main():
         return
     if args.batch_only:
         print(json.dumps({"batched": batched_leg(fm, args)}), flush=True)
+        return
+    if args.large_only:
+        print(json.dumps({"large_phantom": large_leg(fm, args)}), flush=True)
         return
 
     med, sol, pot, eps_imag, prob, Sterm = build_workload(fm)
@@ -420,6 +484,8 @@ def main():
 
     if not args.no_batch:
         out["batched_illuminations"] = batched_leg(fm, args)
+    if not args.no_large:
+        out["large_phantom"] = large_leg(fm, args)
     if not args.no_serve:
         out["serving"] = serving_leg(fm, args)
 
