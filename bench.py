@@ -334,14 +334,21 @@ def read_traffic(summary: str, kernel: str):
 def serving_leg(fm, args):
     """The mixed-size request stream, continuously batched and one at a time."""
     print("[bench] serving stream ...", file=sys.stderr, flush=True)
-    cb, seq = None, None
+    cb, seq, cap = None, None, None
+    rate = args.serve_rate
+    if rate <= 0:
+        # capacity: the same requests back to back, one at a time; arrivals at 80 % of it
+        probe = serve_stream(fm, args.serve_requests, 1e6, 1, 7, args.maxiters)
+        cap = probe["requests_per_s"]
+        rate = 0.8 * cap
+        print(f"[bench] capacity {cap:.2f} requests/s -> arrivals at {rate:.2f}/s",
+              file=sys.stderr, flush=True)
     if args.serve_mode in ("both", "batched"):
-        cb = serve_stream(fm, args.serve_requests, args.serve_rate, args.serve_active, 7,
-                          args.maxiters)
+        cb = serve_stream(fm, args.serve_requests, rate, args.serve_active, 7, args.maxiters)
         print(f"[bench] continuous batching: p50 {cb['p50_ms']:.0f} ms p99 {cb['p99_ms']:.0f} ms",
               file=sys.stderr, flush=True)
     if args.serve_mode in ("both", "sequential"):
-        seq = serve_stream(fm, args.serve_requests, args.serve_rate, 1, 7, args.maxiters)
+        seq = serve_stream(fm, args.serve_requests, rate, 1, 7, args.maxiters)
         print(f"[bench] one at a time: p50 {seq['p50_ms']:.0f} ms p99 {seq['p99_ms']:.0f} ms",
               file=sys.stderr, flush=True)
     return {"workload": "Poisson arrivals of scene requests (permittivity + plane wave in, cropped "
@@ -349,6 +356,7 @@ def serving_leg(fm, args):
                         "crop on the GPU), grids " + ", ".join("x".join(map(str, g)) for g in SERVE_GRIDS)
                         + " (absorbing 16 px padding each side), 1-10 spheres n 1.36-1.60, "
                           "plane wave within NA 1.0, tol 1e-4 or 1e-6, Born FP32+refine",
+            "capacity_requests_per_s": cap, "arrival_rate_per_s": rate,
             "continuous_batching": cb, "one_at_a_time": seq}
 
 
@@ -373,7 +381,8 @@ def main():
     ap.add_argument("--profile-reps", type=int, default=20)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-stream leg")
     ap.add_argument("--serve-requests", type=int, default=16)
-    ap.add_argument("--serve-rate", type=float, default=0.75, help="arrivals per second")
+    ap.add_argument("--serve-rate", type=float, default=0.0,
+                    help="arrivals per second (0: 80 %% of the capacity measured first)")
     ap.add_argument("--serve-active", type=int, default=8)
     ap.add_argument("--serve-only", action="store_true")
     ap.add_argument("--serve-mode", default="both", choices=["both", "batched", "sequential"])
