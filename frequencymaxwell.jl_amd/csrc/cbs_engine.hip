@@ -429,6 +429,13 @@ void validate_desc(const cbs_problem_desc& d) {
   CBS_REQUIRE(std::isfinite(d.k2_re) && std::isfinite(d.k2_im), "k_square must be finite");
   CBS_REQUIRE(d.precision >= CBS_FP64 && d.precision <= CBS_FP32_REFINE, "unknown precision");
   CBS_REQUIRE(d.averaging == 0 || d.averaging == 1, "averaging must be 0 or 1");
+  // the y and z tiles address the three spectrum components of a pipeline
+  // with 32-bit element offsets
+  long long ws32[3], ws64[3];
+  w_strides((int)d.n[0], (int)d.n[1], (int)d.n[2], 8, ws32);
+  w_strides((int)d.n[0], (int)d.n[1], (int)d.n[2], 16, ws64);
+  CBS_REQUIRE(3 * std::max(ws32[2], ws64[2]) < (1LL << 32),
+              "grid too large: more than 2^32 spectrum elements per shift pipeline");
 }
 
 void problem_alloc(Problem& p, Ctx* c, const cbs_problem_desc& d, bool want32) {

@@ -310,10 +310,13 @@ __device__ __forceinline__ void ypass_tile(const DevProb<T>& pr, int tile, int t
     constexpr int Pc = ycols(CT ? NC : 8, eb, false);
     // FP64: the three-stage plan (radix-16 butterflies cost FP64 y tiles occupancy)
     constexpr int PK = eb == 8 ? 1 : 0;
-    const long long rs = pr.ws[0];
-    auto ld = [&](int q, int m) { return q < npx ? base[m * rs + q] : mk<T>(T(0), T(0)); };
+    // 32-bit element offsets from the tile base (within one component)
+    const unsigned rs = (unsigned)pr.ws[0];
+    auto ld = [&](int q, int m) {
+      return q < npx ? base[(unsigned)m * rs + (unsigned)q] : mk<T>(T(0), T(0));
+    };
     auto st = [&](int q, int m, const Cx<T>& v) {
-      if (q < npx) base[m * rs + q] = v;
+      if (q < npx) base[(unsigned)m * rs + (unsigned)q] = v;
     };
     fft_ct_io<T, NC, Pc, Pc, DIR, kThreads, true, true, 0, 1, PK>(lds, lds_twiddles<T>(), tid, ld,
                                                                    st);
@@ -333,7 +336,7 @@ __device__ __forceinline__ void kspace_op(const DevProb<T>& pr, double kxd, doub
                                           Cx<T>& e0, Cx<T>& e1, Cx<T>& e2) {
   if constexpr (sizeof(T) == 4) {
     // complex FP32: the Green tensor and the curl in single precision; the
-    // reciprocal of |k|^2 - k_c^2 by Smith's method (no squares of k^2)
+    // reciprocal of |k|^2 - k_c^2 without squares of k^2 (no overflow)
     const float kx = (float)kxd, ky = (float)kyd, kz = (float)kzd;
     const Cx<float> zs = mk<float>((float)pr.zs_re, (float)pr.zs_im);
     if (pr.zop == ZOP_GREEN) {
@@ -342,14 +345,12 @@ __device__ __forceinline__ void kspace_op(const DevProb<T>& pr, double kxd, doub
       const Cx<float> kdote = ik2c * (kx * e0 + ky * e1 + kz * e2);
       const float dre = kx * kx + ky * ky + kz * kz - (float)pr.k2c_re;
       const float dim = -(float)pr.k2c_im;
-      Cx<float> inv;
-      if (fabsf(dre) >= fabsf(dim)) {
-        const float r = dim / dre, q = 1.0f / (dre + dim * r);
-        inv = mk<float>(q, -r * q);
-      } else {
-        const float r = dre / dim, q = 1.0f / (dre * r + dim);
-        inv = mk<float>(r * q, -q);
-      }
+      // 1 / (dre + i dim) with both parts scaled by the larger magnitude first,
+      // branch-free, from two hardware reciprocals (1 ulp each; dim != 0)
+      const float rs = __builtin_amdgcn_rcpf(fmaxf(fabsf(dre), fabsf(dim)));
+      const float ar = dre * rs, ai = dim * rs;
+      const float t = __builtin_amdgcn_rcpf(ar * ar + ai * ai) * rs;
+      const Cx<float> inv = mk<float>(ar * t, -ai * t);
       const Cx<float> a = zs * inv;
       const Cx<float> f0 = e0 - kx * kdote, f1 = e1 - ky * kdote, f2 = e2 - kz * kdote;
       e0 = a * f0;
@@ -498,14 +499,16 @@ __device__ __forceinline__ void zpass_tile(const DevProb<T>& pr, int tile, int t
   // specialised lengths: the forward transform's first stage reads HBM and the
   // inverse transform's last stage writes it (pencil q = component c, column p)
   constexpr int Pz = CT ? zcols(CT ? NC : 8, eb, false) : 1;
-  const long long cs = pr.ws[2], zs = pr.ws[1];
+  // 32-bit element offsets from the tile base (validate_desc bounds the three
+  // components of a pipeline below 2^32 elements)
+  const unsigned cs = (unsigned)pr.ws[2], zs = (unsigned)pr.ws[1];
   auto ld = [&](int q, int m) {
-    const int c = q / Pz, p = q % Pz;
-    return p < npx ? base[c * cs + m * zs + p] : mk<T>(T(0), T(0));
+    const unsigned c = (unsigned)q / Pz, p = (unsigned)q % Pz;
+    return (int)p < npx ? base[c * cs + (unsigned)m * zs + p] : mk<T>(T(0), T(0));
   };
   auto st = [&](int q, int m, const Cx<T>& v) {
-    const int c = q / Pz, p = q % Pz;
-    if (p < npx) base[c * cs + m * zs + p] = v;
+    const unsigned c = (unsigned)q / Pz, p = (unsigned)q % Pz;
+    if ((int)p < npx) base[c * cs + (unsigned)m * zs + p] = v;
   };
   Cx<T>* res = lds;
   constexpr int PK = zplan_kind(CT ? NC : 8, eb);
