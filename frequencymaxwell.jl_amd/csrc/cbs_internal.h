@@ -101,6 +101,33 @@ __host__ __device__ __forceinline__ Cx<double> cdiv(Cx<double> a, Cx<double> b) 
   return mk<double>((a.re * b.re + a.im * b.im) / d, (a.im * b.re - a.re * b.im) / d);
 }
 
+// Accesses of device buffers through global-address-space pointers.  A
+// pointer read from a descriptor is generic, and generic (flat) accesses count
+// against the LDS wait counter too: every LDS wait of a tile would then wait
+// for the tile's outstanding HBM loads and stores.  Every pointer passed here
+// points into device memory (or host memory mapped for the device).
+template <class T>
+struct GVec2 {
+  typedef T type __attribute__((ext_vector_type(2)));
+};
+template <class T>
+__device__ __forceinline__ Cx<T> gld(const Cx<T>* p) {
+  typedef typename GVec2<T>::type V;
+  const V v = *(const __attribute__((address_space(1))) V*)(p);
+  return mk<T>(v.x, v.y);
+}
+template <class T>
+__device__ __forceinline__ void gst(Cx<T>* p, const Cx<T>& c) {
+  typedef typename GVec2<T>::type V;
+  V v;
+  v.x = c.re;
+  v.y = c.im;
+  *(__attribute__((address_space(1))) V*)(p) = v;
+}
+__device__ __forceinline__ double gld(const double* p) {
+  return *(const __attribute__((address_space(1))) double*)(p);
+}
+
 // ---------------------------------------------------------------------------
 // radix plans: n = r0 * r1 * ... ; powers of two use radix 8 with one or two
 // radix-4 stages, odd factors 3, 5, 7 and any remaining prime as a generic

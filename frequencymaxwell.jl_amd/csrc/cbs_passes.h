@@ -20,7 +20,7 @@ extern __shared__ __align__(16) unsigned char g_smem[];
 template <class T, int NT = kThreads>
 __device__ __forceinline__ Cx<T>* stage_twiddles(const Cx<T>* __restrict__ tw, int n, int tid) {
   Cx<T>* t = reinterpret_cast<Cx<T>*>(g_smem);
-  for (int i = tid; i < n; i += NT) t[i] = tw[i];
+  for (int i = tid; i < n; i += NT) t[i] = gld(tw + i);
   return reinterpret_cast<Cx<T>*>(g_smem + tw_bytes(n, (int)sizeof(Cx<T>)));
 }
 template <class T>
@@ -44,6 +44,8 @@ __device__ __forceinline__ int find_prob(const int* __restrict__ bstart, int npr
 template <class T>
 __device__ __forceinline__ Cx<T> ramp_at(const DevProb<T>& pr, int x, int y, int z) {
 #pragma clang fp contract(off)
+  // generic loads: through global-address-space pointers the x inverse
+  // measured 5-7 % slower on MI355X
   const Cx<T> a = pr.ramp[0][x], b = pr.ramp[1][y], c = pr.ramp[2][z];
   const Cx<T> ab = mk<T>(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
   return mk<T>(ab.re * c.re - ab.im * c.im, ab.re * c.im + ab.im * c.re);
@@ -115,7 +117,7 @@ __device__ __forceinline__ void xfwd_ct_store(const DevProb<T>& pr, Cx<T>* __res
         for (int j = 0; j < J; ++j) {
 #pragma unroll
           for (int c = 0; c < 3; ++c)
-            pr.w[((s0 + j) * 3 + c) * pr.ws[2] + wo] = lds[m * QSc + (3 * j + c) * Pc + p];
+            gst(pr.w + ((s0 + j) * 3 + c) * pr.ws[2] + wo, lds[m * QSc + (3 * j + c) * Pc + p]);
         }
       }
     }
@@ -152,7 +154,7 @@ __device__ __forceinline__ void xfwd_rt_store(const DevProb<T>& pr, Cx<T>* a, Cx
       const int l = l0 + p;
       const long long wo = (long long)(l / ny) * pr.ws[1] + (long long)(l % ny) * pr.ws[0] + m;
 #pragma unroll
-      for (int q = 0; q < 3 * S; ++q) pr.w[q * pr.ws[2] + wo] = res[m * QS + q * P + p];
+      for (int q = 0; q < 3 * S; ++q) gst(pr.w + q * pr.ws[2] + wo, res[m * QS + q * P + p]);
     }
   }
 }
@@ -183,10 +185,10 @@ __device__ __forceinline__ void xfwd_tile(const DevProb<T>& pr, int tile, int ti
       rp[k] = mk<T>(T(1), T(0));
       if (e < Pc * NC && p < np) {
         const long long g = (long long)(l0 + p) * NC + m;
-        const Cx<T> vv = useV ? pr.V[g] : mk<T>(T(1), T(0));
+        const Cx<T> vv = useV ? gld(pr.V + g) : mk<T>(T(1), T(0));
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-          const Cx<T> xv = pr.in0[c * nv + g];
+          const Cx<T> xv = gld(pr.in0 + c * nv + g);
           u[k][c] = useV ? cmul_nc(vv, xv) : xv;
         }
         if (pr.has_ramp) {
@@ -207,10 +209,10 @@ __device__ __forceinline__ void xfwd_tile(const DevProb<T>& pr, int tile, int ti
       Cx<T> rp = mk<T>(T(1), T(0));
       if (p < np) {
         const long long g = (long long)(l0 + p) * n + m;
-        const Cx<T> vv = useV ? pr.V[g] : mk<T>(T(1), T(0));
+        const Cx<T> vv = useV ? gld(pr.V + g) : mk<T>(T(1), T(0));
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-          const Cx<T> xv = pr.in0[c * nv + g];
+          const Cx<T> xv = gld(pr.in0 + c * nv + g);
           u[c] = useV ? cmul_nc(vv, xv) : xv;
         }
         if (pr.has_ramp) {
@@ -313,10 +315,10 @@ __device__ __forceinline__ void ypass_tile(const DevProb<T>& pr, int tile, int t
     // 32-bit element offsets from the tile base (within one component)
     const unsigned rs = (unsigned)pr.ws[0];
     auto ld = [&](int q, int m) {
-      return q < npx ? base[(unsigned)m * rs + (unsigned)q] : mk<T>(T(0), T(0));
+      return q < npx ? gld(base + ((unsigned)m * rs + (unsigned)q)) : mk<T>(T(0), T(0));
     };
     auto st = [&](int q, int m, const Cx<T>& v) {
-      if (q < npx) base[(unsigned)m * rs + (unsigned)q] = v;
+      if (q < npx) gst(base + ((unsigned)m * rs + (unsigned)q), v);
     };
     fft_ct_io<T, NC, Pc, Pc, DIR, kThreads, true, true, 0, 1, PK>(lds, lds_twiddles<T>(), tid, ld,
                                                                    st);
@@ -490,9 +492,9 @@ __device__ __forceinline__ void zpass_tile(const DevProb<T>& pr, int tile, int t
     const double* ty = green ? pr.kk[1][s] : pr.kc[1];
     for (int i = tid; i < zk_count(n, P); i += NT) {
       double v = 0.0;
-      if (i < n) v = tz[i];
-      else if (i < n + P) v = (i - n < npx) ? tx[x0 + i - n] : 0.0;
-      else v = ty[y];
+      if (i < n) v = gld(tz + i);
+      else if (i < n + P) v = (i - n < npx) ? gld(tx + x0 + i - n) : 0.0;
+      else v = gld(ty + y);
       kt[i] = v;
     }
   }
@@ -504,11 +506,11 @@ __device__ __forceinline__ void zpass_tile(const DevProb<T>& pr, int tile, int t
   const unsigned cs = (unsigned)pr.ws[2], zs = (unsigned)pr.ws[1];
   auto ld = [&](int q, int m) {
     const unsigned c = (unsigned)q / Pz, p = (unsigned)q % Pz;
-    return (int)p < npx ? base[c * cs + (unsigned)m * zs + p] : mk<T>(T(0), T(0));
+    return (int)p < npx ? gld(base + (c * cs + (unsigned)m * zs + p)) : mk<T>(T(0), T(0));
   };
   auto st = [&](int q, int m, const Cx<T>& v) {
     const unsigned c = (unsigned)q / Pz, p = (unsigned)q % Pz;
-    if ((int)p < npx) base[c * cs + (unsigned)m * zs + p] = v;
+    if ((int)p < npx) gst(base + (c * cs + (unsigned)m * zs + p), v);
   };
   Cx<T>* res = lds;
   constexpr int PK = zplan_kind(CT ? NC : 8, eb);
@@ -571,11 +573,11 @@ __device__ __forceinline__ void xinv_operands(const DevProb<T>& pr, long long g,
   const bool need1 = valid && (post == POST_BORN || post == POST_RESID || use_b ||
                                (post == POST_PMATVEC && pr.in1 != nullptr));
   const Cx<T>* src1 = post == POST_OPERATOR ? pr.out0 : pr.in1;
-  vv = valid && post >= POST_BORN ? pr.V[g] : mk<T>(T(0), T(0));
+  vv = valid && post >= POST_BORN ? gld(pr.V + g) : mk<T>(T(0), T(0));
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    a0[c] = need0 ? pr.in0[c * nv + g] : mk<T>(T(0), T(0));
-    a1[c] = need1 ? src1[c * nv + g] : mk<T>(T(0), T(0));
+    a0[c] = need0 ? gld(pr.in0 + c * nv + g) : mk<T>(T(0), T(0));
+    a1[c] = need1 ? gld(src1 + c * nv + g) : mk<T>(T(0), T(0));
   }
 }
 
@@ -593,7 +595,7 @@ __device__ __forceinline__ void xinv_finish(const DevProb<T>& pr, long long g, C
   const int post = pr.post;
   if (post == POST_STORE) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) pr.out0[c * nv + g] = w[c];
+    for (int c = 0; c < 3; ++c) gst(pr.out0 + c * nv + g, w[c]);
   } else if (post == POST_OPERATOR) {
     const Cx<T> a = mk<T>((T)pr.a_re, (T)pr.a_im), b = mk<T>((T)pr.b_re, (T)pr.b_im);
     const bool use_b = pr.b_re != 0.0 || pr.b_im != 0.0;
@@ -604,7 +606,7 @@ __device__ __forceinline__ void xinv_finish(const DevProb<T>& pr, long long g, C
         const Cx<T> by = cmul_nc(b, a1[c]);
         y = mk<T>(y.re + by.re, y.im + by.im);
       }
-      pr.out0[c * nv + g] = y;
+      gst(pr.out0 + c * nv + g, y);
     }
   } else {
     // gamma = i V / eps
@@ -616,14 +618,14 @@ __device__ __forceinline__ void xinv_finish(const DevProb<T>& pr, long long g, C
         const Cx<T> t = mk<T>(a1[c].re - a0[c].re + w[c].re, a1[c].im - a0[c].im + w[c].im);
         const Cx<T> r = cmul_nc(gam, t);
         w[c] = mk<T>(a0[c].re + r.re, a0[c].im + r.im);
-        pr.out0[c * nv + g] = w[c];
+        gst(pr.out0 + c * nv + g, w[c]);
         acc[0] += (double)(r.re * r.re + r.im * r.im);
       }
     } else if (post == POST_RESID) {
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         const Cx<T> r = mk<T>(a1[c].re - a0[c].re + w[c].re, a1[c].im - a0[c].im + w[c].im);
-        pr.out0[c * nv + g] = r;
+        gst(pr.out0 + c * nv + g, r);
         const Cx<T> gr = cmul_nc(gam, r);
         acc[0] += (double)(gr.re * gr.re + gr.im * gr.im);
       }
@@ -631,7 +633,7 @@ __device__ __forceinline__ void xinv_finish(const DevProb<T>& pr, long long g, C
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         const Cx<T> o = cmul_nc(gam, mk<T>(a0[c].re - w[c].re, a0[c].im - w[c].im));
-        pr.out0[c * nv + g] = o;
+        gst(pr.out0 + c * nv + g, o);
         if (pr.in1) {
           // <in1, o> = conj(in1) o
           acc[0] += (double)(a1[c].re * o.re + a1[c].im * o.im);
@@ -680,7 +682,7 @@ __device__ __forceinline__ void xinv_tile(const DevProb<T>& pr, int tile, int ti
       for (int s = 0; s < S; ++s) {
 #pragma unroll
         for (int c = 0; c < 3; ++c)
-          w[s][k][c] = valid ? pr.w[(s * 3 + c) * pr.ws[2] + wo] : mk<T>(T(0), T(0));
+          w[s][k][c] = valid ? gld(pr.w + (s * 3 + c) * pr.ws[2] + wo) : mk<T>(T(0), T(0));
       }
       rp[k] = pr.has_ramp && valid ? ramp_at(pr, m, l % ny, l / ny) : mk<T>(T(1), T(0));
       if constexpr (!LATE) xinv_operands(pr, (long long)l * NC + m, valid, vv[k], a0[k], a1[k]);
@@ -762,7 +764,7 @@ __device__ __forceinline__ void xinv_tile(const DevProb<T>& pr, int tile, int ti
       const long long wo = (long long)(l / ny) * pr.ws[1] + (long long)(l % ny) * pr.ws[0] + m;
 #pragma unroll
       for (int q = 0; q < 3 * S; ++q)
-        lds[m * QS + q * P + p] = p < np ? pr.w[q * pr.ws[2] + wo] : mk<T>(T(0), T(0));
+        lds[m * QS + q * P + p] = p < np ? gld(pr.w + q * pr.ws[2] + wo) : mk<T>(T(0), T(0));
     }
     __syncthreads();
     Cx<T>* res = fft_rt<T, 1>(lds, lds + n * QS, n, Q, QS, pr.plan[0], lds_twiddles<T>(), tid);
