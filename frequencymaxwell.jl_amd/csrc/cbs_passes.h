@@ -402,7 +402,7 @@ __device__ __forceinline__ void kspace_op(const DevProb<T>& pr, double kxd, doub
 template <class T, int N, int P, int NT, int PK, bool GIN, bool GOUT, class LD, class ST>
 __device__ __forceinline__ void zpass_mid(const DevProb<T>& pr, Cx<T>* __restrict__ s,
                                           const Cx<T>* __restrict__ tw,
-                                          const double* __restrict__ kt, int npx, int tid,
+                                          const T* __restrict__ kt, int npx, int tid,
                                           const LD& ld, const ST& st) {
   constexpr int R = plan_radix(N, 0, PK);
   constexpr int NB = N / R;
@@ -442,7 +442,7 @@ __device__ __forceinline__ void zpass_mid(const DevProb<T>& pr, Cx<T>* __restric
         dft<-1, R>(v[k][c], tw, N);
       }
       if (p < npx) {
-        const double kx = kt[N + p], ky = kt[N + P];
+        const T kx = kt[N + p], ky = kt[N + P];
 #pragma unroll
         for (int r = 0; r < R; ++r)
           kspace_op(pr, kx, ky, kt[t + r * NB], v[k][0][r], v[k][1][r], v[k][2][r]);
@@ -483,8 +483,9 @@ __device__ __forceinline__ void zpass_tile(const DevProb<T>& pr, int tile, int t
     for (int c = 0; c < 3; ++c)
       tile_load<T, NT>(lds, base + c * pr.ws[2], P, npx, n, pr.ws[1], Q, c * P, tid);
   }
-  // k values of this tile: kt[m] along z, kt[n + p] along x, kt[n + P] at y
-  double* kt = reinterpret_cast<double*>(reinterpret_cast<unsigned char*>(lds) + zk_offset(n, P, eb, !CT));
+  // k values of this tile: kt[m] along z, kt[n + p] along x, kt[n + P] at y,
+  // rounded to the tile's precision (complex-FP32 tiles compute in FP32)
+  T* kt = reinterpret_cast<T*>(reinterpret_cast<unsigned char*>(lds) + zk_offset(n, P, eb, !CT));
   {
     const bool green = pr.zop == ZOP_GREEN;
     const double* tz = green ? pr.kk[2][s] : pr.kc[2];
@@ -495,7 +496,7 @@ __device__ __forceinline__ void zpass_tile(const DevProb<T>& pr, int tile, int t
       if (i < n) v = gld(tz + i);
       else if (i < n + P) v = (i - n < npx) ? gld(tx + x0 + i - n) : 0.0;
       else v = gld(ty + y);
-      kt[i] = v;
+      kt[i] = (T)v;
     }
   }
   // specialised lengths: the forward transform's first stage reads HBM and the
