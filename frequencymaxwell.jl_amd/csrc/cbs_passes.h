@@ -808,7 +808,9 @@ __device__ __forceinline__ void xinv_tile(const DevProb<T>& pr, int tile, int ti
   }
   if (pr.post >= POST_BORN) {
     double* scratch = reinterpret_cast<double*>(g_smem);
-    block_reduce_store<NT>(acc, pr.part + (long long)tile * kNumPart, scratch, tid);
+    // POST_BORN and POST_RESID accumulate only part 0
+    block_reduce_store<NT>(acc, pr.part + (long long)tile * kNumPart, scratch, tid,
+                           pr.post == POST_PMATVEC ? kNumPart : 1);
   }
 }
 }  // namespace cbs

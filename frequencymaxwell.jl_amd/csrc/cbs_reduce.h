@@ -8,29 +8,36 @@
 namespace cbs {
 
 // Reduce kNumPart doubles over a block of NT threads; thread 0 writes them to
-// dst.  `scratch` holds (NT/64)*kNumPart doubles of LDS.
+// dst.  `scratch` holds (NT/64)*kNumPart doubles of LDS.  Only the first nred
+// parts are reduced (block-uniform); the others are known to be zero in every
+// thread and are written as 0.0, which their sums would be.
 template <int NT = kThreads>
 __device__ __forceinline__ void block_reduce_store(double (&acc)[kNumPart], double* dst,
-                                                   double* scratch, int tid) {
+                                                   double* scratch, int tid,
+                                                   int nred = kNumPart) {
 #pragma unroll
   for (int k = 0; k < kNumPart; ++k) {
-    double v = acc[k];
+    if (k < nred) {
+      double v = acc[k];
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-    acc[k] = v;
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+      acc[k] = v;
+    }
   }
   const int lane = tid & 63, wave = tid >> 6;
   __syncthreads();  // the caller's LDS tile may still be read by other waves
   if (lane == 0) {
 #pragma unroll
-    for (int k = 0; k < kNumPart; ++k) scratch[wave * kNumPart + k] = acc[k];
+    for (int k = 0; k < kNumPart; ++k)
+      if (k < nred) scratch[wave * kNumPart + k] = acc[k];
   }
   __syncthreads();
   if (tid == 0) {
 #pragma unroll
     for (int k = 0; k < kNumPart; ++k) {
       double s = 0.0;
-      for (int w = 0; w < NT / 64; ++w) s += scratch[w * kNumPart + k];
+      if (k < nred)
+        for (int w = 0; w < NT / 64; ++w) s += scratch[w * kNumPart + k];
       dst[k] = s;
     }
   }

@@ -131,3 +131,25 @@ def test_green_every_length(fm, oracle, L, avg):
         for prec, tol in (("fp64", FP64_TOL), ("fp32", FP32_TOL)):
             prob = fm.CBSProblem(V, spec, precision=prec)
             assert rel(prob.green_apply(u), want) <= tol, (shape, prec)
+
+
+def test_grid_beyond_32bit_tile_offsets_rejected(fm):
+    """The y and z tiles address the three spectrum components of a shift
+    pipeline with 32-bit element offsets: a grid whose pipeline holds 2^32
+    elements or more is refused before anything is read or allocated."""
+    import ctypes as C
+
+    from frequencymaxwell import _lib as L
+    from frequencymaxwell import engine
+
+    print("case grid 4096x4096x100 rejected", flush=True)
+    k2 = (2 * np.pi * 1.33 / 500e-9) ** 2
+    spec = engine.GreenSpec(shape=(4096, 4096, 100), resolution=tuple(RES), shifts=(0.25,) * 3,
+                            use_averaging=True, k_square=k2 + 0.1j * k2, eps_imag=0.1 * k2)
+    d = engine.make_desc(spec, "fp32")
+    V = np.zeros(4, dtype=np.complex128)  # never read: validation fails first
+    h = C.c_void_p()
+    lib = L.load()
+    with pytest.raises(ValueError, match="2\\^32"):
+        L.check(lib.cbs_problem_create(L.default_context().handle, C.byref(d), L.dptr(V), C.byref(h)))
+    assert not h.value
