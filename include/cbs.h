@@ -128,7 +128,9 @@ void cbs_default_opts(cbs_solve_opts *opts);
 /* ---- problem ------------------------------------------------------------ */
 /* Upload the padded, attenuated potential V (complex, Nx*Ny*Nz) -- the
  * `potential_device` of src/solvers/convergent_born.jl:650 -- and build the
- * Green operator of src/core/dyadic_green.jl:67-92. */
+ * Green operator of src/core/dyadic_green.jl:67-92.  Limits (CBS_EINVAL):
+ * every axis 1..4096, and fewer than 2^32 spectrum elements per shift
+ * pipeline (3 components of the padded-stride grid, about 1.4e9 voxels). */
 cbs_status cbs_problem_create(cbs_ctx *ctx, const cbs_problem_desc *desc, const double *V,
                               cbs_problem **out);
 cbs_status cbs_problem_destroy(cbs_problem *p);
