@@ -56,39 +56,41 @@ __global__ void __launch_bounds__(kThreads) k_bicg_vec(const BiVec<T>* __restric
   const Cx<T> beta = mk<T>((T)st->beta_re, (T)st->beta_im);
   for (long long i = start + threadIdx.x; i < end; i += kThreads) {
     Cx<T> gam = mk<T>(T(0), T(0));
-    if (OP == BI_INIT) gam = mul_i((T)q.eps_inv * q.V[i]);
+    if (OP == BI_INIT) gam = mul_i((T)q.eps_inv * gld(q.V + i));
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const long long j = c * nv + i;
       if (OP == BI_INIT) {
-        Cx<T> r = gam * q.b[j];
-        if (q.use_v) r = r - q.v[j];
-        q.r[j] = r;
-        q.rh[j] = r;
-        q.p[j] = mk<T>(T(0), T(0));
-        q.v[j] = mk<T>(T(0), T(0));
+        Cx<T> r = gam * gld(q.b + j);
+        if (q.use_v) r = r - gld(q.v + j);
+        gst(q.r + j, r);
+        gst(q.rh + j, r);
+        gst(q.p + j, mk<T>(T(0), T(0)));
+        gst(q.v + j, mk<T>(T(0), T(0)));
         acc[0] += (double)abs2(r);
       } else if (OP == BI_P) {
-        q.p[j] = q.r[j] + beta * (q.p[j] - omega * q.v[j]);
+        gst(q.p + j, gld(q.r + j) + beta * (gld(q.p + j) - omega * gld(q.v + j)));
       } else if (OP == BI_S) {
-        const Cx<T> s = q.r[j] - alpha * q.v[j];
-        q.s[j] = s;
-        q.x[j] = q.x[j] + alpha * q.p[j];
+        const Cx<T> s = gld(q.r + j) - alpha * gld(q.v + j);
+        gst(q.s + j, s);
+        gst(q.x + j, gld(q.x + j) + alpha * gld(q.p + j));
         acc[0] += (double)abs2(s);
       } else {  // BI_XR
-        const Cx<T> s = q.s[j];
-        const Cx<T> r = s - omega * q.t[j];
-        q.x[j] = q.x[j] + omega * s;
-        q.r[j] = r;
+        const Cx<T> s = gld(q.s + j);
+        const Cx<T> r = s - omega * gld(q.t + j);
+        gst(q.x + j, gld(q.x + j) + omega * s);
+        gst(q.r + j, r);
         acc[0] += (double)abs2(r);
-        const Cx<T> d = cdot(q.rh[j], r);
+        const Cx<T> d = cdot(gld(q.rh + j), r);
         acc[1] += (double)d.re;
         acc[2] += (double)d.im;
       }
     }
   }
+  // only the parts each update accumulates are reduced
   if (OP != BI_P)
-    block_reduce_store(acc, q.part + (long long)blockIdx.x * kNumPart, scratch, threadIdx.x);
+    block_reduce_store(acc, q.part + (long long)blockIdx.x * kNumPart, scratch, threadIdx.x,
+                       OP == BI_XR ? 3 : 1);
 }
 
 enum { BF_INIT = 0, BF_ALPHA = 1, BF_S = 2, BF_OMEGA = 3, BF_R = 4 };
