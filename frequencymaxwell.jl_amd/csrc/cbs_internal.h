@@ -364,15 +364,17 @@ struct RState {
   double tol, abstol;
   int rule;        // CBS_STOP_*
   int iters;       // operator applications
-  int maxit;
+  int kit;         // iterations as the method counts them: Born sweeps, BiCGStab
+                   // iterations (two applications each), GMRES Arnoldi steps
+  int maxit;       // bound on kit
   int done;        // 0 running, 1 converged, 2 stopped at maxit, 3 numeric failure
+                   // (non-finite residual), 4 breakdown
   int ans;         // Born: index of the buffer holding the answer
   int nhist;       // Krylov: entries written to hist
   int cyc;         // GMRES: the current Arnoldi cycle has stopped (skip its remaining steps)
   int pend;        // GMRES: outcome to apply after the cycle's x update (0 restart, 1 conv, 2 maxit)
   // BiCGStab scalars
   double rho_re, rho_im, alpha_re, alpha_im, omega_re, omega_im, beta_re, beta_im;
-  double s2;         // ||s||^2 of the current BiCGStab half step
   double hist[512];  // residual history (relative), first 512 entries
 };
 

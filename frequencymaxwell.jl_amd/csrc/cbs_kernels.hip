@@ -74,15 +74,15 @@ __global__ void k_state_init(RState* st, const double* __restrict__ sums, double
   st->abstol = abstol;
   st->rule = rule;
   st->iters = 0;
+  st->kit = 0;
   st->maxit = maxit;
   st->ans = 0;
   st->done = bn2 > 0.0 ? 0 : 1;
-  if (!(bn2 == bn2)) st->done = 3;
+  if (!(bn2 <= 1e300)) st->done = 3;  // NaN, Inf or overflow
   st->thr2 = rule == 0 ? tol * tol * bn2 : -1.0;
   st->rho_re = st->alpha_re = st->omega_re = 1.0;
   st->rho_im = st->alpha_im = st->omega_im = 0.0;
   st->beta_re = st->beta_im = 0.0;
-  st->s2 = 0.0;
   st->nhist = 0;
   st->cyc = 0;
   st->pend = 0;

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <limits>
 
 #include "cbs_reduce.h"
 #include "cbs_solve.h"
@@ -16,21 +17,23 @@
 namespace cbs {
 
 constexpr long long kChunk = (long long)kThreads * 16;  // voxels per vector-kernel block
+constexpr int kBicgVectors = 5;  // r, p, v, s, t
 
 // ---------------------------------------------------------------------------
-// BiCGStab (van der Vorst 1992), shadow residual r^ = r0
+// BiCGStab as Krylov.jl's bicgstab runs it for LinearSolve's KrylovJL_BICGSTAB
+// (van der Vorst 1992; Krylov.jl's default shadow vector c = b, the
+// unpreconditioned right-hand side; no test on the half step s).
 // ---------------------------------------------------------------------------
 template <class T>
 struct BiVec {
   Cx<T>* x;
   Cx<T>* r;
-  Cx<T>* rh;
   Cx<T>* p;
   Cx<T>* v;
   Cx<T>* s;
   Cx<T>* t;
   const Cx<T>* V;
-  const Cx<T>* b;
+  const Cx<T>* b;  // right-hand side, also the shadow vector c
   long long nv;
   double eps_inv;
   double* part;
@@ -61,46 +64,52 @@ __global__ void __launch_bounds__(kThreads) k_bicg_vec(const BiVec<T>* __restric
     for (int c = 0; c < 3; ++c) {
       const long long j = c * nv + i;
       if (OP == BI_INIT) {
-        Cx<T> r = gam * gld(q.b + j);
+        const Cx<T> bj = gld(q.b + j);
+        Cx<T> r = gam * bj;
         if (q.use_v) r = r - gld(q.v + j);
         gst(q.r + j, r);
-        gst(q.rh + j, r);
         gst(q.p + j, mk<T>(T(0), T(0)));
         gst(q.v + j, mk<T>(T(0), T(0)));
         acc[0] += (double)abs2(r);
+        const Cx<T> d = cdot(bj, r);
+        acc[1] += (double)d.re;
+        acc[2] += (double)d.im;
       } else if (OP == BI_P) {
         gst(q.p + j, gld(q.r + j) + beta * (gld(q.p + j) - omega * gld(q.v + j)));
       } else if (OP == BI_S) {
-        const Cx<T> s = gld(q.r + j) - alpha * gld(q.v + j);
-        gst(q.s + j, s);
+        gst(q.s + j, gld(q.r + j) - alpha * gld(q.v + j));
         gst(q.x + j, gld(q.x + j) + alpha * gld(q.p + j));
-        acc[0] += (double)abs2(s);
       } else {  // BI_XR
         const Cx<T> s = gld(q.s + j);
         const Cx<T> r = s - omega * gld(q.t + j);
         gst(q.x + j, gld(q.x + j) + omega * s);
         gst(q.r + j, r);
         acc[0] += (double)abs2(r);
-        const Cx<T> d = cdot(gld(q.rh + j), r);
+        const Cx<T> d = cdot(gld(q.b + j), r);
         acc[1] += (double)d.re;
         acc[2] += (double)d.im;
       }
     }
   }
   // only the parts each update accumulates are reduced
-  if (OP != BI_P)
-    block_reduce_store(acc, q.part + (long long)blockIdx.x * kNumPart, scratch, threadIdx.x,
-                       OP == BI_XR ? 3 : 1);
+  if (OP == BI_INIT || OP == BI_XR)
+    block_reduce_store(acc, q.part + (long long)blockIdx.x * kNumPart, scratch, threadIdx.x, 3);
 }
 
-enum { BF_INIT = 0, BF_ALPHA = 1, BF_S = 2, BF_OMEGA = 3, BF_R = 4 };
+enum { BF_INIT = 0, BF_ALPHA = 1, BF_OMEGA = 3, BF_R = 4 };
 
 __device__ __forceinline__ void push_hist(RState* st, double r2) {
   if (st->nhist < 512) st->hist[st->nhist] = st->bnorm2 > 0 ? sqrt(r2 / st->bnorm2) : 0.0;
   st->nhist++;
 }
 
-// one block per RHS; ntiles[i] = partial blocks of the producing kernel
+// a non-finite or overflowing squared norm
+__device__ __forceinline__ bool bad(double v) { return !(v <= 1e300); }
+
+// One block per RHS; ntiles[i] = partial blocks of the producing kernel.
+// Iteration k of Krylov.jl's bicgstab: kit counts it (at its first matvec),
+// iters counts the operator applications (two per iteration, plus the warm
+// start's r0 = gamma (b - A x0)).
 __global__ void __launch_bounds__(kThreads)
 k_bicg_fin(RState* const* __restrict__ sts, double* const* __restrict__ parts,
            const int* __restrict__ ntiles, int mode, int init_iters) {
@@ -112,6 +121,8 @@ k_bicg_fin(RState* const* __restrict__ sts, double* const* __restrict__ parts,
   const int nt = ntiles[rid];
   if (mode == BF_INIT) {
     const double r2 = block_sum_parts(part, nt, 0, red);
+    const double cr = block_sum_parts(part, nt, 1, red);
+    const double ci = block_sum_parts(part, nt, 2, red);
     if (threadIdx.x) return;
     st->r2 = r2;
     st->iters += init_iters;
@@ -121,53 +132,37 @@ k_bicg_fin(RState* const* __restrict__ sts, double* const* __restrict__ parts,
       const double t = st->abstol + st->tol * sqrt(r2);
       st->thr2 = t * t;
     }
-    // rho_new = <r^, r> = ||r||^2; beta is irrelevant while p = v = 0
-    st->beta_re = r2;
-    st->beta_im = 0.0;
-    st->rho_re = r2;
-    st->rho_im = 0.0;
+    // rho_1 = <c, r0>; beta is irrelevant while p = v = 0
+    st->beta_re = st->beta_im = 0.0;
+    st->rho_re = cr;
+    st->rho_im = ci;
     st->alpha_re = st->omega_re = 1.0;
     st->alpha_im = st->omega_im = 0.0;
-    if (!(r2 == r2)) st->done = 3;
-    else if (r2 <= st->thr2) st->done = 1;
-    else if (st->iters + 2 > st->maxit) st->done = 2;
+    if (bad(r2)) st->done = 3;
+    else if (r2 == 0.0 || r2 <= st->thr2) st->done = 1;
+    else if (cr == 0.0 && ci == 0.0) st->done = 4;  // breakdown: <c, r0> = 0
   } else if (mode == BF_ALPHA) {
     const double sr = block_sum_parts(part, nt, 0, red);
     const double si = block_sum_parts(part, nt, 1, red);
     if (threadIdx.x) return;
+    st->kit += 1;
     st->iters += 1;
-    const Cx<double> sig = mk<double>(sr, si);
     if (sr == 0.0 && si == 0.0) {
-      st->done = 4;  // breakdown: <r^, v> = 0
+      st->done = 4;  // breakdown: <c, v> = 0 (x is still the last iterate)
       return;
     }
-    const Cx<double> a = cdiv(mk<double>(st->rho_re, st->rho_im), sig);
+    const Cx<double> a = cdiv(mk<double>(st->rho_re, st->rho_im), mk<double>(sr, si));
     st->alpha_re = a.re;
     st->alpha_im = a.im;
-  } else if (mode == BF_S) {
-    const double s2 = block_sum_parts(part, nt, 0, red);
-    if (threadIdx.x) return;
-    st->s2 = s2;
-    if (!(s2 == s2)) {
-      st->done = 3;
-    } else if (s2 <= st->thr2) {  // x already holds x + alpha p
-      st->r2 = s2;
-      push_hist(st, s2);
-      st->done = 1;
-    }
   } else if (mode == BF_OMEGA) {
     const double ts_r = block_sum_parts(part, nt, 2, red);
     const double ts_i = block_sum_parts(part, nt, 3, red);
     const double tt = block_sum_parts(part, nt, 4, red);
     if (threadIdx.x) return;
     st->iters += 1;
-    if (tt == 0.0) {  // t = 0: s is the residual of x + alpha p
-      st->r2 = st->s2;
-      st->done = 4;
-      return;
-    }
-    st->omega_re = ts_r / tt;
-    st->omega_im = ts_i / tt;
+    // t = 0: omega = 0 leaves r = s, the residual of x + alpha p
+    st->omega_re = tt == 0.0 ? 0.0 : ts_r / tt;
+    st->omega_im = tt == 0.0 ? 0.0 : ts_i / tt;
   } else {  // BF_R
     const double r2 = block_sum_parts(part, nt, 0, red);
     const double pr = block_sum_parts(part, nt, 1, red);
@@ -175,17 +170,19 @@ k_bicg_fin(RState* const* __restrict__ sts, double* const* __restrict__ parts,
     if (threadIdx.x) return;
     st->r2 = r2;
     push_hist(st, r2);
-    const Cx<double> rn = mk<double>(pr, pi);
-    if (!(r2 == r2)) {
+    const bool om0 = st->omega_re == 0.0 && st->omega_im == 0.0;
+    const bool al0 = st->alpha_re == 0.0 && st->alpha_im == 0.0;
+    if (bad(r2)) {
       st->done = 3;
     } else if (r2 <= st->thr2) {
       st->done = 1;
-    } else if (st->iters + 2 > st->maxit) {
+    } else if (al0 || om0) {
+      st->done = 4;  // breakdown (Krylov.jl: alpha == 0)
+    } else if (st->kit >= st->maxit) {
       st->done = 2;
-    } else if (pr == 0.0 && pi == 0.0) {
-      st->done = 4;  // breakdown: <r^, r> = 0
     } else {
-      // beta = (rho_new / rho) (alpha / omega)
+      // beta = (rho_new / rho) (alpha / omega); rho_new = <c, r>
+      const Cx<double> rn = mk<double>(pr, pi);
       const Cx<double> b = cdiv(rn, mk<double>(st->rho_re, st->rho_im)) *
                            cdiv(mk<double>(st->alpha_re, st->alpha_im),
                                 mk<double>(st->omega_re, st->omega_im));
@@ -202,11 +199,10 @@ static BiVec<T> make_bivec(SolveItem<T>& x, int use_v) {
   BiVec<T> q;
   q.x = x.w->X[0].template as<Cx<T>>();
   q.r = x.w->kvec(0);
-  q.rh = x.w->kvec(1);
-  q.p = x.w->kvec(2);
-  q.v = x.w->kvec(3);
-  q.s = x.w->kvec(4);
-  q.t = x.w->kvec(5);
+  q.p = x.w->kvec(1);
+  q.v = x.w->kvec(2);
+  q.s = x.w->kvec(3);
+  q.t = x.w->kvec(4);
   q.V = potential<T>(*x.p);
   q.b = x.b;
   q.nv = x.p->nvox;
@@ -222,7 +218,7 @@ static void bicg_fin(StateSet& f, int n, int mode, int init_iters, hipStream_t s
                      f.parts.as<double*>(), f.ntiles.as<int>(), mode, init_iters);
 }
 
-// x0, r0 = gamma (b - A x0), r^ = r0 and the first state of every new member
+// x0, r0 = gamma (b - A x0), rho_1 = <b, r0> and the first state of every new member
 template <class T>
 void BicgRunner<T>::add(const std::vector<long long>& keys, std::vector<SolveItem<T>>& xs,
                         const IterOpts& io) {
@@ -239,7 +235,7 @@ void BicgRunner<T>::add(const std::vector<long long>& keys, std::vector<SolveIte
   Pipe<T> m0;
   for (int i = 0; i < n; ++i) {
     SolveItem<T>& x = xs[i];
-    x.w->ensure_k(ctx, 6);
+    x.w->ensure_k(ctx, kBicgVectors);
     BiVec<T>& q = hv[i] = make_bivec<T>(x, io.x0_zero ? 0 : 1);
     mc = std::max(mc, (q.nv + kChunk - 1) / kChunk);
     if (io.x0_zero) launch_fill<T>(q.x, 3 * q.nv, s);
@@ -266,7 +262,7 @@ void BicgRunner<T>::add(const std::vector<long long>& keys, std::vector<SolveIte
   CBS_HIP(hipGetLastError());
   CBS_HIP(hipStreamSynchronize(s));  // dvi, fv and m0 go out of scope
   for (int i = 0; i < n; ++i)
-    this->items.push_back({keys[i], xs[i], 0, (long long)io.maxit / 2 + this->k + 2});
+    this->items.push_back({keys[i], xs[i], 0, (long long)io.maxit + this->k + 2});
   this->changed();
 }
 
@@ -288,8 +284,8 @@ void BicgRunner<T>::rebuild() {
     SolveItem<T>& x = this->items[i].it;
     BiVec<T>& q = hv[i] = make_bivec<T>(x, 1);
     Cx<T>* W = x.w->W.template as<Cx<T>>();
-    // v = A^ p (with <r^, v>), t = A^ s (with <t, s>, |t|^2)
-    m1.h.push_back(make_desc<T>(*x.p, PRE_V, ZOP_GREEN, POST_PMATVEC, q.p, q.rh, q.v, W, q.part,
+    // v = A^ p (with <c, v>, c = b), t = A^ s (with <t, s>, |t|^2)
+    m1.h.push_back(make_desc<T>(*x.p, PRE_V, ZOP_GREEN, POST_PMATVEC, q.p, q.b, q.v, W, q.part,
                                 q.st));
     m2.h.push_back(make_desc<T>(*x.p, PRE_V, ZOP_GREEN, POST_PMATVEC, q.s, nullptr, q.t, W,
                                 q.part, q.st));
@@ -326,7 +322,6 @@ void BicgRunner<T>::iteration(int na) {
   launch_pipeline<T>(L1, s);
   bicg_fin(fin_mv, na, BF_ALPHA, 0, s);
   hipLaunchKernelGGL((k_bicg_vec<T, BI_S>), vgrid, dim3(kThreads), 0, s, d_vs);
-  bicg_fin(fin_vec, na, BF_S, 0, s);
   launch_pipeline<T>(L2, s);
   bicg_fin(fin_mv, na, BF_OMEGA, 0, s);
   hipLaunchKernelGGL((k_bicg_vec<T, BI_XR>), vgrid, dim3(kThreads), 0, s, d_vs);
@@ -592,11 +587,11 @@ k_gm_fin_start(RState* const* __restrict__ sts, double* const* __restrict__ part
   st->cyc = 0;
   st->pend = 0;
   sm[GM_KDONE] = 0.0;
-  if (!(r2 == r2)) {
+  if (bad(r2)) {
     st->done = 3;
-  } else if (r2 <= st->thr2) {
+  } else if (r2 == 0.0 || r2 <= st->thr2) {
     st->done = 1;
-  } else if (st->iters >= st->maxit) {
+  } else if (st->kit >= st->maxit) {
     st->done = 2;
   } else {
     const double beta = sqrt(r2);
@@ -607,9 +602,12 @@ k_gm_fin_start(RState* const* __restrict__ sts, double* const* __restrict__ part
 }
 
 // finish Arnoldi step k: h = ||w||, rotations, residual estimate, stop decision
+// (Krylov.jl's gmres: solved when the estimate |g_(k+1)| meets the threshold,
+// breakdown when h <= btol, tired after maxit Arnoldi steps in all cycles)
 __global__ void __launch_bounds__(kThreads)
 k_gm_fin_arnoldi(RState* const* __restrict__ sts, double* const* __restrict__ parts,
-                 const int* __restrict__ ntiles, double* const* __restrict__ sms, int m, int k) {
+                 const int* __restrict__ ntiles, double* const* __restrict__ sms, int m, int k,
+                 double btol) {
   __shared__ double red[kThreads / 64];
   const int rid = blockIdx.x;
   RState* st = sts[rid];
@@ -650,17 +648,21 @@ k_gm_fin_arnoldi(RState* const* __restrict__ sts, double* const* __restrict__ pa
   st2(sm, L.g(k), c * gk);
   const double res2 = abs2(ld(sm, L.g(k + 1)));
   st->iters += 1;
+  st->kit += 1;
   sm[GM_KDONE] = (double)(k + 1);
   st->r2 = res2;
   push_hist(st, res2);
   sm[GM_SCALE] = h > 0.0 ? 1.0 / h : 0.0;
-  if (!(res2 == res2) || !(h == h)) {
+  if (bad(res2) || bad(h2)) {
     st->cyc = 1;
     st->pend = 3;
-  } else if (res2 <= st->thr2 || h == 0.0) {
+  } else if (res2 <= st->thr2) {
     st->cyc = 1;
     st->pend = 1;
-  } else if (st->iters >= st->maxit) {
+  } else if (h <= btol) {
+    st->cyc = 1;
+    st->pend = 4;  // breakdown short of the tolerance
+  } else if (st->kit >= st->maxit) {
     st->cyc = 1;
     st->pend = 2;
   } else if (k + 1 == m) {
@@ -669,9 +671,10 @@ k_gm_fin_arnoldi(RState* const* __restrict__ sts, double* const* __restrict__ pa
   }
 }
 
-// back substitution y = R^-1 g over the kdone completed steps
+// back substitution y = R^-1 g over the kdone completed steps (a diagonal entry
+// |r_ii| <= btol gives y_i = 0, as in Krylov.jl)
 __global__ void k_gm_solve(RState* const* __restrict__ sts, double* const* __restrict__ sms,
-                           int m) {
+                           int m, double btol) {
   const int rid = blockIdx.x;
   RState* st = sts[rid];
   if (st->done || threadIdx.x) return;
@@ -681,7 +684,8 @@ __global__ void k_gm_solve(RState* const* __restrict__ sts, double* const* __res
   for (int i = kd - 1; i >= 0; --i) {
     Cx<double> acc = ld(sm, L.g(i));
     for (int j = i + 1; j < kd; ++j) acc = acc - ld(sm, L.H(i, j)) * ld(sm, L.y(j));
-    st2(sm, L.y(i), cdiv(acc, ld(sm, L.H(i, i))));
+    const Cx<double> d = ld(sm, L.H(i, i));
+    st2(sm, L.y(i), sqrt(abs2(d)) <= btol ? mk<double>(0.0, 0.0) : cdiv(acc, d));
   }
 }
 
@@ -716,6 +720,7 @@ __global__ void k_gm_fin_cycle(RState* const* __restrict__ sts) {
   if (st->pend == 1) st->done = 1;
   else if (st->pend == 2) st->done = 2;
   else if (st->pend == 3) st->done = 3;
+  else if (st->pend == 4) st->done = 4;
   st->cyc = 0;
 }
 
@@ -723,8 +728,13 @@ template <class T>
 void gmres_solve(Ctx* ctx, std::vector<SolveItem<T>>& it, const IterOpts& io, hipStream_t s,
                  std::vector<RState>& states, std::vector<const Cx<T>*>& result) {
   const int n = (int)it.size();
-  const int m = std::max(1, std::min(io.restart > 0 ? io.restart : io.maxit, 400));
+  CBS_REQUIRE(io.restart <= kGmresMaxBasis && (io.restart > 0 || io.maxit <= kGmresMaxBasis),
+              "GMRES keeps at most 400 basis vectors: full GMRES needs maxiters <= 400, "
+              "otherwise set gmres_restart (1..400)");
+  const int m = std::max(1, io.restart > 0 ? io.restart : io.maxit);
   const GmLayout lay{m};
+  // Krylov.jl's breakdown tolerance eps(T)^(3/4) in the working precision
+  const double btol = std::pow((double)std::numeric_limits<T>::epsilon(), 0.75);
   StateSet gn, fv;
   long long maxchunk = 1;
   std::vector<GmVec<T>> hv(n);
@@ -810,11 +820,11 @@ void gmres_solve(Ctx* ctx, std::vector<SolveItem<T>>& it, const IterOpts& io, hi
         hipLaunchKernelGGL((k_gm_update<T>), vgrid, dim3(kThreads), 0, s, d_vs, m, k);
       }
       hipLaunchKernelGGL(k_gm_fin_arnoldi, dim3(n), dim3(kThreads), 0, s, d_sts, d_parts, d_nt,
-                         d_sms, m, k);
+                         d_sms, m, k, btol);
       if (k + 1 < m)
         hipLaunchKernelGGL((k_gm_scale<T>), vgrid, dim3(kThreads), 0, s, d_vs, k + 1);
     }
-    hipLaunchKernelGGL(k_gm_solve, dim3(n), dim3(64), 0, s, d_sts, d_sms, m);
+    hipLaunchKernelGGL(k_gm_solve, dim3(n), dim3(64), 0, s, d_sts, d_sms, m, btol);
     hipLaunchKernelGGL((k_gm_xupdate<T>), vgrid, dim3(kThreads), 0, s, d_vs, m);
     hipLaunchKernelGGL(k_gm_fin_cycle, dim3(n), dim3(64), 0, s, d_sts);
   };

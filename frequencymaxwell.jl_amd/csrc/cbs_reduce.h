@@ -110,18 +110,19 @@ __device__ __forceinline__ void born_finalize_rhs(const BornFin& f, int i, doubl
     st->r2 = r2;
     const int it = st->iters;
     st->iters = it + 1;
+    st->kit = it + 1;
     if (st->thr2 < 0.0) {
       const double t = st->abstol + st->tol * sqrt(r2);
       st->thr2 = t * t;
     }
     if (it < 512) st->hist[it] = st->bnorm2 > 0 ? sqrt(r2 / st->bnorm2) : 0.0;
-    if (!(r2 == r2) || r2 > 1e300) {
+    if (!(r2 <= 1e300)) {  // NaN, Inf or overflow: numeric failure
       st->done = 3;
       st->ans = f.parity;
     } else if (r2 <= st->thr2) {
       st->done = 1;
       st->ans = f.parity;  // the operand of this sweep met the tolerance
-    } else if (st->iters >= st->maxit) {
+    } else if (st->kit >= st->maxit) {
       st->done = 2;
       st->ans = 1 - f.parity;  // the freshly updated iterate
     }

@@ -63,6 +63,7 @@ struct Job {
   int phase = PH_WAIT_LOAD;
   double bn2 = 0, thr2 = -1.0;
   int used = 0, refines = 0;
+  bool numeric = false;  // a residual became NaN or Inf: the request ends CBS_ENUMERIC
   SolveItem<double> it64;
   SolveItem<float> it32;
   int set = -1;  // solver set the job is a member of (-1: none)
@@ -387,8 +388,9 @@ void Server::resid_step() {
       Job& j = *js[q];
       const cbs_solve_opts& o = j.o;
       cbs_solve_stats& st = j.stats;
-      st.iters += 1;
+      st.iters += 1;  // the outer step: one FP64 residual evaluation
       st.iters_fp64 += 1;
+      st.applications += 1;
       j.used += 1;
       st.bytes_moved += algo_bytes_per_iter(*j.p, CBS_FP64);
       if (j.thr2 < 0) {
@@ -399,16 +401,20 @@ void Server::resid_step() {
       st.resid = j.bn2 > 0 ? std::sqrt(g2[q] / j.bn2) : 0.0;
       st.resid_true_fp64 = st.resid;
       const int max_refine = o.max_refine > 0 ? o.max_refine : 20;
-      if (j.bn2 == 0.0 || g2[q] <= j.thr2) {
+      if (!(g2[q] <= 1e300)) {
+        j.numeric = true;
+        j.phase = PH_RETIRE;
+      } else if (j.bn2 == 0.0 || g2[q] <= j.thr2) {
         st.converged = 1;
         j.phase = PH_RETIRE;
-      } else if (!(g2[q] == g2[q]) || j.refines >= max_refine || j.used >= o.maxiters) {
+      } else if (j.refines >= max_refine || j.used + 1 >= o.maxiters) {
         j.phase = PH_RETIRE;
       } else {
         j.refines += 1;
         st.refine_steps += 1;
         launch_convert<float, double>(j.it32.w->B.as<Cx<float>>(), rs[q], 3 * j.p->nvox, stream);
-        IterOpts io = iter_opts(o, std::max(1, o.maxiters - j.used));
+        // the budget keeps one iteration for the final FP64 residual evaluation
+        IterOpts io = iter_opts(o, std::max(1, o.maxiters - j.used - 1));
         io.tol = refine_inner_tol(std::sqrt(j.thr2 / std::max(g2[q], 1e-300)), o.inner_tol);
         io.abstol = 0.0;
         io.rule = CBS_STOP_RHS;
@@ -466,19 +472,23 @@ void Server::wait_sets() {
         // x += d (FP32 correction), then back to the FP64 residual
         launch_accum<double, float>(j.X.as<Cx<double>>(),
                                     reinterpret_cast<const Cx<float>*>(f.x), nc, stream);
-        j.stats.iters += f.st.iters;
-        j.used += f.st.iters;
+        j.stats.iters += f.st.kit;
+        j.stats.applications += f.st.iters;
+        j.used += f.st.kit;
         j.stats.bytes_moved += algo_bytes_per_iter(*j.p, CBS_FP32) * f.st.iters;
+        if (f.st.done == 3) j.numeric = true;
         j.phase = PH_RESID;
       } else if (fp32) {
         launch_convert<double, float>(j.X.as<Cx<double>>(),
                                       reinterpret_cast<const Cx<float>*>(f.x), nc, stream);
         fill_stats(j.stats, f.st, *j.p, CBS_FP32);
+        j.numeric = f.st.done == 3;
         j.phase = PH_RETIRE;
       } else {
         copy_dev<double>(j.X.as<Cx<double>>(), reinterpret_cast<const Cx<double>*>(f.x), nc,
                          stream);
         fill_stats(j.stats, f.st, *j.p, CBS_FP64);
+        j.numeric = f.st.done == 3;
         j.phase = PH_RETIRE;
       }
     }
@@ -499,7 +509,7 @@ void Server::begin_retire(Job& j) {
                  j.Eo.as<Cx<double>>(), j.o.compute_h ? j.Ho.as<Cx<double>>() : nullptr, stream);
   CBS_HIP(hipEventCreateWithFlags(&j.ev, hipEventDisableTiming));
   CBS_HIP(hipEventRecord(j.ev, stream));
-  j.stats.voxel_iters = (int64_t)j.stats.iters * p.nvox;
+  j.stats.voxel_iters = (int64_t)j.stats.applications * p.nvox;
   j.phase = PH_WAIT_DOWN;
 }
 
@@ -544,6 +554,12 @@ void Server::tick() {
     landed->phase = PH_DOWN_DONE;
   }
   resid_step();
+  // a request whose residual stopped being finite ends without H or download
+  std::vector<std::shared_ptr<Job>> bad;
+  for (auto& j : active)
+    if (j->phase == PH_RETIRE && j->numeric) bad.push_back(j);
+  for (auto& j : bad)
+    end_job(j, CBS_ENUMERIC, "the residual became NaN or Inf (or overflowed)");
   for (auto& j : active)
     if (j->phase == PH_RETIRE) begin_retire(*j);
   launch_sets();

@@ -2,6 +2,7 @@
 #pragma once
 
 #include "cbs_engine.h"
+#include "cbs_refine.h"
 
 namespace cbs {
 
@@ -14,6 +15,8 @@ struct SolveItem {
 
 // kernels timed by time_sweeps: the five sweeps and the fused x pass
 constexpr int kSweepKernels = 6;
+// GMRES basis vectors at most (full GMRES keeps one per iteration)
+constexpr int kGmresMaxBasis = 400;
 
 struct IterOpts {
   double tol = 1e-6, abstol = 0.0;
@@ -62,8 +65,6 @@ void resid64(Ctx* ctx, std::vector<SolveItem<double>>& it, std::vector<const Cx<
              std::vector<Cx<double>*>& r, std::vector<double>& g2, hipStream_t s);
 void gnorm64(Ctx* ctx, std::vector<SolveItem<double>>& it, std::vector<double>& g2,
              hipStream_t s);
-// inner tolerance of the FP32 correction solve of iterative refinement
-double refine_inner_tol(double need, double user_inner_tol);
 // iterations per runner chunk for a method (Born: even)
 int runner_chunk(int method, int check_every);
 // algorithmic HBM bytes of one operator application (five sweeps)
@@ -72,6 +73,8 @@ double algo_bytes_per_iter(const Problem& p, int prec);
 IterOpts iter_opts(const cbs_solve_opts& o, int maxit);
 // add the outcome of one solver run (state rs, precision prec) to st
 void fill_stats(cbs_solve_stats& st, const RState& rs, const Problem& p, int prec);
+// the C-ABI checks of a solve's option block (CBS_EINVAL)
+void check_opts(const cbs_solve_opts& o);
 // allocate a work slot of problem p for one right-hand side
 template <class T>
 Work<T>& prepare_work(Problem& p, int slot, bool need_r);

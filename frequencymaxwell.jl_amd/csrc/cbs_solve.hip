@@ -10,9 +10,11 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <string>
 #include <type_traits>
 
 #include "cbs_engine.h"
+#include "cbs_refine.h"
 #include "cbs_solve.h"
 
 namespace cbs {
@@ -479,25 +481,6 @@ static void prepare_items(std::vector<RhsIO>& io, std::vector<SolveItem<T>>& it,
   }
 }
 
-// eta: the reduction the outer test still needs (x 0.3), kept within
-// [1e-3, 0.1]: tighter inner targets stall the FP32 Born iteration on the slow
-// modes left by the previous correction (measured: 1e-4 does not finish in
-// 4000 sweeps where 1e-3 needs 927 in total)
-// Inner tolerance of an FP32+refine step that must reduce the FP64 residual by
-// `need`.  A Born (or Krylov) solve of the correction equation from d = 0
-// continues the iteration, so splitting the reduction into m equal inner
-// reductions costs no extra sweeps, but each inner solve must stay above the
-// FP32 floor (9e-4: far below it the FP32 Born iteration stagnates); a 10 %
-// margin covers the FP32 rounding of the inner residual.
-double refine_inner_tol(double need, double user_inner_tol) {
-  if (user_inner_tol > 0) return user_inner_tol;
-  const double kFloor = 9e-4;
-  const double target = 0.9 * need;
-  if (target >= kFloor) return std::min(0.5, target);
-  const int m = (int)std::ceil(std::log(target) / std::log(kFloor));
-  return std::pow(target, 1.0 / m);
-}
-
 IterOpts iter_opts(const cbs_solve_opts& o, int maxit) {
   IterOpts io;
   io.tol = o.tol;
@@ -513,11 +496,34 @@ IterOpts iter_opts(const cbs_solve_opts& o, int maxit) {
 }
 
 void fill_stats(cbs_solve_stats& st, const RState& rs, const Problem& p, int prec) {
-  st.iters += rs.iters;
+  st.iters += rs.kit;
+  st.applications += rs.iters;
   if (prec == CBS_FP64) st.iters_fp64 += rs.iters;
   st.converged = rs.done == 1;
   st.resid = rs.bnorm2 > 0 ? std::sqrt(rs.r2 / rs.bnorm2) : 0.0;
   st.bytes_moved += algo_bytes_per_iter(p, prec) * rs.iters;
+}
+
+void check_opts(const cbs_solve_opts& o) {
+  CBS_REQUIRE(o.tol >= 0 && o.tol <= 1, "tolerance must be in [0, 1]");
+  CBS_REQUIRE(o.maxiters >= 1, "maxiters must be >= 1");
+  CBS_REQUIRE(o.method >= CBS_BORN && o.method <= CBS_GMRES, "unknown method");
+  CBS_REQUIRE(o.precision >= CBS_FP64 && o.precision <= CBS_FP32_REFINE, "unknown precision");
+  CBS_REQUIRE(o.stop_rule == CBS_STOP_RHS || o.stop_rule == CBS_STOP_KRYLOV, "unknown stop rule");
+  CBS_REQUIRE(o.method != CBS_GMRES ||
+                  (o.gmres_restart >= 0 && o.gmres_restart <= kGmresMaxBasis &&
+                   (o.gmres_restart > 0 || o.maxiters <= kGmresMaxBasis)),
+              "GMRES keeps at most 400 basis vectors: full GMRES (gmres_restart 0) needs "
+              "maxiters <= 400, otherwise set gmres_restart in 1..400");
+}
+
+// a right-hand side whose residual stopped being finite ends the call with
+// CBS_ENUMERIC (its iterate is garbage; the other right-hand sides finished)
+static void check_numeric(const std::vector<int>& numeric) {
+  for (size_t i = 0; i < numeric.size(); ++i)
+    if (numeric[i])
+      throw Error(CBS_ENUMERIC, "right-hand side " + std::to_string(i) +
+                                    ": the residual became NaN or Inf (or overflowed)");
 }
 
 template <class T>
@@ -569,13 +575,10 @@ void solve_set(Ctx* ctx, std::vector<RhsIO>& io, const cbs_solve_opts& o,
                std::vector<cbs_solve_stats>& stats, hipStream_t s) {
   const int n = (int)io.size();
   CBS_REQUIRE(n > 0, "no right-hand sides");
-  CBS_REQUIRE(o.tol >= 0 && o.tol <= 1, "tolerance must be in [0, 1]");
-  CBS_REQUIRE(o.maxiters >= 1, "maxiters must be >= 1");
-  CBS_REQUIRE(o.method >= CBS_BORN && o.method <= CBS_GMRES, "unknown method");
-  CBS_REQUIRE(o.precision >= CBS_FP64 && o.precision <= CBS_FP32_REFINE, "unknown precision");
-  CBS_REQUIRE(o.stop_rule == CBS_STOP_RHS || o.stop_rule == CBS_STOP_KRYLOV, "unknown stop rule");
+  check_opts(o);
   for (auto& r : io) CBS_REQUIRE(r.p->S == io[0].p->S, "a set mixes averaging and periodic problems");
   stats.assign(n, cbs_solve_stats{});
+  std::vector<int> numeric(n, 0);
   hipEvent_t e0, e1;
   CBS_HIP(hipEventCreate(&e0));
   CBS_HIP(hipEventCreate(&e1));
@@ -597,6 +600,7 @@ void solve_set(Ctx* ctx, std::vector<RhsIO>& io, const cbs_solve_opts& o,
     for (int i = 0; i < n; ++i) {
       copy_dev<double>(io[i].E, res[i], 3 * io[i].p->nvox, s);
       fill_stats(stats[i], st[i], *io[i].p, CBS_FP64);
+      numeric[i] = st[i].done == 3;
     }
   } else if (o.precision == CBS_FP32) {
     for (auto& r : io) ensure_v32(*r.p, s);
@@ -617,6 +621,7 @@ void solve_set(Ctx* ctx, std::vector<RhsIO>& io, const cbs_solve_opts& o,
     for (int i = 0; i < n; ++i) {
       launch_convert<double, float>(io[i].E, res[i], 3 * io[i].p->nvox, s);
       fill_stats(stats[i], st[i], *io[i].p, CBS_FP32);
+      numeric[i] = st[i].done == 3;
     }
   } else {
     // FP32 bulk inside an FP64 iterative-refinement loop
@@ -663,8 +668,9 @@ void solve_set(Ctx* ctx, std::vector<RhsIO>& io, const cbs_solve_opts& o,
       for (size_t j = 0; j < idx.size(); ++j) {
         const int i = idx[j];
         cbs_solve_stats& stt = stats[i];
-        stt.iters += 1;
+        stt.iters += 1;  // the outer step: one FP64 residual evaluation
         stt.iters_fp64 += 1;
+        stt.applications += 1;
         used[i] += 1;
         stt.bytes_moved += algo_bytes_per_iter(*io[i].p, CBS_FP64);
         if (thr2[i] < 0) {
@@ -674,12 +680,13 @@ void solve_set(Ctx* ctx, std::vector<RhsIO>& io, const cbs_solve_opts& o,
         }
         stt.resid = bn2[i] > 0 ? std::sqrt(g2[j] / bn2[i]) : 0.0;
         stt.resid_true_fp64 = stt.resid;
-        if (bn2[i] == 0.0 || g2[j] <= thr2[i]) {
+        if (!(g2[j] <= 1e300)) {
+          numeric[i] = 1;
+          active[i] = 0;
+        } else if (bn2[i] == 0.0 || g2[j] <= thr2[i]) {
           stt.converged = 1;
           active[i] = 0;
-        } else if (!(g2[j] == g2[j])) {
-          active[i] = 0;
-        } else if (k >= max_refine || used[i] >= o.maxiters) {
+        } else if (k >= max_refine || used[i] + 1 >= o.maxiters) {
           active[i] = 0;
         } else {
           go.push_back(i);
@@ -696,7 +703,8 @@ void solve_set(Ctx* ctx, std::vector<RhsIO>& io, const cbs_solve_opts& o,
         if (!active[i]) continue;
         launch_convert<float, double>(it32[i].w->B.as<Cx<float>>(), rs[i], 3 * io[i].p->nvox, s);
         a32.push_back(it32[i]);
-        IterOpts io32 = iter_opts(o, std::max(1, o.maxiters - used[i]));
+        // the budget keeps one iteration for the final FP64 residual evaluation
+        IterOpts io32 = iter_opts(o, std::max(1, o.maxiters - used[i] - 1));
         io32.tol = refine_inner_tol(std::sqrt(thr2[i] / std::max(g2[j], 1e-300)), o.inner_tol);
         io32.abstol = 0.0;
         io32.rule = CBS_STOP_RHS;
@@ -719,9 +727,11 @@ void solve_set(Ctx* ctx, std::vector<RhsIO>& io, const cbs_solve_opts& o,
       for (size_t j = 0; j < go.size(); ++j) {
         const int i = go[j];
         launch_accum<double, float>(xs[i], res[j], 3 * io[i].p->nvox, s);
-        stats[i].iters += st[j].iters;
-        used[i] += st[j].iters;
+        stats[i].iters += st[j].kit;
+        stats[i].applications += st[j].iters;
+        used[i] += st[j].kit;
         stats[i].bytes_moved += algo_bytes_per_iter(*io[i].p, CBS_FP32) * st[j].iters;
+        if (st[j].done == 3) numeric[i] = 1;
       }
       CBS_HIP(hipGetLastError());
     }
@@ -734,8 +744,9 @@ void solve_set(Ctx* ctx, std::vector<RhsIO>& io, const cbs_solve_opts& o,
   (void)hipEventDestroy(e1);
   for (int i = 0; i < n; ++i) {
     stats[i].time_ms = ms;
-    stats[i].voxel_iters = (int64_t)stats[i].iters * io[i].p->nvox;
+    stats[i].voxel_iters = (int64_t)stats[i].applications * io[i].p->nvox;
   }
+  check_numeric(numeric);
 }
 
 template <class T>

@@ -84,6 +84,7 @@ class SolveStats(C.Structure):
         ("time_ms", C.c_double),
         ("bytes_moved", C.c_double),
         ("voxel_iters", C.c_int64),
+        ("applications", C.c_int32),
     ]
 
     def as_dict(self) -> dict:

@@ -3,9 +3,12 @@
  *
  * The library replaces the device side of FrequencyMaxwell.jl's solve path:
  * the dyadic Green operator, the CBS linear operator and preconditioner, the
- * curl used for H, and the Krylov loop that LinearSolve.jl runs.  Host code
- * (potential build, incident plane wave, source term, crop) stays with the
- * caller, exactly as in the reference.
+ * curl used for H, and the Krylov loop that LinearSolve.jl runs.  Two routes:
+ * cbs_solve and its relatives take the potential and source term the caller
+ * built (the reference's host stages stay with the caller); cbs_solve_scene
+ * and its relatives take the permittivity and plane waves and build the
+ * potential, incident field and source term, and add the incident field and
+ * crop the result, on the GPU.
  *
  * Conventions
  *   - Complex numbers are interleaved (re, im) doubles: a Julia
@@ -48,7 +51,8 @@ enum {
   CBS_ENOTFOUND = -4, /* unknown ticket */
   CBS_EBUSY = -5,     /* serving queue full */
   CBS_ECANCELED = -6, /* request was cancelled */
-  CBS_ENUMERIC = -7   /* NaN / Inf detected in an input or iterate */
+  CBS_ENUMERIC = -7   /* NaN / Inf in an input, or a residual that became NaN,
+                         Inf or overflowed during the iteration */
 };
 
 /* Working precision of the iteration. */
@@ -88,10 +92,17 @@ typedef struct {
   int32_t method;       /* CBS_BORN | CBS_BICGSTAB | CBS_GMRES              */
   int32_t precision;    /* CBS_FP64 | CBS_FP32 | CBS_FP32_REFINE            */
   int32_t stop_rule;    /* CBS_STOP_RHS | CBS_STOP_KRYLOV                   */
-  int32_t maxiters;     /* bound on operator applications                  */
+  int32_t maxiters;     /* bound on iterations as the method counts them:
+                         * Born sweeps; BiCGStab iterations (two operator
+                         * applications each) and GMRES Arnoldi steps, as
+                         * Krylov.jl's itmax (LinearSolve maxiters);
+                         * FP32_REFINE: inner iterations plus FP64 residual
+                         * evaluations */
   double tol;           /* relative tolerance (reltol)                     */
   double abstol;        /* absolute tolerance (CBS_STOP_KRYLOV only)       */
-  int32_t gmres_restart;/* 0 = no restart (Krylov.jl default)              */
+  int32_t gmres_restart;/* 0 = no restart (Krylov.jl default).  GMRES keeps at
+                         * most 400 basis vectors: restart 0 needs maxiters
+                         * <= 400, restart must be <= 400 (CBS_EINVAL) */
   int32_t max_refine;   /* FP32_REFINE: bound on FP64 refinement steps     */
   double inner_tol;     /* FP32_REFINE: inner relative tolerance (0 = auto)*/
   int32_t check_every;  /* host convergence check cadence (>=1)            */
@@ -107,15 +118,24 @@ typedef struct {
 } cbs_solve_opts;
 
 typedef struct {
-  int32_t iters;          /* operator applications (voxel-iterations / N_vox) */
-  int32_t iters_fp64;     /* of which in FP64                                */
+  int32_t iters;          /* iterations as the method counts them (the
+                           * reference's solver.iteration_count = LinearSolve
+                           * sol.iters, convergent_born.jl:761-763): Born
+                           * sweeps; BiCGStab iterations; GMRES Arnoldi steps
+                           * (the warm-start residual not counted);
+                           * FP32_REFINE: inner iterations plus the FP64
+                           * residual evaluations                            */
+  int32_t iters_fp64;     /* operator applications in FP64                   */
   int32_t refine_steps;   /* FP64 refinement steps (FP32_REFINE)             */
   int32_t converged;      /* 1 when the stopping rule was met                */
   double resid;           /* ||gamma (b - A x)|| / ||gamma b|| (FP64) of the returned x */
   double resid_true_fp64; /* same, recomputed in FP64 after the solve (0 if not run) */
   double time_ms;         /* device time of the solve                        */
   double bytes_moved;     /* algorithmic HBM bytes of the solve              */
-  int64_t voxel_iters;    /* iters * N_vox                                   */
+  int64_t voxel_iters;    /* applications * N_vox                            */
+  int32_t applications;   /* operator applications (Born: one per sweep;
+                           * BiCGStab: two per iteration plus the warm start;
+                           * GMRES: one per step plus one per cycle start)   */
 } cbs_solve_stats;
 
 /* ---- context ------------------------------------------------------------ */

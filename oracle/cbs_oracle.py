@@ -25,12 +25,22 @@ tests live in ``tests/test_oracle.py``):
 * incident field                     -- closed-form plane wave, transversality,
   impedance ``|E|/|H| = 377/sqrt(eps_bg)`` and the reference's own tests
   (test/sources/plane_wave.jl).
+* field analysis                     -- a plane wave's Poynting vector
+  ``|A|^2/Z`` along k, the energy of constant fields in closed form, the
+  right-hand rule of the cross product on unit vectors, planes by index.
 * Born / BiCGStab / GMRES solves     -- dense brute-force solve of
   ``(I - G_avg V) x = G_avg S`` on tiny grids, the spectral PDE residual
   ``M(k) x = F[V x + S]`` (periodic), homogeneous and uniform-medium closed
   forms, mirror symmetry.
-* Krylov stopping sequence of LinearSolve.jl 2 / Krylov.jl (third party, not
-  vendored): *parity unpinned* -- only the converged solution is pinned.
+* Krylov iterates                   -- fixed-budget BiCGStab (shadow r0) and GMRES
+  (full and restarted) iterates equal scipy.sparse.linalg's ``bicgstab`` /
+  ``gmres`` on the explicit left-preconditioned operator; every GMRES iterate
+  is the least-squares minimiser over the Krylov space (brute force); every
+  BiCGStab iterate, for either shadow vector, satisfies the Petrov-Galerkin
+  characterisation r_k = Omega_k(A) phi_k(A) r0 (brute force).
+* What stays unpinned: the stopping point and count convention of Krylov.jl
+  itself (third party, not vendored), restated here from its published
+  algorithm.
 """
 
 from __future__ import annotations
@@ -505,8 +515,11 @@ def cbs_precond(x: np.ndarray, V: np.ndarray, eps_imag: float) -> np.ndarray:
 
 @dataclass
 class SolveStats:
-    iters: int = 0          # operator applications (voxel-iterations / N_vox)
+    iters: int = 0          # iterations as the method counts them (Krylov.jl ``niter``;
+                            # Born: sweeps)
+    applications: int = 0   # operator applications (voxel-iterations / N_vox)
     converged: bool = False
+    breakdown: bool = False
     resid: float = float("nan")  # ||gamma (b - A x)|| / ||gamma b|| of the returned x
     history: list = field(default_factory=list)
 
@@ -530,7 +543,7 @@ def solve_born(b, x0, V, eps_imag, gp, tol, maxiters, rule="rhs", abstol=0.0):
 
     Each pass costs one operator application.  The residual of the current
     iterate is measured in the same pass; when it meets the threshold that
-    iterate is returned unchanged.
+    iterate is returned unchanged.  ``maxiters`` bounds the passes.
     """
     precond = lambda v: cbs_precond(v, V, eps_imag)
     bnorm = _nrm(precond(b))
@@ -543,6 +556,7 @@ def solve_born(b, x0, V, eps_imag, gp, tol, maxiters, rule="rhs", abstol=0.0):
     while True:
         r = precond(b - cbs_operator(x, V, gp))
         st.iters += 1
+        st.applications += 1
         rn = _nrm(r)
         st.history.append(rn / bnorm)
         if thr is None:
@@ -557,16 +571,34 @@ def solve_born(b, x0, V, eps_imag, gp, tol, maxiters, rule="rhs", abstol=0.0):
         x = x + r
 
 
-def solve_bicgstab(b, x0, V, eps_imag, gp, tol, maxiters, rule="rhs", abstol=0.0):
-    """Left-preconditioned BiCGStab (van der Vorst 1992), shadow residual = r0.
-
-    Follows the published algorithm used by Krylov.jl ``bicgstab`` (the
-    LinearSolve ``KrylovJL_BICGSTAB`` route of examples/two_beam_interference.jl:58);
-    the exact iterate sequence of Krylov.jl is third party: parity unpinned.
-    ``maxiters`` bounds operator applications (2 per iteration).
-    """
+def _left_system(b, V, eps_imag, gp):
+    """The left-preconditioned operator v -> gamma A v and gamma b
+    (LinearSolve passes Pl = CBSPreconditioner to Krylov.jl, which applies it
+    with ldiv!, convergent_born.jl:398-431, 750)."""
     A = lambda v: cbs_precond(cbs_operator(v, V, gp), V, eps_imag)
-    bh = cbs_precond(b, V, eps_imag)
+    return A, cbs_precond(b, V, eps_imag)
+
+
+def solve_bicgstab(b, x0, V, eps_imag, gp, tol, maxiters, rule="rhs", abstol=0.0, shadow="b"):
+    """Left-preconditioned BiCGStab as Krylov.jl ``bicgstab`` runs it for
+    LinearSolve's ``KrylovJL_BICGSTAB`` (the route of
+    examples/two_beam_interference.jl:58; Krylov.jl is third party, not vendored).
+
+    Krylov.jl's published algorithm (van der Vorst 1992 in its notation): warm
+    start ``r0 = M (b - A x0)`` (one operator application outside the iteration
+    count), shadow vector ``c = b`` (Krylov.jl's default, the unpreconditioned
+    right-hand side; ``shadow='r0'`` gives the textbook choice), ``p1 = r0``;
+    iteration k: ``v = M A p``, ``alpha = rho / <c, v>``, ``s = r - alpha v``,
+    ``x += alpha p``, ``t = M A s``, ``omega = <t, s> / <t, t>``, ``x += omega s``,
+    ``r = s - omega t``, ``rho' = <c, r>``, ``beta = (rho'/rho)(alpha/omega)``,
+    ``p = r + beta (p - omega v)``; then stop when ``||r|| <= eps`` (solved),
+    ``k >= maxiters`` (tired) or ``alpha`` is 0 or NaN (breakdown).  There is no
+    test on the half step ``s``.  Krylov.jl also stops when ``||r|| + 1 <= 1``, an
+    absolute test that depends on the units of the source term; it is left out.  ``maxiters`` and
+    ``iters`` count iterations (two operator applications each); the iterate
+    sequence of the third-party code itself is not imported here.
+    """
+    A, bh = _left_system(b, V, eps_imag, gp)
     bnorm = _nrm(bh)
     st = SolveStats()
     x = x0.copy(order="F")
@@ -574,74 +606,97 @@ def solve_bicgstab(b, x0, V, eps_imag, gp, tol, maxiters, rule="rhs", abstol=0.0
         st.converged, st.resid = True, 0.0
         return np.zeros_like(x), st
     r = bh - A(x)
-    st.iters += 1
+    st.applications += 1
     rn = _nrm(r)
     st.history.append(rn / bnorm)
     thr = _stop_threshold(rule, tol, abstol, bnorm, rn)
-    if rn <= thr:
+    if rn == 0.0 or rn <= thr:
         st.converged, st.resid = True, rn / bnorm
         return x, st
-    rhat = r.copy()
-    rho = alpha = omega = 1.0 + 0j
-    v = np.zeros_like(x)
-    p = np.zeros_like(x)
-    while st.iters + 2 <= maxiters:
-        rho_new = np.vdot(rhat, r)
-        beta = (rho_new / rho) * (alpha / omega)
-        p = r + beta * (p - omega * v)
+    c = b.copy(order="F") if shadow == "b" else r.copy(order="F")
+    rho = np.vdot(c, r)
+    if rho == 0:
+        st.breakdown, st.resid = True, rn / bnorm
+        return x, st
+    p = r.copy(order="F")
+    while True:
+        st.iters += 1
         v = A(p)
-        st.iters += 1
-        alpha = rho_new / np.vdot(rhat, v)
+        st.applications += 1
+        alpha = rho / np.vdot(c, v)
         s = r - alpha * v
-        sn = _nrm(s)
-        if sn <= thr:
-            x = x + alpha * p
-            st.history.append(sn / bnorm)
-            st.converged, st.resid = True, sn / bnorm
-            return x, st
+        x = x + alpha * p
         t = A(s)
-        st.iters += 1
-        omega = np.vdot(t, s) / np.vdot(t, t)
-        x = x + alpha * p + omega * s
+        st.applications += 1
+        tt = np.vdot(t, t).real
+        omega = np.vdot(t, s) / tt if tt != 0.0 else 0.0
+        x = x + omega * s
         r = s - omega * t
-        rho = rho_new
+        rho_next = np.vdot(c, r)
+        if omega != 0:
+            beta = (rho_next / rho) * (alpha / omega)
+            p = r + beta * (p - omega * v)
+        rho = rho_next
         rn = _nrm(r)
         st.history.append(rn / bnorm)
-        if rn <= thr:
-            st.converged, st.resid = True, rn / bnorm
+        solved = rn <= thr
+        breakdown = alpha == 0 or np.isnan(alpha) or (omega == 0 and not solved)
+        if solved or breakdown or st.iters >= maxiters:
+            st.converged, st.breakdown = bool(solved), bool(breakdown and not solved)
+            st.resid = rn / bnorm
             return x, st
-    st.resid = rn / bnorm
-    return x, st
 
 
-def solve_gmres(b, x0, V, eps_imag, gp, tol, maxiters, rule="rhs", abstol=0.0, restart=0):
-    """Left-preconditioned GMRES with modified Gram-Schmidt and Givens rotations.
+def _givens(a, b):
+    """Rotation (c real, s complex) with [c s; -conj(s) c] [a; b] = [rho; 0]."""
+    na = abs(a)
+    den = math.sqrt(abs(a) ** 2 + abs(b) ** 2)
+    if den == 0.0:
+        return 1.0, 0.0j
+    if na == 0.0:
+        return 0.0, 1.0 + 0.0j
+    return na / den, (a / na) * np.conj(b) / den
 
-    The published algorithm of Krylov.jl ``gmres`` (default route of the
-    reference: ``KrylovJL_GMRES()``, convergent_born.jl:197, no restart).
-    ``restart > 0`` restarts every ``restart`` inner steps.  Iterate sequence of
-    Krylov.jl itself: parity unpinned.  ``maxiters`` bounds operator applications.
+
+def solve_gmres(b, x0, V, eps_imag, gp, tol, maxiters, rule="rhs", abstol=0.0, restart=0,
+                btol=None):
+    """Left-preconditioned GMRES as Krylov.jl ``gmres`` runs it for the reference's
+    default ``KrylovJL_GMRES()`` (convergent_born.jl:197, 743-750): warm start
+    ``r0 = M (b - A x0)`` (one application outside the iteration count),
+    modified Gram-Schmidt Arnoldi, Givens rotations, stop after an Arnoldi step
+    when the residual estimate ``|z_{k+1}| <= eps`` (solved; Krylov.jl's absolute
+    ``|z| + 1 <= 1`` is left out as for BiCGStab),
+    ``h_{k+1,k} <= btol`` (breakdown; Krylov.jl's default ``eps(T)^(3/4)``), or
+    ``maxiters`` Arnoldi steps in total.  ``restart = m > 0`` runs GMRES(m): after
+    m steps x is updated and the residual recomputed with one more
+    application.  ``iters`` counts Arnoldi steps (Krylov.jl ``niter``).
     """
-    A = lambda v: cbs_precond(cbs_operator(v, V, gp), V, eps_imag)
-    bh = cbs_precond(b, V, eps_imag)
+    A, bh = _left_system(b, V, eps_imag, gp)
+    if btol is None:
+        btol = np.finfo(np.float64).eps ** 0.75
     bnorm = _nrm(bh)
     st = SolveStats()
     x = x0.copy(order="F")
     if bnorm == 0.0:
         st.converged, st.resid = True, 0.0
         return np.zeros_like(x), st
-    thr = None
+    r = bh - A(x)
+    st.applications += 1
+    beta = _nrm(r)
+    st.history.append(beta / bnorm)
+    thr = _stop_threshold(rule, tol, abstol, bnorm, beta)
+    if beta == 0.0 or beta <= thr:
+        st.converged, st.resid = True, beta / bnorm
+        return x, st
+    first = True
     while True:
-        r = bh - A(x)
-        st.iters += 1
-        beta = _nrm(r)
-        st.history.append(beta / bnorm)
-        if thr is None:
-            thr = _stop_threshold(rule, tol, abstol, bnorm, beta)
-        if beta <= thr:
-            st.converged, st.resid = True, beta / bnorm
-            return x, st
-        m = restart if restart > 0 else maxiters
+        if not first:  # restart: the true residual of the updated x
+            r = bh - A(x)
+            st.applications += 1
+            beta = _nrm(r)
+        first = False
+        left = maxiters - st.iters
+        m = min(restart, left) if restart > 0 else left
         Vb = [r / beta]
         H = np.zeros((m + 1, m), dtype=np.complex128)
         cs = np.zeros(m, dtype=np.float64)
@@ -649,47 +704,42 @@ def solve_gmres(b, x0, V, eps_imag, gp, tol, maxiters, rule="rhs", abstol=0.0, r
         g = np.zeros(m + 1, dtype=np.complex128)
         g[0] = beta
         k = 0
-        done = False
-        while k < m and st.iters < maxiters:
+        solved = breakdown = False
+        while True:
             w = A(Vb[k])
+            st.applications += 1
             st.iters += 1
-            for i in range(k + 1):
+            for i in range(k + 1):  # modified Gram-Schmidt
                 H[i, k] = np.vdot(Vb[i], w)
                 w = w - H[i, k] * Vb[i]
             hnext = _nrm(w)
             H[k + 1, k] = hnext
-            for i in range(k):  # apply the previous rotations to column k
+            for i in range(k):  # the previous rotations on column k
                 t = cs[i] * H[i, k] + sn[i] * H[i + 1, k]
                 H[i + 1, k] = -np.conj(sn[i]) * H[i, k] + cs[i] * H[i + 1, k]
                 H[i, k] = t
-            a, bb = H[k, k], H[k + 1, k]
-            den = math.sqrt(abs(a) ** 2 + abs(bb) ** 2)
-            if den == 0.0:
-                cs[k], sn[k] = 1.0, 0.0
-            elif abs(a) == 0.0:
-                cs[k], sn[k] = 0.0, 1.0
-            else:
-                cs[k] = abs(a) / den
-                sn[k] = (a / abs(a)) * np.conj(bb) / den
-            H[k, k] = cs[k] * a + sn[k] * bb
+            cs[k], sn[k] = _givens(H[k, k], H[k + 1, k])
+            H[k, k] = cs[k] * H[k, k] + sn[k] * H[k + 1, k]
             H[k + 1, k] = 0.0
             g[k + 1] = -np.conj(sn[k]) * g[k]
             g[k] = cs[k] * g[k]
             resid = abs(g[k + 1])
             st.history.append(resid / bnorm)
             k += 1
-            if resid <= thr or hnext == 0.0:
-                done = resid <= thr or hnext == 0.0
+            solved = resid <= thr
+            breakdown = hnext <= btol
+            if solved or breakdown or k >= m:
                 break
             Vb.append(w / hnext)
-        y = np.linalg.solve(np.triu(H[:k, :k]), g[:k])
+        y = np.zeros(k, dtype=np.complex128)
+        for i in range(k - 1, -1, -1):  # back substitution R y = g
+            acc = g[i] - H[i, i + 1:k] @ y[i + 1:k]
+            y[i] = 0.0 if abs(H[i, i]) <= btol else acc / H[i, i]
         for i in range(k):
             x = x + y[i] * Vb[i]
-        if done:
-            st.converged, st.resid = True, abs(g[k]) / bnorm
-            return x, st
-        if st.iters >= maxiters:
-            st.resid = abs(g[k]) / bnorm
+        st.resid = abs(g[k]) / bnorm
+        if solved or breakdown or st.iters >= maxiters:
+            st.converged, st.breakdown = bool(solved), bool(breakdown and not solved)
             return x, st
 
 
@@ -753,6 +803,46 @@ def solve(s: Solver, sources, eps_r: np.ndarray, eps_bg: float, method: str = "g
     E_tot = x + E_inc  # :596 (quirk Q12)
     H_tot = Hs + H_inc
     return Result(crop_to_roi(E_tot, s), crop_to_roi(H_tot, s), x, Hs, st, init, rhs, S)
+
+
+# ---------------------------------------------------------------------------
+# Field analysis -- src/fields/electromagnetic_field.jl:112-235
+# ---------------------------------------------------------------------------
+
+EPS0_FIELD = 8.854187817e-12  # electromagnetic_field.jl:113
+MU0_FIELD = 4e-7 * math.pi     # :114
+
+
+def field_energy(E: np.ndarray, H: np.ndarray, resolution) -> float:
+    """(eps0 sum|E|^2 + mu0 sum|H|^2) / 2 times the voxel volume (:112-127)."""
+    e = float((E.real ** 2 + E.imag ** 2).sum())
+    h = float((H.real ** 2 + H.imag ** 2).sum())
+    return (EPS0_FIELD * e + MU0_FIELD * h) / 2 * float(np.prod(resolution))
+
+
+def poynting_vector(E: np.ndarray, H: np.ndarray) -> np.ndarray:
+    """Complex S = E x conj(H), component by component as at :135-154."""
+    Hc = np.conj(H)
+    S = np.empty_like(E)
+    for c in range(3):
+        a, b = (c + 1) % 3, (c + 2) % 3
+        S[..., c] = E[..., a] * Hc[..., b] - E[..., b] * Hc[..., a]
+    return np.asfortranarray(S)
+
+
+def field_intensity(E: np.ndarray, component: str = "total") -> np.ndarray:
+    """|E|^2 summed over the components (``:total``) or of one (:162-180)."""
+    a = E.real ** 2 + E.imag ** 2
+    if component == "total":
+        return np.asfortranarray(a[..., 0] + a[..., 1] + a[..., 2])
+    return np.asfortranarray(a[..., "xyz".index(component)])
+
+
+def extract_plane(F: np.ndarray, plane_axis: int, plane_index: int) -> np.ndarray:
+    """The 1-based plane ``plane_index`` normal to ``plane_axis``, kept 4-D (:192-235)."""
+    idx = [slice(None)] * 4
+    idx[plane_axis - 1] = slice(plane_index - 1, plane_index)
+    return np.asfortranarray(F[tuple(idx)])
 
 
 # ---------------------------------------------------------------------------

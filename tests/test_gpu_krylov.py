@@ -1,8 +1,10 @@
 """GPU parity of the on-device Krylov solvers (BiCGStab, GMRES) with the oracle.
 
-* FP64, same algorithm and fixed operator-application budget: BiCGStab
-  iterates match to 1e-9; GMRES (device CGS2 vs oracle MGS, equal in exact
-  arithmetic) to 1e-9;
+* FP64, same algorithm and a fixed budget of Krylov.jl iterations (BiCGStab:
+  two operator applications each, GMRES: one Arnoldi step each; the warm
+  start's residual outside the count): BiCGStab iterates match to 1e-9; GMRES
+  (device CGS2 vs oracle MGS, equal in exact arithmetic) to 1e-9; iteration
+  and application counts match exactly;
 * converged FP64 solves (tol 1e-10) match the oracle's converged solution
   within the error the residual allows;
 * FP32 + FP64 refinement with each Krylov inner solver meets an FP64-verified
@@ -38,11 +40,13 @@ def _bc(oracle, absorb):
 def test_bicgstab_fixed_budget_fp64(fm, oracle, absorb, graph):
     print(f"case bicgstab fixed absorb={absorb} graph={graph}", flush=True)
     init, S, rhs, prob = _setup(fm, oracle, (16, 12, 8), _bc(oracle, absorb), seed=31)
-    budget = 15  # 1 + 7 iterations of 2 applications
+    budget = 7  # iterations: 2 * 7 + 1 operator applications
     x_o, st_o = oracle.solve_bicgstab(rhs, rhs, init.potential, init.eps_imag, init.green, 0.0, budget)
     E, _, st = prob.solve(S, fm.make_opts(method="bicgstab", tol=0.0, maxiters=budget,
                                           use_graph=graph, check_every=3), want_h=False)
     assert st.iters == st_o.iters == budget
+    assert st.applications == st_o.applications == 2 * budget + 1
+    assert st.voxel_iters == st.applications * int(np.prod(init.potential.shape))
     assert rel(E, x_o) <= 1e-9
     assert st.resid == pytest.approx(st_o.resid, rel=1e-8)
 
@@ -58,6 +62,8 @@ def test_gmres_fixed_budget_fp64(fm, oracle, absorb, restart):
     E, _, st = prob.solve(S, fm.make_opts(method="gmres", tol=0.0, maxiters=budget,
                                           gmres_restart=restart), want_h=False)
     assert st.iters == st_o.iters == budget
+    # one application per Arnoldi step and one per cycle start (5 + 5 + 4 steps)
+    assert st.applications == st_o.applications == budget + (1 if restart == 0 else 3)
     assert rel(E, x_o) <= 1e-9
     assert st.resid == pytest.approx(st_o.resid, rel=1e-7)
 
@@ -67,9 +73,10 @@ def test_krylov_converged_fp64(fm, oracle, method):
     print(f"case {method} converged fp64", flush=True)
     init, S, rhs, prob = _setup(fm, oracle, (32, 32, 16), oracle.PeriodicBC(), seed=33)
     fn = oracle.solve_bicgstab if method == "bicgstab" else oracle.solve_gmres
-    x_o, st_o = fn(rhs, rhs, init.potential, init.eps_imag, init.green, 1e-10, 2000)
+    budget = 1000 if method == "bicgstab" else 400  # full GMRES: at most 400 basis vectors
+    x_o, st_o = fn(rhs, rhs, init.potential, init.eps_imag, init.green, 1e-10, budget)
     assert st_o.converged
-    E, _, st = prob.solve(S, fm.make_opts(method=method, tol=1e-10, maxiters=2000, use_graph=True,
+    E, _, st = prob.solve(S, fm.make_opts(method=method, tol=1e-10, maxiters=budget, use_graph=True,
                                           check_every=8), want_h=False)
     assert st.converged == 1
     r_true = oracle.true_residual(E, rhs, init.potential, init.eps_imag, init.green)
@@ -84,7 +91,8 @@ def test_krylov_fp32_refine(fm, oracle, method):
     init, S, rhs, prob = _setup(fm, oracle, (32, 32, 16), oracle.PeriodicBC(), seed=34,
                                 precision="fp32")
     E, _, st = prob.solve(S, fm.make_opts(method=method, precision="fp32_refine", tol=1e-6,
-                                          maxiters=3000, use_graph=True, check_every=4),
+                                          maxiters=3000 if method == "bicgstab" else 400,
+                                          use_graph=True, check_every=4),
                           want_h=False)
     r_true = oracle.true_residual(E, rhs, init.potential, init.eps_imag, init.green)
     assert st.converged == 1
@@ -121,9 +129,23 @@ def test_multi_rhs_equals_single(fm, oracle):
     rng = np.random.default_rng(0)
     Ss = [S, S * np.exp(1j * 0.3), S * rng.standard_normal(S.shape)]
     for method in ("born", "bicgstab", "gmres"):
-        o = fm.make_opts(method=method, tol=1e-8, maxiters=600)
+        o = fm.make_opts(method=method, tol=1e-8, maxiters=600 if method != "gmres" else 400)
         Es, _, sts = prob.solve_multi(Ss, o, want_h=False)
         for k, Sk in enumerate(Ss):
             E1, _, st1 = prob.solve(Sk, o, want_h=False)
             assert sts[k].iters == st1.iters, method
             assert np.array_equal(Es[k], E1), method
+
+
+def test_gmres_basis_limit(fm, oracle):
+    """Full GMRES keeps one basis vector per iteration, at most 400: a larger
+    maxiters without a restart length is refused (CBS_EINVAL), as is a restart
+    length above 400; GMRES(400) with more iterations runs."""
+    print("case gmres basis limit", flush=True)
+    init, S, rhs, prob = _setup(fm, oracle, (8, 8, 8), oracle.PeriodicBC(), seed=36)
+    for kw in (dict(maxiters=401), dict(maxiters=20, gmres_restart=401)):
+        with pytest.raises(ValueError, match="400 basis vectors"):  # CBS_EINVAL
+            prob.solve(S, fm.make_opts(method="gmres", tol=1e-6, **kw), want_h=False)
+    _, _, st = prob.solve(S, fm.make_opts(method="gmres", tol=1e-6, maxiters=1000,
+                                          gmres_restart=400), want_h=False)
+    assert st.converged == 1

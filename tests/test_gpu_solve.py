@@ -149,3 +149,28 @@ def test_born_mixed_precision(fm, oracle, precision):
         assert abs(st.resid_true_fp64 - r_true) <= 1e-3 * r_true + 1e-14
         assert st.iters_fp64 >= 2
     assert rel(E, x_o) <= 50 * tol
+
+
+@pytest.mark.parametrize("method", ["born", "bicgstab"])
+def test_fp32_refine_tolerance_zero_runs_its_budget(fm, oracle, method):
+    """Tolerance 0 with FP32+refine is a fixed budget: the correction solves
+    run at the FP32 floor until maxiters (inner iterations plus FP64 residual
+    evaluations) ends the solve, and the FP64 residual keeps falling."""
+    print(f"case fp32_refine tol 0 {method}", flush=True)
+    init, S, rhs, prob = _problem_and_rhs(fm, oracle, (16, 16, 12), oracle.PeriodicBC(), seed=24,
+                                          precision="fp32")
+    budget = 60
+    E, _, st = prob.solve(S, fm.make_opts(method=method, precision="fp32_refine", tol=0.0,
+                                          maxiters=budget), want_h=False)
+    assert st.converged == 0
+    # an inner solve that stops at the floor one short of the budget leaves no
+    # room for another correction and the final FP64 evaluation
+    assert budget - 1 <= st.iters <= budget
+    assert st.refine_steps >= 1 and st.iters_fp64 == st.refine_steps + 1
+    per = 2 if method == "bicgstab" else 1
+    assert st.applications == st.iters_fp64 + per * (st.iters - st.iters_fp64)
+    r_true = oracle.true_residual(E, rhs, init.potential, init.eps_imag, init.green)
+    assert st.resid_true_fp64 == pytest.approx(r_true, rel=1e-6)
+    # 60 iterations of a contraction reduce the residual well below the start
+    x1, st1 = oracle.solve_born(rhs, rhs, init.potential, init.eps_imag, init.green, 0.0, 1)
+    assert r_true < 0.25 * st1.resid

@@ -56,7 +56,7 @@ int main(void) {
   F(cbs_solve_opts, compute_h) F(cbs_solve_opts, h_scale_re) F(cbs_solve_opts, h_scale_im)
   F(cbs_solve_stats, iters) F(cbs_solve_stats, iters_fp64) F(cbs_solve_stats, refine_steps) F(cbs_solve_stats, converged)
   F(cbs_solve_stats, resid) F(cbs_solve_stats, resid_true_fp64) F(cbs_solve_stats, time_ms) F(cbs_solve_stats, bytes_moved)
-  F(cbs_solve_stats, voxel_iters)
+  F(cbs_solve_stats, voxel_iters) F(cbs_solve_stats, applications)
   F(cbs_request, desc) F(cbs_request, V) F(cbs_request, S) F(cbs_request, opts)
   printf("cbs_scene_desc %zu\ncbs_plane_wave %zu\ncbs_scene_info %zu\n", sizeof(cbs_scene_desc),
          sizeof(cbs_plane_wave), sizeof(cbs_scene_info));
@@ -293,3 +293,45 @@ def test_field_analysis_plane_wave(fm):
     p = fm.extract_plane(f, 3, 4)
     assert p.E.shape == (6, 5, 1, 3) and np.array_equal(p.E[:, :, 0], E[:, :, 3])
     assert np.array_equal(fm.extract_plane(f, 1, 2).H[0], H[1])
+
+
+def test_refine_inner_tol(tmp_path):
+    """The inner tolerance rule of FP32+refine (csrc/cbs_refine.h), compiled
+    on its own with g++: equal inner reductions above the FP32 floor 9e-4, and
+    the floor for a need of zero (tolerance 0) or a need that is not finite."""
+    import shutil
+    import subprocess
+
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ not found")
+    src = tmp_path / "t.cpp"
+    src.write_text(r'''
+#include <cstdio>
+#include <limits>
+#include "cbs_refine.h"
+int main() {
+  const double inf = std::numeric_limits<double>::infinity();
+  const double nan = std::numeric_limits<double>::quiet_NaN();
+  const double v[] = {0.0, 1e-1, 1e-3, 1e-6, 1e-12, 1e-300, 5e-324, inf, nan, -1.0};
+  for (double x : v) std::printf("%.17g\n", cbs::refine_inner_tol(x, 0.0));
+  std::printf("%.17g\n", cbs::refine_inner_tol(0.0, 0.25));
+  return 0;
+}
+''')
+    exe = tmp_path / "t"
+    inc = ROOT / "frequencymaxwell.jl_amd" / "csrc"
+    subprocess.run([gxx, "-std=c++17", "-O2", f"-I{inc}", str(src), "-o", str(exe)], check=True)
+    out = [float(x) for x in subprocess.run([str(exe)], check=True, capture_output=True,
+                                            text=True).stdout.split()]
+    floor = 9e-4
+    assert out[0] == floor                               # tolerance 0
+    assert out[1] == pytest.approx(0.09)                 # one inner solve, 0.9 need
+    assert out[2] == pytest.approx(0.9e-3)               # at the floor: one inner solve
+    for need, eta in zip((1e-6, 1e-12, 1e-300, 5e-324), out[3:7]):
+        t = 0.9 * need
+        m = math.ceil(math.log(t) / math.log(floor))
+        assert m >= 2 and eta == pytest.approx(t ** (1.0 / m), rel=1e-12)
+        assert floor <= eta < 1.0                       # m equal reductions above the floor
+    assert out[7] == floor and out[8] == floor and out[9] == floor
+    assert out[10] == 0.25                               # a caller's inner_tol wins

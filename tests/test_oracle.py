@@ -352,3 +352,204 @@ def test_linearity_in_source(oracle):
     a, b = 0.3 - 1.2j, 2.0 + 0.5j
     G = lambda f: o.green_conv(f, init.green)
     assert rel(G(a * u + b * v), a * G(u) + b * G(v)) < 1e-13
+
+
+# ---------------------------------------------------------------------------
+# Krylov iterates at fixed budgets: scipy and brute-force characterisations
+# ---------------------------------------------------------------------------
+def _krylov_case(oracle, absorb, grid=(6, 5, 4), seed=11):
+    """Tiny problem with the dense left-preconditioned operator A^ = gamma A,
+    b^ = gamma b and the reference's warm start x0 = rhs (convergent_born.jl:727-728)."""
+    o = oracle
+    bc = o.AbsorbingBC(thickness=100e-9, attenuation_thickness=100e-9) if absorb else o.PeriodicBC()
+    s, eps, init = oracle_setup(o, grid, bc, seed)
+    V, ei, gp = init.potential, init.eps_imag, init.green
+    S = random_field(V.shape, seed + 1)
+    rhs = o.green_conv(S, gp)
+    # unit-scale right-hand side (G ~ 1/k^2 leaves b ~ 1e-14; scipy's breakdown
+    # tests are absolute); the iterations are scale-equivariant
+    rhs = np.asfortranarray(rhs * (math.sqrt(rhs.size) / np.linalg.norm(rhs)))
+    Ad = o.dense_operator(V, gp)
+    gam = (1j / ei) * np.tile(o.flat(V), 3)
+    Ah = gam[:, None] * Ad
+    bh = gam * o.flat(rhs)
+    return o, V, ei, gp, rhs, Ah, bh
+
+
+def _scipy_op(Ah):
+    import scipy.sparse.linalg as sla
+    n = Ah.shape[0]
+    return sla.LinearOperator((n, n), matvec=lambda v: Ah @ v, dtype=complex)
+
+
+def _orth_krylov(M, v, k):
+    """Orthonormal basis of span{v, M v, ..., M^(k-1) v} (Arnoldi, two passes of
+    Gram-Schmidt per vector)."""
+    Q = np.zeros((v.size, k), dtype=complex)
+    Q[:, 0] = v / np.linalg.norm(v)
+    for j in range(1, k):
+        w = M @ Q[:, j - 1]
+        for _ in range(2):
+            w = w - Q[:, :j] @ (Q[:, :j].conj().T @ w)
+        Q[:, j] = w / np.linalg.norm(w)
+    return Q
+
+
+@pytest.mark.parametrize("absorb", [False, True])
+def test_bicgstab_iterates_match_scipy(oracle, absorb):
+    """Shadow r^ = r0 (scipy's choice): the oracle's k-th BiCGStab iterate equals
+    scipy.sparse.linalg.bicgstab's after k iterations on the same explicit
+    left-preconditioned system, for k = 1..8."""
+    import scipy.sparse.linalg as sla
+    o, V, ei, gp, rhs, Ah, bh = _krylov_case(oracle, absorb)
+    op = _scipy_op(Ah)
+    for k in range(1, 9):
+        xs, info = sla.bicgstab(op, bh, x0=o.flat(rhs).copy(), rtol=1e-300, atol=0.0, maxiter=k)
+        assert info == k
+        x, st = o.solve_bicgstab(rhs, rhs, V, ei, gp, 0.0, k, shadow="r0")
+        assert st.iters == k and st.applications == 2 * k + 1 and not st.converged
+        assert rel(o.flat(x), xs) < 1e-10, k
+
+
+@pytest.mark.parametrize("absorb", [False, True])
+def test_gmres_iterates_match_scipy(oracle, absorb):
+    """Full GMRES after k Arnoldi steps (scipy: one cycle of restart k) and
+    GMRES(4) over two and three whole cycles equal scipy.sparse.linalg.gmres."""
+    import scipy.sparse.linalg as sla
+    o, V, ei, gp, rhs, Ah, bh = _krylov_case(oracle, absorb)
+    op = _scipy_op(Ah)
+    x0 = o.flat(rhs)
+    for k in range(1, 9):
+        xs, _ = sla.gmres(op, bh, x0=x0.copy(), rtol=1e-300, atol=0.0, restart=k, maxiter=1,
+                          callback=lambda r: None, callback_type="pr_norm")
+        x, st = o.solve_gmres(rhs, rhs, V, ei, gp, 0.0, k)
+        assert st.iters == k and st.applications == k + 1
+        assert rel(o.flat(x), xs) < 1e-10, k
+    for cycles in (2, 3):
+        xs, _ = sla.gmres(op, bh, x0=x0.copy(), rtol=1e-300, atol=0.0, restart=4, maxiter=cycles,
+                          callback=lambda r: None, callback_type="pr_norm")
+        x, st = o.solve_gmres(rhs, rhs, V, ei, gp, 0.0, 4 * cycles, restart=4)
+        # one application per Arnoldi step, one for r0 and one per restart
+        assert st.iters == 4 * cycles and st.applications == 4 * cycles + cycles
+        assert rel(o.flat(x), xs) < 1e-10, cycles
+
+
+@pytest.mark.parametrize("absorb", [False, True])
+def test_gmres_iterates_minimise_residual(oracle, absorb):
+    """The k-th full-GMRES iterate is x0 + W y with y the least-squares minimiser
+    of ||r0 - A^ W y|| over an orthonormal basis W of the Krylov space
+    K_k(A^, r0); restarted GMRES(3) repeats that from each cycle's residual."""
+    o, V, ei, gp, rhs, Ah, bh = _krylov_case(oracle, absorb)
+    x0 = o.flat(rhs)
+    r0 = bh - Ah @ x0
+    for k in range(1, 9):
+        W = _orth_krylov(Ah, r0, k)
+        y = np.linalg.lstsq(Ah @ W, r0, rcond=None)[0]
+        x, _ = o.solve_gmres(rhs, rhs, V, ei, gp, 0.0, k)
+        assert rel(o.flat(x), x0 + W @ y) < 1e-9, k
+    xb = x0.copy()
+    for _ in range(3):
+        r = bh - Ah @ xb
+        W = _orth_krylov(Ah, r, 3)
+        xb = xb + W @ np.linalg.lstsq(Ah @ W, r, rcond=None)[0]
+    x, st = o.solve_gmres(rhs, rhs, V, ei, gp, 0.0, 9, restart=3)
+    assert st.iters == 9 and rel(o.flat(x), xb) < 1e-9
+
+
+@pytest.mark.parametrize("absorb", [False, True])
+@pytest.mark.parametrize("shadow", ["b", "r0"])
+def test_bicgstab_iterates_petrov_galerkin(oracle, absorb, shadow):
+    """BiCGStab characterised without its recurrences (van der Vorst 1992):
+    r_k = Omega_k(A^) phi_k(A^) r0, where phi_k (degree k, phi_k(0) = 1) is the
+    BiCG residual polynomial, fixed by <(A^H)^i c, phi_k(A^) r0> = 0 for i < k,
+    and Omega_k = prod_j (1 - omega_j t) with omega_j minimising
+    ||(1 - omega A^) s_j||, s_j = Omega_(j-1)(A^) phi_j(A^) r0.  The k-th iterate
+    is x0 + A^-1 (r0 - r_k).  Both shadow vectors c: b (Krylov.jl's default) and
+    r0."""
+    o, V, ei, gp, rhs, Ah, bh = _krylov_case(oracle, absorb)
+    x0 = o.flat(rhs)
+    r0 = bh - Ah @ x0
+    c = x0.copy() if shadow == "b" else r0.copy()
+    AhH = Ah.conj().T
+    omegas = []
+    for k in range(1, 7):
+        # phi_k(A^) r0 = r0 + Q y over an orthonormal basis Q of A^ K_k(A^, r0);
+        # the conditions are taken against an orthonormal basis P of K_k(A^H, c)
+        Q, _ = np.linalg.qr(Ah @ _orth_krylov(Ah, r0, k))
+        P = _orth_krylov(AhH, c, k)
+        y = np.linalg.solve(P.conj().T @ Q, -(P.conj().T @ r0))
+        s = r0 + Q @ y
+        for w in omegas:
+            s = s - w * (Ah @ s)
+        t = Ah @ s
+        w = np.vdot(t, s) / np.vdot(t, t)
+        omegas.append(w)
+        rk = s - w * t
+        xk = x0 + np.linalg.solve(Ah, r0 - rk)
+        x, st = o.solve_bicgstab(rhs, rhs, V, ei, gp, 0.0, k, shadow=shadow)
+        assert st.iters == k
+        assert rel(o.flat(x), xk) < 1e-8, k
+
+
+def test_krylov_counts_and_budget(oracle):
+    """Krylov.jl counting: maxiters bounds iterations (BiCGStab: two
+    applications each; GMRES: one Arnoldi step each), the warm-start residual
+    is an application outside the count, and a converged solve stops at the
+    first iteration whose residual meets the threshold."""
+    o, V, ei, gp, rhs, Ah, bh = _krylov_case(oracle, True)
+    x, st = o.solve_bicgstab(rhs, rhs, V, ei, gp, 1e-9, 500)
+    assert st.converged and st.applications == 2 * st.iters + 1
+    assert st.history[-1] <= 1e-9 < st.history[-2]
+    x, st = o.solve_gmres(rhs, rhs, V, ei, gp, 1e-9, 500)
+    assert st.converged and st.applications == st.iters + 1
+    assert st.history[-1] <= 1e-9 < st.history[-2]
+    # the GMRES estimate is the true residual of the returned iterate
+    assert abs(o.true_residual(x, rhs, V, ei, gp) - st.resid) < 1e-6 * st.resid
+    x, st = o.solve_gmres(rhs, rhs, V, ei, gp, 1e-9, 500, restart=20)
+    cycles = (st.iters + 19) // 20
+    assert st.converged and st.applications == st.iters + cycles
+
+
+# ---------------------------------------------------------------------------
+# field analysis (electromagnetic_field.jl:112-235)
+# ---------------------------------------------------------------------------
+def test_field_analysis_closed_forms(oracle):
+    o = oracle
+    n = (5, 4, 6)
+    res = (30e-9, 40e-9, 50e-9)
+    # constant fields: energy = (eps0 N |E|^2 + mu0 N |H|^2) / 2 * voxel volume
+    E = np.zeros(n + (3,), complex)
+    H = np.zeros(n + (3,), complex)
+    E[..., 1] = 2.0 - 1.0j
+    H[..., 2] = 0.5j
+    N = np.prod(n)
+    vol = res[0] * res[1] * res[2]
+    want = (8.854187817e-12 * N * 5.0 + 4e-7 * math.pi * N * 0.25) / 2 * vol
+    assert o.field_energy(E, H, res) == pytest.approx(want, rel=1e-14)
+    # right-hand rule: y x z = x, z x x = y, x x y = z (conj(H) of real unit H)
+    for a, b, c in ((1, 2, 0), (2, 0, 1), (0, 1, 2)):
+        Eu = np.zeros(n + (3,), complex)
+        Hu = np.zeros(n + (3,), complex)
+        Eu[..., a] = 1.0
+        Hu[..., b] = 1.0
+        S = o.poynting_vector(Eu, Hu)
+        assert np.all(S[..., c] == 1.0) and np.all(np.delete(S, c, axis=3) == 0.0)
+    # plane wave x-polarised along z: S_z = |A|^2 / Z, intensity |A|^2 in x
+    z = np.arange(n[2]) * 50e-9
+    A = 0.7 * np.exp(2j * np.pi * z / 500e-9)
+    Z = 377.0 / 1.33
+    E = np.zeros(n + (3,), complex)
+    H = np.zeros(n + (3,), complex)
+    E[..., 0] = A
+    H[..., 1] = A / Z
+    S = o.poynting_vector(E, H)
+    assert np.allclose(S[..., 2], 0.49 / Z, rtol=1e-14, atol=0)
+    assert np.allclose(o.field_intensity(E), 0.49, rtol=1e-14)
+    assert np.allclose(o.field_intensity(E, "x"), 0.49, rtol=1e-14)
+    assert np.all(o.field_intensity(E, "y") == 0) and np.all(o.field_intensity(E, "z") == 0)
+    # planes: 1-based index along each axis
+    F = random_field(n, 3)
+    assert np.array_equal(o.extract_plane(F, 1, 2)[0], F[1])
+    assert np.array_equal(o.extract_plane(F, 2, 4)[:, 0], F[:, 3])
+    assert np.array_equal(o.extract_plane(F, 3, 6)[:, :, 0], F[:, :, 5])
+    assert o.extract_plane(F, 3, 1).shape == (5, 4, 1, 3)
