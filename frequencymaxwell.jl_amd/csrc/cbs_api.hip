@@ -7,6 +7,7 @@
 
 #include "cbs_engine.h"
 #include "cbs_field.h"
+#include "cbs_noalloc.h"
 #include "cbs_scene.h"
 #include "cbs_serve.h"
 #include "cbs_solve.h"
@@ -123,6 +124,7 @@ cbs_status cbs_ctx_create(cbs_ctx** out, int32_t device, uint64_t hbm_budget_byt
     c->impl.budget = hbm_budget_bytes;
     CBS_HIP(hipSetDevice(device));
     CBS_HIP(hipStreamCreateWithFlags(&c->impl.stream, hipStreamNonBlocking));
+    noalloc_init(device);  // this device's copy of the allocation-free scratch
     *out = c.release();
   });
 }
@@ -200,6 +202,16 @@ cbs_status cbs_operator_apply(cbs_problem* pp, double* y, const double* x, doubl
       operator_apply_dev<double>(p, fy.d64.as<Cx<double>>(), fx.d64.as<Cx<double>>(), a_re, a_im,
                                  b_re, b_im, s);
     to_host(p, fy, y, nc, f32, s);
+  });
+}
+
+cbs_status cbs_operator_apply_noalloc(cbs_problem* pp, void* y, const void* x, double a_re,
+                                      double a_im, double b_re, double b_im) {
+  return guarded([&] {
+    CBS_REQUIRE(pp && y && x, "NULL argument");
+    Problem& p = pp->impl;
+    set_device(*p.ctx);
+    operator_apply_noalloc(p, y, x, a_re, a_im, b_re, b_im);
   });
 }
 

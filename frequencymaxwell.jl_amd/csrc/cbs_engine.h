@@ -142,10 +142,16 @@ struct Pipe {
   std::vector<int> gmem[4][kMaxGroups];
   std::vector<int> gbs[4][kMaxGroups];
   void build(Ctx* c, hipStream_t s);
+  // build with the device tables placed at `dev` (cap bytes) instead of
+  // buffers of its own: allocates nothing on the device
+  void build_in(void* dev, size_t cap, hipStream_t s);
   void launch(hipStream_t s) const { launch_pipeline<T>(L, s); }
   // the launch restricted to the first n entries of h (every group launches
   // only the blocks of its members below n)
   PipeLaunch<T> prefix(int n) const;
+
+ private:
+  void build_impl(Ctx* c, void* dev, size_t cap, hipStream_t s);
 };
 
 // device arrays describing the RHS of a Born/Krylov solve (for finalize kernels)

@@ -193,6 +193,15 @@ template DevProb<float> make_desc<float>(Problem&, int, int, int, const Cx<float
 
 template <class T>
 void Pipe<T>::build(Ctx* c, hipStream_t s) {
+  build_impl(c, nullptr, 0, s);
+}
+template <class T>
+void Pipe<T>::build_in(void* dev, size_t cap, hipStream_t s) {
+  CBS_REQUIRE(dev, "no device memory for the launch tables");
+  build_impl(nullptr, dev, cap, s);
+}
+template <class T>
+void Pipe<T>::build_impl(Ctx* c, void* dev, size_t cap, hipStream_t s) {
   const int np = (int)h.size();
   CBS_REQUIRE(np > 0, "empty pipeline");
   for (auto& d : h) count_tiles(d);  // the fused x pass has tiles of its own
@@ -252,12 +261,22 @@ void Pipe<T>::build(Ctx* c, hipStream_t s) {
       grps.push_back(g);
     }
   }
-  d_probs.ensure(c, hp.size() * sizeof(DevProb<T>));
-  d_bstart.ensure(c, hb.size() * sizeof(int));
-  CBS_HIP(hipMemcpyAsync(d_probs.p, hp.data(), hp.size() * sizeof(DevProb<T>),
-                         hipMemcpyHostToDevice, s));
-  CBS_HIP(hipMemcpyAsync(d_bstart.p, hb.data(), hb.size() * sizeof(int), hipMemcpyHostToDevice,
-                         s));
+  const size_t pb = hp.size() * sizeof(DevProb<T>), bb = hb.size() * sizeof(int);
+  DevProb<T>* dp;
+  int* db;
+  if (dev) {
+    const size_t pb_al = (pb + 255) / 256 * 256;
+    CBS_REQUIRE(pb_al + bb <= cap, "launch tables exceed the memory given for them");
+    dp = reinterpret_cast<DevProb<T>*>(dev);
+    db = reinterpret_cast<int*>(static_cast<char*>(dev) + pb_al);
+  } else {
+    d_probs.ensure(c, pb);
+    d_bstart.ensure(c, bb);
+    dp = d_probs.as<DevProb<T>>();
+    db = d_bstart.as<int>();
+  }
+  CBS_HIP(hipMemcpyAsync(dp, hp.data(), pb, hipMemcpyHostToDevice, s));
+  CBS_HIP(hipMemcpyAsync(db, hb.data(), bb, hipMemcpyHostToDevice, s));
   CBS_HIP(hipStreamSynchronize(s));
   for (int pass = 0; pass < 4; ++pass)
     for (int i = 0; i < kMaxGroups; ++i) {
@@ -268,8 +287,8 @@ void Pipe<T>::build(Ctx* c, hipStream_t s) {
     gmem[g.pass][L.ng[g.pass]] = g.mem;
     gbs[g.pass][L.ng[g.pass]].assign(hb.begin() + g.off_b, hb.begin() + g.off_b + g.n + 1);
     PassLaunch<T>& G = L.g[g.pass][L.ng[g.pass]++];
-    G.probs = d_probs.as<DevProb<T>>() + g.off_p;
-    G.bstart = d_bstart.as<int>() + g.off_b;
+    G.probs = dp + g.off_p;
+    G.bstart = db + g.off_b;
     G.nprob = g.n;
     G.blocks = g.blocks;
     G.lds = g.lds;

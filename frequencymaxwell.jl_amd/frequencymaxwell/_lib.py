@@ -35,7 +35,7 @@ EXPORTED = (
     "cbs_wait", "cbs_cancel", "cbs_time_sweeps", "cbs_stream_copy", "cbs_scene_build",
     "cbs_solve_scene", "cbs_submit_scene", "cbs_field_create", "cbs_field_destroy", "cbs_field_get",
     "cbs_field_energy", "cbs_field_intensity", "cbs_field_poynting", "cbs_field_extract_plane",
-    "cbs_solve_scene_field", "cbs_solve_scene_multi",
+    "cbs_solve_scene_field", "cbs_solve_scene_multi", "cbs_operator_apply_noalloc",
 )
 
 
@@ -164,6 +164,8 @@ def _declare(lib):
         "cbs_problem_destroy": (s, [_P]),
         "cbs_green_apply": (s, [_P, _D]),
         "cbs_operator_apply": (s, [_P, _D, _D, C.c_double, C.c_double, C.c_double, C.c_double]),
+        "cbs_operator_apply_noalloc": (s, [_P, C.c_void_p, C.c_void_p, C.c_double, C.c_double,
+                                           C.c_double, C.c_double]),
         "cbs_precond_apply": (s, [_P, _D, _D]),
         "cbs_curl_apply": (s, [_P, _D, C.c_double, C.c_double]),
         "cbs_solve": (s, [_P, _D, _D, _D, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),

@@ -164,6 +164,23 @@ cbs_status cbs_green_apply(cbs_problem *p, double *field_inout);
 cbs_status cbs_operator_apply(cbs_problem *p, double *y, const double *x, double a_re,
                               double a_im, double b_re, double b_im);
 
+/* Allocation-free variant of cbs_operator_apply -- no counterpart in the
+ * reference.  For callers that cannot allocate device memory: code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc.  The
+ * same operator and kernels on DEVICE arrays x and y (3*Nx*Ny*Nz complex in
+ * the problem's working precision: complex FP64 for CBS_FP64 problems,
+ * complex FP32 otherwise); the spectrum workspace and the launch tables live
+ * in a device array of CBS_NOALLOC_SCRATCH_BYTES that the library holds on
+ * every GPU it runs on until the process ends, so the call allocates nothing.
+ * A grid whose workspace does not fit (FP64 with averaging: about 2.8 million
+ * voxels; FP32 or periodic: more) is refused with CBS_EINVAL.  Calls on one GPU
+ * share its scratch array: calls that overlap on the same GPU race and must be
+ * serialised by the caller.  Runs on the context's stream and returns when the
+ * result is in y. */
+#define CBS_NOALLOC_SCRATCH_BYTES 268435456ULL /* 256 MiB */
+cbs_status cbs_operator_apply_noalloc(cbs_problem *p, void *y, const void *x, double a_re,
+                                      double a_im, double b_re, double b_im);
+
 /* ldiv!(y, ::CBSPreconditioner, x): y <- (i/eps) V x.
  * Replaces src/solvers/convergent_born.jl:398-410. */
 cbs_status cbs_precond_apply(cbs_problem *p, double *y, const double *x);
