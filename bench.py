@@ -7,8 +7,8 @@ along +z, absorbing boundaries on every axis (1.6 um = 16 px padding and
 attenuation, sharpness 0.9: padded FFT grid 160x160x96), tolerance 1e-6 on the
 preconditioned residual ||gamma (b - A x)|| / ||gamma b|| (FP64-verified).
 A step is one full solve: FP32 Born sweeps inside an FP64
-iterative-refinement loop, the iteration captured as HIP graphs, inputs
-resident in HBM.  (With periodic boundaries this bead lattice is
+iterative-refinement loop, inputs resident in HBM (sets of 2^20 voxels or more
+launch their sweep kernels directly; smaller ones replay HIP graphs).  (With periodic boundaries this bead lattice is
 near-resonant: no method reaches 1e-6 within 3000 operator applications.)
 
 value        voxel-iterations/s of those steps (operator applications x padded voxels)
@@ -101,7 +101,7 @@ def serve_request(fm, grid, rng, tol):
     return S.scene(sol, src, med)
 
 
-def serve_stream(fm, n_req, rate, max_active, seed, maxiters):
+def serve_stream(fm, n_req, rate, max_active, seed, maxiters, field_every=0):
     """Poisson arrivals (rate per second) of scene solves on grids drawn from
     SERVE_GRIDS with tolerance 1e-4 or 1e-6; returns latency percentiles and
     throughput.  The requests are generated before the clock starts."""
@@ -131,27 +131,42 @@ def serve_stream(fm, n_req, rate, max_active, seed, maxiters):
 
     th = threading.Thread(target=feed)
     th.start()
-    lat, vi, conv, worst = [], 0, 0, 0.0
+    lat, vi, conv, worst, fields = [], 0, 0, 0.0, {}
     try:
         for i in range(n_req):
             while tickets[i] is None:
                 time.sleep(0.001)
-            _, _, st = srv.wait(tickets[i])
+            E, _, st = srv.wait(tickets[i])
             lat.append(st.time_ms)
             vi += st.voxel_iters
             conv += int(st.converged)
             worst = max(worst, st.resid_true_fp64 / reqs[i][1])
+            if field_every and i % field_every == 0:
+                fields[i] = E
     finally:
         th.join()
         srv.stop()
     wall = time.perf_counter() - t0
-    return {
+    out = {
         "requests": n_req, "max_active": max_active, "arrival_rate_per_s": rate,
         "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
         "mean_ms": float(np.mean(lat)), "voxel_iters_per_s": vi / wall,
         "requests_per_s": n_req / wall, "wall_s": wall, "converged": conv,
         "max_resid_over_tol": worst,
     }
+    if fields:
+        # FP64 solves of a sample of the same requests: field error of the served results
+        from frequencymaxwell import engine
+        errs = []
+        for i, E in fields.items():
+            (d, eps, waves, keep), tol = reqs[i]
+            o64 = fm.make_opts(method="born", precision="fp64", tol=tol, maxiters=maxiters,
+                               compute_h=False)
+            E64, _, _, _ = engine.solve_scene(d, eps, waves, o64, want_h=False)
+            errs.append(float(np.linalg.norm(E - E64) / np.linalg.norm(E64)))
+        out["field_err_vs_fp64_max"] = max(errs)
+        out["field_err_sample"] = f"every {field_every}th request ({len(errs)}), FP64 Born to the same tolerance"
+    return out
 
 
 def cell_phantom(fm, grid=(128, 128, 64), seed=4):
@@ -226,21 +241,27 @@ def large_leg(fm, args):
     d, eps, waves, keep = S.scene(sol, src, med)
     out = {"workload": "256x256x128, 20 seeded spheres r 4-16 px n 1.36-1.60 in water, absorbing "
                        "16 px every axis (padded 288x288x160), 532 nm, tol 1e-6 (FP64-verified)"}
+    fields = {}
     for method in ("born", "bicgstab"):
-        for prec in ("fp32_refine", "fp64"):
+        for prec in ("fp64", "fp32_refine"):
             o = fm.make_opts(method=method, precision=prec, tol=1e-6, maxiters=args.maxiters,
                              check_every=args.check_every, use_graph=not args.no_graph)
             t = time.perf_counter()
-            _, _, st, info = engine.solve_scene(d, eps, waves, o)
+            E, _, st, info = engine.solve_scene(d, eps, waves, o)
             wall = time.perf_counter() - t
+            fields[(method, prec)] = E
             # FP32+refine reports the FP64 true residual; a plain FP64 solve's own
             # residual is FP64
             resid = st.resid_true_fp64 if prec == "fp32_refine" else st.resid
             out[f"{method}_{prec}"] = {
                 "solve_ms": st.time_ms, "wall_ms": wall * 1e3, "iters": st.iters,
-                "fp64_iters": st.iters_fp64, "resid_fp64": resid,
-                "converged": bool(st.converged),
+                "applications": st.applications, "fp64_applications": st.iters_fp64,
+                "resid_fp64": resid, "converged": bool(st.converged),
                 "voxel_iters_per_s": st.voxel_iters / (st.time_ms * 1e-3)}
+            # every result against the FP64 Born solution of the same scene
+            ref = fields[("born", "fp64")]
+            out[f"{method}_{prec}"]["field_err_vs_fp64"] = float(
+                np.linalg.norm(E - ref) / np.linalg.norm(ref))
             print(f"[bench] large {method} {prec}: {st.time_ms:.0f} ms, {st.iters} iters, "
                   f"resid {resid:.2e}", file=sys.stderr, flush=True)
     return out
@@ -276,20 +297,44 @@ def batched_leg(fm, args):
         _, _, s1, _ = solve_scene_multi(d, eps, [[w]], o, want_e=False, want_h=False)
         vi1 += s1[0].voxel_iters
     dt1 = time.perf_counter() - t
+    # field error of right-hand side 0 (its batched result equals its single
+    # solve bit for bit) against an FP64 solve to the same tolerance
+    E1, _, _, _ = solve_scene_multi(d, eps, [[waves[0]]], o, want_h=False)
+    d64, eps64, waves64, keep64 = S.scene(sol, srcs[:1], med, precision="fp64")
+    o64 = fm.make_opts(method="born", precision="fp64", tol=1e-5, stop_rule="rhs",
+                       maxiters=args.maxiters, compute_h=False)
+    E64, _, _, _ = solve_scene_multi(d64, eps64, [[waves64[0]]], o64, want_h=False)
+    err = float(np.linalg.norm(E1[0] - E64[0]) / np.linalg.norm(E64[0]))
     return {"workload": f"128x128x64 cell phantom (GRF n 1.33-1.40 + ellipsoid n 1.38), absorbing "
                         f"16 px every axis (padded 160x160x96), "
                         f"{len(waves)} RHS ({args.batch_dirs} directions in NA 1.2 x 2 polarisations), "
                         "FP32+refine Born, tol 1e-5, potential and sources built on the GPU",
             "rhs": len(waves), "ms": dt * 1e3, "voxel_iters_per_s": vi / dt,
+            "field_err_vs_fp64_rhs0": err,
             "converged": sum(int(s.converged) for s in sts),
             "max_resid_fp64": max(s.resid_true_fp64 for s in sts),
             "iters_mean": float(np.mean([s.iters for s in sts])),
+            "applications_mean": float(np.mean([s.applications for s in sts])),
             "one_at_a_time_voxel_iters_per_s": vi1 / dt1}
 
 
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(med, seconds: float):
-    """FP64 Born sweeps of the restatement for about `seconds` seconds."""
+    """FP64 Born sweeps of the restatement for about `seconds` seconds, its FFTs
+    on the host's CPU share (OMP_NUM_THREADS, which the GPU box sets to its
+    one-GPU share of the node; else every core)."""
     from oracle import cbs_oracle as o
+
+    o.WORKERS = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
 
     s = o.Solver(resolution=(50e-9,) * 3, grid_size=med.permittivity.shape,
                  boundary_conditions=o.AbsorbingBC(thickness=1.6e-6, attenuation_thickness=1.6e-6,
@@ -315,7 +360,8 @@ def cpu_baseline(med, seconds: float):
     g = "x".join(str(v) for v in init.potential.shape)
     return {"value": n * nvox / dt, "unit": "voxel-iters/s", "cores": o.WORKERS, "kind": "port",
             "sample": f"{n} FP64 Born sweeps of the padded {g} grid in {dt:.1f} s "
-                      f"(NumPy + scipy.fft restatement, {o.WORKERS} FFT threads)"}
+                      f"(NumPy + scipy.fft restatement, {o.WORKERS} FFT threads)",
+            "host_cpus": os.cpu_count(), "cpu_model": cpu_model()}
 
 
 def read_traffic(summary: str, kernel: str):
@@ -344,7 +390,8 @@ def serving_leg(fm, args):
         print(f"[bench] capacity {cap:.2f} requests/s -> arrivals at {rate:.2f}/s",
               file=sys.stderr, flush=True)
     if args.serve_mode in ("both", "batched"):
-        cb = serve_stream(fm, args.serve_requests, rate, args.serve_active, 7, args.maxiters)
+        cb = serve_stream(fm, args.serve_requests, rate, args.serve_active, 7, args.maxiters,
+                          field_every=args.serve_field_every)
         print(f"[bench] continuous batching: p50 {cb['p50_ms']:.0f} ms p99 {cb['p99_ms']:.0f} ms",
               file=sys.stderr, flush=True)
     if args.serve_mode in ("both", "sequential"):
@@ -368,7 +415,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--check-every", type=int, default=8)
+    ap.add_argument("--check-every", type=int, default=32)
     ap.add_argument("--maxiters", type=int, default=4000)
     ap.add_argument("--method", default="born", choices=["bicgstab", "gmres", "born"])
     ap.add_argument("--no-krylov", action="store_true",
@@ -380,14 +427,16 @@ def main():
                     help="profiler run: only the timed FP32 sweeps and the 1 GiB calibration copy")
     ap.add_argument("--profile-reps", type=int, default=20)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-stream leg")
-    ap.add_argument("--serve-requests", type=int, default=16)
+    ap.add_argument("--serve-requests", type=int, default=100)
+    ap.add_argument("--serve-field-every", type=int, default=10,
+                    help="FP64 field error of every k-th served request (0: none)")
     ap.add_argument("--serve-rate", type=float, default=0.0,
                     help="arrivals per second (0: 80 %% of the capacity measured first)")
     ap.add_argument("--serve-active", type=int, default=8)
     ap.add_argument("--serve-only", action="store_true")
     ap.add_argument("--serve-mode", default="both", choices=["both", "batched", "sequential"])
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-illumination leg")
-    ap.add_argument("--batch-dirs", type=int, default=16, help="illumination directions (x 2 pol)")
+    ap.add_argument("--batch-dirs", type=int, default=64, help="illumination directions (x 2 pol)")
     ap.add_argument("--batch-only", action="store_true")
     ap.add_argument("--no-large", action="store_true",
                     help="skip the large-phantom leg (256x256x128, 20 spheres)")
@@ -441,6 +490,8 @@ def main():
     total = time.perf_counter() - t0
     value = vi / total
     E32, _ = prob.get_fields()
+    # the library replays HIP graphs for sets below 2^20 voxels, else launches directly
+    graphs = (not args.no_graph) and nvox < (1 << 20)
 
     out = {
         "metric": METRIC, "value": value, "unit": "voxel-iters/s", "n_gpus": 1,
@@ -450,14 +501,17 @@ def main():
         "config": {"workload": "128x128x64 grid (padded 160x160x96, absorbing 16 px every axis), "
                                "bead n=1.5 r=16px in water n=1.33, x-pol 500 nm along +z, "
                                "50 nm voxels, tol 1e-6 (FP64-verified)",
-                   "method": f"{args.method} FP32 inner solves + FP64 iterative refinement"
-                             + ("" if args.no_graph else ", hipGraph chunks"),
+                   "method": f"{args.method} FP32 inner solves + FP64 iterative refinement, "
+                             + ("hipGraph chunks" if graphs else "direct kernel launches")
+                             + f" (chunks of {args.check_every})",
                    "nvox": nvox},
         "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-        "iters_per_solve": iters[-1], "fp64_iters_per_solve": last.iters_fp64,
+        "iters_per_solve": iters[-1], "applications_per_solve": last.applications,
+        "fp64_applications_per_solve": last.iters_fp64,
         "resid_fp64": last.resid_true_fp64, "converged": bool(last.converged),
     }
 
+    E64, t64 = None, None
     if not args.no_fp64:
         p64 = fm.CBSProblem(pot, prob.spec, precision="fp64")
         p64.set_source(Sterm)
@@ -469,6 +523,7 @@ def main():
         st64 = p64.run(plain)
         dt = time.perf_counter() - t
         E64, _ = p64.get_fields()
+        t64 = dt
         v64 = st64.voxel_iters / dt
         out["fp64_plain"] = {"value": v64, "ms_per_solve": dt * 1e3, "iters": st64.iters,
                              "converged": bool(st64.converged), "resid": st64.resid,
@@ -486,10 +541,17 @@ def main():
         t = time.perf_counter()
         stk = prob.run(kry)
         dtk = time.perf_counter() - t
+        Ek, _ = prob.get_fields()
         out["bicgstab_fp32_refine"] = {"ms_per_solve": dtk * 1e3, "iters": stk.iters,
-                                       "fp64_iters": stk.iters_fp64, "resid_fp64": stk.resid_true_fp64,
+                                       "applications": stk.applications,
+                                       "fp64_applications": stk.iters_fp64,
+                                       "resid_fp64": stk.resid_true_fp64,
                                        "converged": bool(stk.converged),
                                        "voxel_iters_per_s": stk.voxel_iters / dtk}
+        if E64 is not None:
+            out["bicgstab_fp32_refine"]["field_err_vs_fp64"] = float(
+                np.linalg.norm(Ek - E64) / np.linalg.norm(E64))
+            out["bicgstab_fp32_refine"]["time_to_tol_speedup_vs_fp64_plain"] = t64 / dtk
 
     if not args.no_batch:
         out["batched_illuminations"] = batched_leg(fm, args)
@@ -517,7 +579,11 @@ def main():
     out["sweep_roofline_frac"] = (sum(by[i] for i in SWEEP) / (sweep_ms * 1e-3) / 1e9
                                   / HBM_PEAK_GBS)
     cms = stream_copy_ms(1 << 30, reps=10)
-    out["stream_copy_GBs"] = 2 * (1 << 30) / (cms * 1e-3) / 1e9  # read + write
+    copy_gbs = 2 * (1 << 30) / (cms * 1e-3) / 1e9  # read + write
+    out["stream_copy_GBs"] = copy_gbs
+    # the same kernel against the copy ceiling measured on this box
+    out["roofline"]["copy_ceiling_GBs"] = copy_gbs
+    out["roofline"]["frac_of_copy_ceiling"] = achieved / copy_gbs
 
     if not args.no_cpu:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)

@@ -93,7 +93,10 @@ struct Finished {
 // Chunks of sets with fewer voxels than this run as one hipGraph; larger sets
 // launch their kernels directly (the host enqueues a chunk in a small part of
 // its GPU time, and a graph launch stalls the queue for 60-100 us per chunk).
-constexpr long long kGraphMaxVoxels = 1LL << 20;
+#ifndef CBS_GRAPH_MAX_VOXELS
+#define CBS_GRAPH_MAX_VOXELS (1LL << 20)
+#endif
+constexpr long long kGraphMaxVoxels = CBS_GRAPH_MAX_VOXELS;
 
 // state shared by both runners: membership, chunk graph, state gather
 template <class T>

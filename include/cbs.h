@@ -232,10 +232,10 @@ cbs_status cbs_time_sweeps(cbs_problem *p, int32_t precision, int32_t reps, doub
                            double *bytes);
 
 /* Measurement aid -- no counterpart in the reference: a plain device copy of
- * `bytes` bytes (16-byte vector loads and stores), timed with HIP events over
- * `reps` launches; *ms is the average duration of one launch.  It gives the
- * achievable HBM bandwidth of the box and the calibration kernel for HBM
- * byte counters. */
+ * `bytes` bytes (each thread loads four non-temporal 16-byte values, then
+ * stores them), timed with HIP events over `reps` launches; *ms is the average
+ * duration of one launch.  It gives the achievable HBM bandwidth of the box
+ * and the calibration kernel for HBM byte counters. */
 cbs_status cbs_stream_copy(cbs_ctx *ctx, uint64_t bytes, int32_t reps, double *ms);
 
 /* ---- device-side setup and teardown (scene) -----------------------------
