@@ -35,7 +35,7 @@ constexpr int kLdsBudget = 65536; // LDS bytes a pass tile may use (two blocks p
 #ifndef CBS_ZMERGE
 #define CBS_ZMERGE 1
 #endif
-constexpr int kNumPart = 5;       // doubles of partial sums written per xinv tile
+constexpr int kNumPart = 8;       // doubles of partial sums written per xinv tile
 constexpr int kXLines = 4;        // x lines per x-pass tile where no tuned width exists
 
 // Axis lengths with compile-time specialised transforms (one kernel per
@@ -345,14 +345,20 @@ __host__ __device__ constexpr int zk_count(int nz, int P) { return nz + P + 1; }
 // ---------------------------------------------------------------------------
 // device descriptors
 // ---------------------------------------------------------------------------
-enum PreOp { PRE_COPY = 0, PRE_V = 1 };
+// pre-ops of the x forward: u = in0, u = V in0, or u = V a where a is a
+// BiCGStab vector the pass forms (and stores) from the vectors kv of DevProb:
+//   PRE_BICG_P  x += alpha p + omega s and r = s - omega t of the previous
+//               iteration (when RState::pend_x), then p = r + beta (p - omega v)
+//               into in0 (p);  PRE_BICG_S  s = r - alpha v into kv[2] (s)
+enum PreOp { PRE_COPY = 0, PRE_V = 1, PRE_BICG_P = 2, PRE_BICG_S = 3 };
 enum ZOp { ZOP_GREEN = 0, ZOP_CURL = 1 };
 enum PostOp {
   POST_STORE = 0,    // out0 = w
   POST_OPERATOR = 1, // out0 = alpha (in0 - w) + beta out0
   POST_BORN = 2,     // r = gamma (in1 - in0 + w); out0 = in0 + r; part: |r|^2
   POST_RESID = 3,    // out0 = in1 - in0 + w;  part: |gamma out0|^2
-  POST_PMATVEC = 4   // out0 = gamma (in0 - w); part: <in1,out0>, <out0,in0>, |out0|^2
+  POST_PMATVEC = 4   // out0 = gamma (in0 - w); part: <in1,out0> (0, 1), <out0,in0> (2, 3),
+                     // |out0|^2 (4), |in0|^2 (5), <in1,in0> (6, 7)
 };
 
 // Per right-hand-side solver state kept on the device so that an iteration
@@ -373,6 +379,7 @@ struct RState {
   int nhist;       // Krylov: entries written to hist
   int cyc;         // GMRES: the current Arnoldi cycle has stopped (skip its remaining steps)
   int pend;        // GMRES: outcome to apply after the cycle's x update (0 restart, 1 conv, 2 maxit)
+  int pend_x;      // BiCGStab: x += alpha p + omega s of the last iteration not applied yet
   // BiCGStab scalars
   double rho_re, rho_im, alpha_re, alpha_im, omega_re, omega_im, beta_re, beta_im;
   double hist[512];  // residual history (relative), first 512 entries
@@ -403,6 +410,7 @@ struct DevProb {
   const Cx<T>* in0;
   const Cx<T>* in1;
   Cx<T>* out0;
+  Cx<T>* kv[5];     // PRE_BICG_*: the BiCGStab vectors r, x, s, t, v (in0 is p or s)
   Cx<T>* w;         // spectrum workspace [S][3][nvox]
   double* part;     // partial sums of this RHS, kNumPart per xinv tile
   const RState* st; // skip the launch work when st->done != 0 (may be null)

@@ -86,6 +86,7 @@ __global__ void k_state_init(RState* st, const double* __restrict__ sums, double
   st->nhist = 0;
   st->cyc = 0;
   st->pend = 0;
+  st->pend_x = 0;
 }
 
 // ---------------------------------------------------------------------------

@@ -38,13 +38,13 @@ struct BiVec {
   double eps_inv;
   double* part;
   RState* st;
-  int use_v;  // BI_INIT: r = gamma b - v (x0 given) instead of r = gamma b
+  int use_v;  // k_bicg_init: r = gamma b - v (x0 given) instead of r = gamma b
 };
 
-enum { BI_INIT = 0, BI_P = 1, BI_S = 2, BI_XR = 3 };
-
-template <class T, int OP>
-__global__ void __launch_bounds__(kThreads) k_bicg_vec(const BiVec<T>* __restrict__ vs) {
+// r0 = gamma b (- v = gamma A x0 when x0 is given), p = v = 0, partials of
+// ||r0||^2 and <c, r0> (c = b)
+template <class T>
+__global__ void __launch_bounds__(kThreads) k_bicg_init(const BiVec<T>* __restrict__ vs) {
   __shared__ double scratch[(kThreads / 64) * kNumPart];
   const BiVec<T>& q = vs[blockIdx.y];
   if (q.st->done) return;
@@ -52,51 +52,52 @@ __global__ void __launch_bounds__(kThreads) k_bicg_vec(const BiVec<T>* __restric
   if (start >= q.nv) return;
   const long long end = min(start + kChunk, q.nv);
   const long long nv = q.nv;
-  const RState* st = q.st;
   double acc[kNumPart] = {0, 0, 0, 0, 0};
-  const Cx<T> alpha = mk<T>((T)st->alpha_re, (T)st->alpha_im);
-  const Cx<T> omega = mk<T>((T)st->omega_re, (T)st->omega_im);
-  const Cx<T> beta = mk<T>((T)st->beta_re, (T)st->beta_im);
   for (long long i = start + threadIdx.x; i < end; i += kThreads) {
-    Cx<T> gam = mk<T>(T(0), T(0));
-    if (OP == BI_INIT) gam = mul_i((T)q.eps_inv * gld(q.V + i));
+    const Cx<T> gam = mul_i((T)q.eps_inv * gld(q.V + i));
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const long long j = c * nv + i;
-      if (OP == BI_INIT) {
-        const Cx<T> bj = gld(q.b + j);
-        Cx<T> r = gam * bj;
-        if (q.use_v) r = r - gld(q.v + j);
-        gst(q.r + j, r);
-        gst(q.p + j, mk<T>(T(0), T(0)));
-        gst(q.v + j, mk<T>(T(0), T(0)));
-        acc[0] += (double)abs2(r);
-        const Cx<T> d = cdot(bj, r);
-        acc[1] += (double)d.re;
-        acc[2] += (double)d.im;
-      } else if (OP == BI_P) {
-        gst(q.p + j, gld(q.r + j) + beta * (gld(q.p + j) - omega * gld(q.v + j)));
-      } else if (OP == BI_S) {
-        gst(q.s + j, gld(q.r + j) - alpha * gld(q.v + j));
-        gst(q.x + j, gld(q.x + j) + alpha * gld(q.p + j));
-      } else {  // BI_XR
-        const Cx<T> s = gld(q.s + j);
-        const Cx<T> r = s - omega * gld(q.t + j);
-        gst(q.x + j, gld(q.x + j) + omega * s);
-        gst(q.r + j, r);
-        acc[0] += (double)abs2(r);
-        const Cx<T> d = cdot(gld(q.b + j), r);
-        acc[1] += (double)d.re;
-        acc[2] += (double)d.im;
-      }
+      const Cx<T> bj = gld(q.b + j);
+      Cx<T> r = gam * bj;
+      if (q.use_v) r = r - gld(q.v + j);
+      gst(q.r + j, r);
+      gst(q.p + j, mk<T>(T(0), T(0)));
+      gst(q.v + j, mk<T>(T(0), T(0)));
+      acc[0] += (double)abs2(r);
+      const Cx<T> d = cdot(bj, r);
+      acc[1] += (double)d.re;
+      acc[2] += (double)d.im;
     }
   }
-  // only the parts each update accumulates are reduced
-  if (OP == BI_INIT || OP == BI_XR)
-    block_reduce_store(acc, q.part + (long long)blockIdx.x * kNumPart, scratch, threadIdx.x, 3);
+  block_reduce_store(acc, q.part + (long long)blockIdx.x * kNumPart, scratch, threadIdx.x, 3);
 }
 
-enum { BF_INIT = 0, BF_ALPHA = 1, BF_OMEGA = 3, BF_R = 4 };
+// the last iteration's pending x += alpha p + omega s of the members idx[y]
+// that leave the set (their later iterations would have applied it)
+template <class T>
+__global__ void __launch_bounds__(kThreads) k_bicg_final(const BiVec<T>* __restrict__ vs,
+                                                         const int* __restrict__ idx) {
+  const BiVec<T>& q = vs[idx[blockIdx.y]];
+  const RState* st = q.st;
+  if (!st->pend_x) return;
+  const long long start = (long long)blockIdx.x * kChunk;
+  if (start >= q.nv) return;
+  const long long end = min(start + kChunk, q.nv);
+  const long long nv = q.nv;
+  const Cx<T> al = mk<T>((T)st->alpha_re, (T)st->alpha_im);
+  const Cx<T> om = mk<T>((T)st->omega_re, (T)st->omega_im);
+  for (long long i = start + threadIdx.x; i < end; i += kThreads) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const long long j = c * nv + i;
+      const Cx<T> x1 = gld(q.x + j) + al * gld(q.p + j);
+      gst(q.x + j, x1 + om * gld(q.s + j));
+    }
+  }
+}
+
+enum { BF_INIT = 0, BF_ALPHA = 1, BF_OMEGA = 2 };
 
 __device__ __forceinline__ void push_hist(RState* st, double r2) {
   if (st->nhist < 512) st->hist[st->nhist] = st->bnorm2 > 0 ? sqrt(r2 / st->bnorm2) : 0.0;
@@ -109,24 +110,27 @@ __device__ __forceinline__ bool bad(double v) { return !(v <= 1e300); }
 // One block per RHS; ntiles[i] = partial blocks of the producing kernel.
 // Iteration k of Krylov.jl's bicgstab: kit counts it (at its first matvec),
 // iters counts the operator applications (two per iteration, plus the warm
-// start's r0 = gamma (b - A x0)).
+// start's r0 = gamma (b - A x0)).  BF_OMEGA takes everything the end of the
+// iteration needs from the second matvec's partials: omega = <t,s>/<t,t>,
+// ||r||^2 = ||s - omega t||^2 = ||s||^2 - |<t,s>|^2/||t||^2 and
+// <c, r> = <c, s> - omega <c, t>; the vector updates themselves run in the
+// next iteration's first x forward (pend_x).
 __global__ void __launch_bounds__(kThreads)
 k_bicg_fin(RState* const* __restrict__ sts, double* const* __restrict__ parts,
            const int* __restrict__ ntiles, int mode, int init_iters) {
-  __shared__ double red[kThreads / 64];
+  __shared__ double red[kNumPart * (kThreads / 64)];
   RState* st = sts[blockIdx.y * gridDim.x + blockIdx.x];
   const int rid = blockIdx.y * gridDim.x + blockIdx.x;
   if (st->done) return;
-  const double* part = parts[rid];
-  const int nt = ntiles[rid];
+  double sum[kNumPart];
+  block_sum_all(parts[rid], ntiles[rid], sum, red);
+  if (threadIdx.x) return;
   if (mode == BF_INIT) {
-    const double r2 = block_sum_parts(part, nt, 0, red);
-    const double cr = block_sum_parts(part, nt, 1, red);
-    const double ci = block_sum_parts(part, nt, 2, red);
-    if (threadIdx.x) return;
+    const double r2 = sum[0], cr = sum[1], ci = sum[2];
     st->r2 = r2;
     st->iters += init_iters;
     st->nhist = 0;
+    st->pend_x = 0;
     push_hist(st, r2);
     if (st->thr2 < 0.0) {
       const double t = st->abstol + st->tol * sqrt(r2);
@@ -142,37 +146,36 @@ k_bicg_fin(RState* const* __restrict__ sts, double* const* __restrict__ parts,
     else if (r2 == 0.0 || r2 <= st->thr2) st->done = 1;
     else if (cr == 0.0 && ci == 0.0) st->done = 4;  // breakdown: <c, r0> = 0
   } else if (mode == BF_ALPHA) {
-    const double sr = block_sum_parts(part, nt, 0, red);
-    const double si = block_sum_parts(part, nt, 1, red);
-    if (threadIdx.x) return;
+    const double sr = sum[0], si = sum[1];
     st->kit += 1;
     st->iters += 1;
+    st->pend_x = 0;  // this iteration's first x forward applied it
     if (sr == 0.0 && si == 0.0) {
-      st->done = 4;  // breakdown: <c, v> = 0 (x is still the last iterate)
+      st->done = 4;  // breakdown: <c, v> = 0 (x is the last iterate)
       return;
     }
     const Cx<double> a = cdiv(mk<double>(st->rho_re, st->rho_im), mk<double>(sr, si));
     st->alpha_re = a.re;
     st->alpha_im = a.im;
-  } else if (mode == BF_OMEGA) {
-    const double ts_r = block_sum_parts(part, nt, 2, red);
-    const double ts_i = block_sum_parts(part, nt, 3, red);
-    const double tt = block_sum_parts(part, nt, 4, red);
-    if (threadIdx.x) return;
+  } else {  // BF_OMEGA
+    const double ctr = sum[0], cti = sum[1], tsr = sum[2], tsi = sum[3];
+    const double tt = sum[4], ss = sum[5], csr = sum[6], csi = sum[7];
     st->iters += 1;
+    st->pend_x = 1;  // x += alpha p + omega s, r = s - omega t
     // t = 0: omega = 0 leaves r = s, the residual of x + alpha p
-    st->omega_re = tt == 0.0 ? 0.0 : ts_r / tt;
-    st->omega_im = tt == 0.0 ? 0.0 : ts_i / tt;
-  } else {  // BF_R
-    const double r2 = block_sum_parts(part, nt, 0, red);
-    const double pr = block_sum_parts(part, nt, 1, red);
-    const double pi = block_sum_parts(part, nt, 2, red);
-    if (threadIdx.x) return;
+    const double om_re = tt == 0.0 ? 0.0 : tsr / tt;
+    const double om_im = tt == 0.0 ? 0.0 : tsi / tt;
+    st->omega_re = om_re;
+    st->omega_im = om_im;
+    const double r2 = fmax(0.0, ss - (tt == 0.0 ? 0.0 : (tsr * tsr + tsi * tsi) / tt));
+    // rho_new = <c, r> = <c, s> - omega <c, t>
+    const Cx<double> rn = mk<double>(csr - (om_re * ctr - om_im * cti),
+                                     csi - (om_re * cti + om_im * ctr));
     st->r2 = r2;
     push_hist(st, r2);
-    const bool om0 = st->omega_re == 0.0 && st->omega_im == 0.0;
+    const bool om0 = om_re == 0.0 && om_im == 0.0;
     const bool al0 = st->alpha_re == 0.0 && st->alpha_im == 0.0;
-    if (bad(r2)) {
+    if (bad(ss) || bad(tt) || !(r2 == r2)) {
       st->done = 3;
     } else if (r2 <= st->thr2) {
       st->done = 1;
@@ -181,11 +184,9 @@ k_bicg_fin(RState* const* __restrict__ sts, double* const* __restrict__ parts,
     } else if (st->kit >= st->maxit) {
       st->done = 2;
     } else {
-      // beta = (rho_new / rho) (alpha / omega); rho_new = <c, r>
-      const Cx<double> rn = mk<double>(pr, pi);
+      // beta = (rho_new / rho) (alpha / omega)
       const Cx<double> b = cdiv(rn, mk<double>(st->rho_re, st->rho_im)) *
-                           cdiv(mk<double>(st->alpha_re, st->alpha_im),
-                                mk<double>(st->omega_re, st->omega_im));
+                           cdiv(mk<double>(st->alpha_re, st->alpha_im), mk<double>(om_re, om_im));
       st->beta_re = b.re;
       st->beta_im = b.im;
       st->rho_re = rn.re;
@@ -211,6 +212,16 @@ static BiVec<T> make_bivec(SolveItem<T>& x, int use_v) {
   q.st = x.w->st.template as<RState>();
   q.use_v = use_v;
   return q;
+}
+
+// the BiCGStab vectors the x-forward pre-ops read and write (DevProb::kv)
+template <class T>
+static void set_kv(DevProb<T>& d, const BiVec<T>& q) {
+  d.kv[0] = q.r;
+  d.kv[1] = q.x;
+  d.kv[2] = q.s;
+  d.kv[3] = q.t;
+  d.kv[4] = q.v;
 }
 
 static void bicg_fin(StateSet& f, int n, int mode, int init_iters, hipStream_t s) {
@@ -256,8 +267,8 @@ void BicgRunner<T>::add(const std::vector<long long>& keys, std::vector<SolveIte
     m0.build(ctx, s);
     m0.launch(s);
   }
-  hipLaunchKernelGGL((k_bicg_vec<T, BI_INIT>), dim3((unsigned)mc, (unsigned)n), dim3(kThreads), 0,
-                     s, dvi.as<BiVec<T>>());
+  hipLaunchKernelGGL((k_bicg_init<T>), dim3((unsigned)mc, (unsigned)n), dim3(kThreads), 0, s,
+                     dvi.as<BiVec<T>>());
   bicg_fin(fv, n, BF_INIT, io.x0_zero ? 0 : 1, s);
   CBS_HIP(hipGetLastError());
   CBS_HIP(hipStreamSynchronize(s));  // dvi, fv and m0 go out of scope
@@ -276,7 +287,7 @@ void BicgRunner<T>::rebuild() {
   std::vector<BiVec<T>> hv(n);
   std::vector<RState*> sts;
   std::vector<double*> parts;
-  std::vector<int> nt_mv, nt_vec;
+  std::vector<int> nt_mv;
   m1.h.clear();
   m2.h.clear();
   maxchunk = 1;
@@ -284,15 +295,18 @@ void BicgRunner<T>::rebuild() {
     SolveItem<T>& x = this->items[i].it;
     BiVec<T>& q = hv[i] = make_bivec<T>(x, 1);
     Cx<T>* W = x.w->W.template as<Cx<T>>();
-    // v = A^ p (with <c, v>, c = b), t = A^ s (with <t, s>, |t|^2)
-    m1.h.push_back(make_desc<T>(*x.p, PRE_V, ZOP_GREEN, POST_PMATVEC, q.p, q.b, q.v, W, q.part,
-                                q.st));
-    m2.h.push_back(make_desc<T>(*x.p, PRE_V, ZOP_GREEN, POST_PMATVEC, q.s, nullptr, q.t, W,
+    // v = A^ p with <c, v> (c = b); its x forward first applies the last
+    // iteration's x and r updates and forms p.  t = A^ s with <c,t>, <t,s>,
+    // |t|^2, |s|^2 and <c,s>; its x forward forms s = r - alpha v.
+    m1.h.push_back(make_desc<T>(*x.p, PRE_BICG_P, ZOP_GREEN, POST_PMATVEC, q.p, q.b, q.v, W,
                                 q.part, q.st));
+    set_kv(m1.h.back(), q);
+    m2.h.push_back(make_desc<T>(*x.p, PRE_BICG_S, ZOP_GREEN, POST_PMATVEC, q.s, q.b, q.t, W,
+                                q.part, q.st));
+    set_kv(m2.h.back(), q);
     sts.push_back(q.st);
     parts.push_back(q.part);
     nt_mv.push_back(m1.h.back().tiles[3]);
-    nt_vec.push_back((int)((q.nv + kChunk - 1) / kChunk));
     maxchunk = std::max(maxchunk, (q.nv + kChunk - 1) / kChunk);
   }
   dv.ensure(ctx, n * sizeof(BiVec<T>));
@@ -300,7 +314,6 @@ void BicgRunner<T>::rebuild() {
   m1.build(ctx, s);
   m2.build(ctx, s);
   fin_mv.build(ctx, sts, parts, nt_mv, s);
-  fin_vec.build(ctx, sts, parts, nt_vec, s);
   if (this->want_graph()) {
     CBS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     for (int j = 0; j < this->k; ++j) iteration(n);
@@ -314,18 +327,34 @@ template <class T>
 void BicgRunner<T>::iteration(int na) {
   const int n = (int)this->items.size();
   hipStream_t s = this->s;
-  const dim3 vgrid((unsigned)maxchunk, (unsigned)na);
-  const BiVec<T>* d_vs = dv.as<BiVec<T>>();
   const PipeLaunch<T> L1 = na == n ? m1.L : m1.prefix(na);
   const PipeLaunch<T> L2 = na == n ? m2.L : m2.prefix(na);
-  hipLaunchKernelGGL((k_bicg_vec<T, BI_P>), vgrid, dim3(kThreads), 0, s, d_vs);
   launch_pipeline<T>(L1, s);
   bicg_fin(fin_mv, na, BF_ALPHA, 0, s);
-  hipLaunchKernelGGL((k_bicg_vec<T, BI_S>), vgrid, dim3(kThreads), 0, s, d_vs);
   launch_pipeline<T>(L2, s);
   bicg_fin(fin_mv, na, BF_OMEGA, 0, s);
-  hipLaunchKernelGGL((k_bicg_vec<T, BI_XR>), vgrid, dim3(kThreads), 0, s, d_vs);
-  bicg_fin(fin_vec, na, BF_R, 0, s);
+}
+
+template <class T>
+void BicgRunner<T>::collect(std::vector<Finished>& out) {
+  // members that leave still owe x their last iteration's alpha p + omega s
+  if (this->hg.size() == 4 * this->items.size()) {
+    std::vector<int> idx;
+    for (size_t i = 0; i < this->items.size(); ++i)
+      if (this->hg[4 * i] != 0.0 || this->items[i].launched >= this->items[i].cap)
+        idx.push_back((int)i);
+    if (!idx.empty()) {
+      DevBuf di;
+      di.alloc(this->ctx, idx.size() * sizeof(int));
+      CBS_HIP(hipMemcpyAsync(di.p, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice,
+                             this->s));
+      hipLaunchKernelGGL((k_bicg_final<T>), dim3((unsigned)maxchunk, (unsigned)idx.size()),
+                         dim3(kThreads), 0, this->s, dv.as<BiVec<T>>(), di.as<int>());
+      CBS_HIP(hipGetLastError());
+      CBS_HIP(hipStreamSynchronize(this->s));  // di goes out of scope
+    }
+  }
+  RunnerBase<T>::collect(out, false);
 }
 
 template <class T>
@@ -350,7 +379,7 @@ void BicgRunner<T>::launch(int kk) {
   }
   CBS_HIP(hipGetLastError());
   for (int i = 0; i < n; ++i) this->items[i].launched += bud[i];
-  this->launch_gather(fin_vec.sts.as<RState*>());
+  this->launch_gather(fin_mv.sts.as<RState*>());
 }
 
 template <class T>

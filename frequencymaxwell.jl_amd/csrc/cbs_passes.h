@@ -70,6 +70,54 @@ struct alignas(16) C2f {
   Cx<float> a, b;
 };
 
+// The x-forward operand a of voxel g (u = V a for PRE_V and the BiCGStab
+// pre-ops, u = a for PRE_COPY).  The BiCGStab pre-ops form a from the vectors
+// of the iteration and store it (and the updated x and r) on the way: the
+// vector updates of BiCGStab cost no pass of their own.
+template <class T>
+__device__ __forceinline__ void xfwd_operand(const DevProb<T>& pr, long long g,
+                                             Cx<T> (&a)[3]) {
+  const long long nv = pr.nvox;
+  if (pr.pre == PRE_BICG_P) {
+    const RState* st = pr.st;
+    const Cx<T> al = mk<T>((T)st->alpha_re, (T)st->alpha_im);
+    const Cx<T> om = mk<T>((T)st->omega_re, (T)st->omega_im);
+    const Cx<T> be = mk<T>((T)st->beta_re, (T)st->beta_im);
+    const bool pend = st->pend_x != 0;
+    Cx<T>* pp = const_cast<Cx<T>*>(pr.in0);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const long long j = c * nv + g;
+      const Cx<T> po = gld(pp + j);
+      Cx<T> r;
+      if (pend) {  // the previous iteration's x and r (Krylov.jl: x += alpha p; x += omega s)
+        const Cx<T> sv = gld(pr.kv[2] + j);
+        r = sv - om * gld(pr.kv[3] + j);
+        const Cx<T> x1 = gld(pr.kv[1] + j) + al * po;
+        gst(pr.kv[1] + j, x1 + om * sv);
+        gst(pr.kv[0] + j, r);
+      } else {
+        r = gld(pr.kv[0] + j);
+      }
+      const Cx<T> pn = r + be * (po - om * gld(pr.kv[4] + j));
+      gst(pp + j, pn);
+      a[c] = pn;
+    }
+  } else if (pr.pre == PRE_BICG_S) {
+    const Cx<T> al = mk<T>((T)pr.st->alpha_re, (T)pr.st->alpha_im);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const long long j = c * nv + g;
+      const Cx<T> sv = gld(pr.kv[0] + j) - al * gld(pr.kv[4] + j);
+      gst(pr.kv[2] + j, sv);
+      a[c] = sv;
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) a[c] = gld(pr.in0 + c * nv + g);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // sweep A: x forward transform, pre-op u = V x (or x), shift ramps.
 // Every path transforms the 3 S P pencils of its P lines (the S ramped copies
@@ -171,8 +219,7 @@ __device__ __forceinline__ void xfwd_tile(const DevProb<T>& pr, int tile, int ti
   const int nlines = ny * pr.n[2];
   const int l0 = tile * P;
   const int np = min(P, nlines - l0);
-  const long long nv = pr.nvox;
-  const bool useV = pr.pre == PRE_V;
+  const bool useV = pr.pre != PRE_COPY;
   if constexpr (CT) {
     constexpr int Pc = xlines(CT ? NC : 8, eb, S, false);
     constexpr int E = (Pc * NC + NT - 1) / NT;
@@ -186,11 +233,10 @@ __device__ __forceinline__ void xfwd_tile(const DevProb<T>& pr, int tile, int ti
       if (e < Pc * NC && p < np) {
         const long long g = (long long)(l0 + p) * NC + m;
         const Cx<T> vv = useV ? gld(pr.V + g) : mk<T>(T(1), T(0));
+        Cx<T> xv[3];
+        xfwd_operand(pr, g, xv);
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const Cx<T> xv = gld(pr.in0 + c * nv + g);
-          u[k][c] = useV ? cmul_nc(vv, xv) : xv;
-        }
+        for (int c = 0; c < 3; ++c) u[k][c] = useV ? cmul_nc(vv, xv[c]) : xv[c];
         if (pr.has_ramp) {
           const int l = l0 + p;
           rp[k] = ramp_at(pr, m, l % ny, l / ny);
@@ -210,11 +256,10 @@ __device__ __forceinline__ void xfwd_tile(const DevProb<T>& pr, int tile, int ti
       if (p < np) {
         const long long g = (long long)(l0 + p) * n + m;
         const Cx<T> vv = useV ? gld(pr.V + g) : mk<T>(T(1), T(0));
+        Cx<T> xv[3];
+        xfwd_operand(pr, g, xv);
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const Cx<T> xv = gld(pr.in0 + c * nv + g);
-          u[c] = useV ? cmul_nc(vv, xv) : xv;
-        }
+        for (int c = 0; c < 3; ++c) u[c] = useV ? cmul_nc(vv, xv[c]) : xv[c];
         if (pr.has_ramp) {
           const int l = l0 + p;
           rp = ramp_at(pr, m, l % ny, l / ny);
@@ -584,8 +629,10 @@ __device__ __forceinline__ void xinv_operands(const DevProb<T>& pr, long long g,
 
 // The fused epilogue of voxel g from w = G u and the operands above;
 // acc collects the FP64 partial sums of the post-op.  For POST_BORN w
-// returns the new iterate written to out0.
-template <class T>
+// returns the new iterate written to out0.  BORN: the caller runs POST_BORN
+// only (the fused x pass), so the other post-ops are not compiled in (their
+// accumulators would cost the tile registers).
+template <class T, bool BORN = false>
 __device__ __forceinline__ void xinv_finish(const DevProb<T>& pr, long long g, Cx<T> (&w)[3],
                                             const Cx<T>& vv, const Cx<T> (&a0)[3],
                                             const Cx<T> (&a1)[3], double (&acc)[kNumPart]) {
@@ -593,7 +640,7 @@ __device__ __forceinline__ void xinv_finish(const DevProb<T>& pr, long long g, C
   // round the update identically
 #pragma clang fp contract(off)
   const long long nv = pr.nvox;
-  const int post = pr.post;
+  const int post = BORN ? (int)POST_BORN : pr.post;
   if (post == POST_STORE) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) gst(pr.out0 + c * nv + g, w[c]);
@@ -630,20 +677,25 @@ __device__ __forceinline__ void xinv_finish(const DevProb<T>& pr, long long g, C
         const Cx<T> gr = cmul_nc(gam, r);
         acc[0] += (double)(gr.re * gr.re + gr.im * gr.im);
       }
-    } else {  // POST_PMATVEC
+    } else {  // POST_PMATVEC: products in FP64
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         const Cx<T> o = cmul_nc(gam, mk<T>(a0[c].re - w[c].re, a0[c].im - w[c].im));
         gst(pr.out0 + c * nv + g, o);
+        const double ore = o.re, oim = o.im, are = a0[c].re, aim = a0[c].im;
         if (pr.in1) {
-          // <in1, o> = conj(in1) o
-          acc[0] += (double)(a1[c].re * o.re + a1[c].im * o.im);
-          acc[1] += (double)(a1[c].re * o.im - a1[c].im * o.re);
+          const double cre = a1[c].re, cim = a1[c].im;
+          // <in1, o> = conj(in1) o and <in1, in0>
+          acc[0] += cre * ore + cim * oim;
+          acc[1] += cre * oim - cim * ore;
+          acc[6] += cre * are + cim * aim;
+          acc[7] += cre * aim - cim * are;
         }
-        // <o, in0>
-        acc[2] += (double)(o.re * a0[c].re + o.im * a0[c].im);
-        acc[3] += (double)(o.re * a0[c].im - o.im * a0[c].re);
-        acc[4] += (double)(o.re * o.re + o.im * o.im);
+        // <o, in0>, |o|^2, |in0|^2
+        acc[2] += ore * are + oim * aim;
+        acc[3] += ore * aim - oim * are;
+        acc[4] += ore * ore + oim * oim;
+        acc[5] += are * are + aim * aim;
       }
     }
   }
@@ -740,7 +792,7 @@ __device__ __forceinline__ void xinv_tile(const DevProb<T>& pr, int tile, int ti
 #pragma unroll
           for (int c = 0; c < 3; ++c) w[0][k][c] = T(0.5) * w[0][k][c];
         }
-        xinv_finish(pr, (long long)(l0 + p) * NC + m, w[0][k], vv[k], a0[k], a1[k], acc);
+        xinv_finish<T, FUSE>(pr, (long long)(l0 + p) * NC + m, w[0][k], vv[k], a0[k], a1[k], acc);
       }
     }
     if constexpr (FUSE) {
@@ -796,7 +848,7 @@ __device__ __forceinline__ void xinv_tile(const DevProb<T>& pr, int tile, int ti
       const long long g = (long long)l * n + m;
       Cx<T> vv, a0[3], a1[3];
       xinv_operands(pr, g, true, vv, a0, a1);
-      xinv_finish(pr, g, w, vv, a0, a1, acc);
+      xinv_finish<T, FUSE>(pr, g, w, vv, a0, a1, acc);
       if constexpr (fuse) {
         Cx<T> u[3];
 #pragma unroll
@@ -810,7 +862,7 @@ __device__ __forceinline__ void xinv_tile(const DevProb<T>& pr, int tile, int ti
     double* scratch = reinterpret_cast<double*>(g_smem);
     // POST_BORN and POST_RESID accumulate only part 0
     block_reduce_store<NT>(acc, pr.part + (long long)tile * kNumPart, scratch, tid,
-                           pr.post == POST_PMATVEC ? kNumPart : 1);
+                           (!FUSE && pr.post == POST_PMATVEC) ? kNumPart : 1);
   }
 }
 }  // namespace cbs

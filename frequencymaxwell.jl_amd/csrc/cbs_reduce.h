@@ -59,6 +59,47 @@ __device__ __forceinline__ double block_sum_parts(const double* __restrict__ par
   return s;
 }
 
+// Every column of kNumPart-strided partials [0, ntile) in one pass (rows of
+// kNumPart doubles read as 16-byte pairs); tot gets the column totals in every
+// thread.  red holds kNumPart * (kThreads / 64) doubles.  Fixed summation
+// order: each thread adds its rows in ascending order, then waves and blocks
+// combine in a fixed tree.
+__device__ __forceinline__ void block_sum_all(const double* __restrict__ part, int ntile,
+                                              double (&tot)[kNumPart], double* red) {
+  static_assert(kNumPart % 2 == 0, "rows are read as pairs of doubles");
+  const int tid = threadIdx.x;
+  double v[kNumPart];
+#pragma unroll
+  for (int k = 0; k < kNumPart; ++k) v[k] = 0.0;
+  for (int i = tid; i < ntile; i += kThreads) {
+    typedef double D2 __attribute__((ext_vector_type(2)));
+    const D2* row = reinterpret_cast<const D2*>(part + (long long)i * kNumPart);
+#pragma unroll
+    for (int k = 0; k < kNumPart / 2; ++k) {
+      const D2 d = row[k];
+      v[2 * k] += d.x;
+      v[2 * k + 1] += d.y;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kNumPart; ++k) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off, 64);
+  }
+  __syncthreads();
+  if ((tid & 63) == 0) {
+#pragma unroll
+    for (int k = 0; k < kNumPart; ++k) red[k * (kThreads / 64) + (tid >> 6)] = v[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kNumPart; ++k) {
+    double t = 0.0;
+    for (int w = 0; w < kThreads / 64; ++w) t += red[k * (kThreads / 64) + w];
+    tot[k] = t;
+  }
+}
+
 // Column k of kNumPart-strided partials [0, ntile) summed by a block of NT
 // threads, four rows in flight per thread (each thread still adds its rows in
 // ascending order); every thread gets the total.

@@ -197,11 +197,13 @@ struct BornRunner : RunnerBase<T> {
   void rebuild();
 };
 
-// BiCGStab on the preconditioned system; x is X[0] of each member
+// BiCGStab on the preconditioned system; x is X[0] of each member.  An
+// iteration is two sweep pipelines and two finalizes: the vector updates run
+// in the pipelines' x forwards, the last one of a member when it leaves.
 template <class T>
 struct BicgRunner : RunnerBase<T> {
   Pipe<T> m1, m2;
-  StateSet fin_mv, fin_vec;
+  StateSet fin_mv;
   DevBuf dv;
   long long maxchunk = 1;
   void add(const std::vector<long long>& keys, std::vector<SolveItem<T>>& xs,
@@ -211,7 +213,8 @@ struct BicgRunner : RunnerBase<T> {
     launch();
     this->wait();
   }
-  void collect(std::vector<Finished>& out) { RunnerBase<T>::collect(out, false); }
+  // finished members leave (after their pending x update)
+  void collect(std::vector<Finished>& out);
   void rebuild();
   void iteration(int na);  // one iteration of the first na members
 };
