@@ -101,7 +101,7 @@ def serve_request(fm, grid, rng, tol):
     return S.scene(sol, src, med)
 
 
-def serve_stream(fm, n_req, rate, max_active, seed, maxiters, field_every=0):
+def serve_stream(fm, n_req, rate, max_active, seed, maxiters, field_every=0, policy="lead"):
     """Poisson arrivals (rate per second) of scene solves on grids drawn from
     SERVE_GRIDS with tolerance 1e-4 or 1e-6; returns latency percentiles and
     throughput.  The requests are generated before the clock starts."""
@@ -114,7 +114,7 @@ def serve_stream(fm, n_req, rate, max_active, seed, maxiters, field_every=0):
         tol = 1e-4 if rng.random() < 0.5 else 1e-6
         reqs.append((serve_request(fm, g, rng, tol), tol))
     gaps = rng.exponential(1.0 / rate, n_req)
-    srv = fm.CBSServer(max_active=max_active)
+    srv = fm.CBSServer(max_active=max_active, policy=policy)
     tickets = [None] * n_req
     t0 = time.perf_counter()
 
@@ -148,7 +148,7 @@ def serve_stream(fm, n_req, rate, max_active, seed, maxiters, field_every=0):
         srv.stop()
     wall = time.perf_counter() - t0
     out = {
-        "requests": n_req, "max_active": max_active, "arrival_rate_per_s": rate,
+        "requests": n_req, "max_active": max_active, "policy": policy, "arrival_rate_per_s": rate,
         "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
         "mean_ms": float(np.mean(lat)), "voxel_iters_per_s": vi / wall,
         "requests_per_s": n_req / wall, "wall_s": wall, "converged": conv,
@@ -380,7 +380,7 @@ def read_traffic(summary: str, kernel: str):
 def serving_leg(fm, args):
     """The mixed-size request stream, continuously batched and one at a time."""
     print("[bench] serving stream ...", file=sys.stderr, flush=True)
-    cb, seq, cap = None, None, None
+    cb, cbs, seq, cap = None, None, None, None
     rate = args.serve_rate
     if rate <= 0:
         # capacity: the same requests back to back, one at a time; arrivals at 80 % of it
@@ -391,9 +391,14 @@ def serving_leg(fm, args):
               file=sys.stderr, flush=True)
     if args.serve_mode in ("both", "batched"):
         cb = serve_stream(fm, args.serve_requests, rate, args.serve_active, 7, args.maxiters,
-                          field_every=args.serve_field_every)
-        print(f"[bench] continuous batching: p50 {cb['p50_ms']:.0f} ms p99 {cb['p99_ms']:.0f} ms",
-              file=sys.stderr, flush=True)
+                          field_every=args.serve_field_every, policy="lead")
+        print(f"[bench] continuous batching, lead policy: p50 {cb['p50_ms']:.0f} ms "
+              f"p99 {cb['p99_ms']:.0f} ms", file=sys.stderr, flush=True)
+    if args.serve_mode in ("both", "share"):
+        cbs = serve_stream(fm, args.serve_requests, rate, args.serve_active, 7, args.maxiters,
+                           policy="share")
+        print(f"[bench] continuous batching, equal shares: p50 {cbs['p50_ms']:.0f} ms "
+              f"p99 {cbs['p99_ms']:.0f} ms", file=sys.stderr, flush=True)
     if args.serve_mode in ("both", "sequential"):
         seq = serve_stream(fm, args.serve_requests, rate, 1, 7, args.maxiters)
         print(f"[bench] one at a time: p50 {seq['p50_ms']:.0f} ms p99 {seq['p99_ms']:.0f} ms",
@@ -404,7 +409,8 @@ def serving_leg(fm, args):
                         + " (absorbing 16 px padding each side), 1-10 spheres n 1.36-1.60, "
                           "plane wave within NA 1.0, tol 1e-4 or 1e-6, Born FP32+refine",
             "capacity_requests_per_s": cap, "arrival_rate_per_s": rate,
-            "continuous_batching": cb, "one_at_a_time": seq}
+            "continuous_batching": cb, "continuous_batching_equal_shares": cbs,
+            "one_at_a_time": seq}
 
 
 def main():
@@ -434,7 +440,7 @@ def main():
                     help="arrivals per second (0: 80 %% of the capacity measured first)")
     ap.add_argument("--serve-active", type=int, default=8)
     ap.add_argument("--serve-only", action="store_true")
-    ap.add_argument("--serve-mode", default="both", choices=["both", "batched", "sequential"])
+    ap.add_argument("--serve-mode", default="both", choices=["both", "batched", "share", "sequential"])
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-illumination leg")
     ap.add_argument("--batch-dirs", type=int, default=64, help="illumination directions (x 2 pol)")
     ap.add_argument("--batch-only", action="store_true")

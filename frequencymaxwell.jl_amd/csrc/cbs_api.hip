@@ -798,6 +798,13 @@ cbs_status cbs_server_stop(cbs_ctx* ctx) {
   });
 }
 
+cbs_status cbs_server_policy(cbs_ctx* ctx, int32_t policy, double kappa) {
+  return guarded([&] {
+    CBS_REQUIRE(ctx, "NULL argument");
+    server_policy(ctx->impl, policy, kappa);
+  });
+}
+
 cbs_status cbs_submit(cbs_ctx* ctx, const cbs_request* req, uint64_t* ticket) {
   return guarded([&] {
     CBS_REQUIRE(ctx && req && ticket, "NULL argument");

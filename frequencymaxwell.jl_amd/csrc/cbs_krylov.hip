@@ -368,6 +368,7 @@ void BicgRunner<T>::launch(int kk) {
   }
   const std::vector<int> bud = this->budgets(kk, false);
   const int K = bud[0];
+  if (K == 0) return;  // no member runs in this chunk
   if (this->gexec && K == this->k && bud[n - 1] == K) {
     CBS_HIP(hipGraphLaunch(this->gexec, this->s));
   } else {

@@ -7,6 +7,7 @@ namespace cbs {
 
 void server_start(Ctx& c, int max_active);
 void server_stop(Ctx& c);
+void server_policy(Ctx& c, int policy, double kappa);
 // stop and free the server of a context that is being destroyed
 void server_destroy(Ctx& c);
 uint64_t server_submit(Ctx& c, const cbs_request& r);

@@ -64,6 +64,7 @@ struct Job {
   double bn2 = 0, thr2 = -1.0;
   int used = 0, refines = 0;
   bool numeric = false;  // a residual became NaN or Inf: the request ends CBS_ENUMERIC
+  double est = 0.0;      // estimated work: nvox^(4/3) log10(1/tol), x2 FP64, x0.5 BiCGStab
   SolveItem<double> it64;
   SolveItem<float> it32;
   int set = -1;  // solver set the job is a member of (-1: none)
@@ -122,6 +123,12 @@ struct Server {
   int max_active = 1;
   uint64_t capacity = 0;  // admission limit in bytes
   double target_ms = 20.0;  // aimed duration of one chunk
+  // chunk scheduling (cbs_server_policy) and the mean estimated work of the
+  // requests submitted so far (under mu)
+  std::atomic<int> policy{CBS_SERVE_LEAD};
+  std::atomic<double> kappa{0.2};
+  double est_sum = 0.0;
+  long long est_n = 0;
   hipStream_t stream = nullptr, io = nullptr;
   hipEvent_t ev_up = nullptr, ev_down = nullptr;
   Pinned up, down;
@@ -160,6 +167,7 @@ struct Server {
   void leave_set(Job& j);
   void resid_step();
   void launch_sets();
+  void launch_lead();
   void wait_sets();
   void begin_retire(Job& j);
   void start_download(Job& j);
@@ -431,23 +439,84 @@ void Server::resid_step() {
 // iteration (FP32 sweeps move ~62 x 8 bytes per voxel, FP64 twice that,
 // BiCGStab two operator applications), so small grids are not held to the
 // pace of large ones.
+// iterations of one member worth `ms` of GPU time (FP32 sweeps move ~62 x 8
+// bytes per voxel, FP64 twice that, BiCGStab two operator applications)
+static int chunk_budget(long long nvox, int idx, double ms) {
+  const bool bicg = (idx & 2) != 0;
+  const double per_vox = 62.0 * ((idx & 4) ? 8.0 : 16.0) * (bicg ? 2.0 : 1.0);
+  const double iter_ms = (double)nvox * per_vox / 3.0e9;  // 3 TB/s sustained
+  const int k = (int)std::floor(ms / std::max(iter_ms, 1e-6));
+  return bicg ? std::max(1, std::min(32, k)) : std::max(2, std::min(64, k & ~1));
+}
+
+// CBS_SERVE_LEAD: one member runs per tick, the oldest request whose estimated
+// work is at most kappa times the mean of the requests submitted so far, else
+// the oldest request: first come first served, with small requests passing.
+// With one GPU kept busy by any member, FIFO order bounds the worst latency
+// best; letting the small requests pass costs the large ones little.
+void Server::launch_lead() {
+  double mean;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    mean = est_n ? est_sum / (double)est_n : 0.0;
+  }
+  const double k = kappa.load();
+  Job* lead = nullptr;
+  bool lead_small = false;
+  for (auto& j : active) {
+    if (j->set < 0) continue;
+    const bool small = j->est <= k * mean;
+    if (!lead || (small && !lead_small) ||
+        (small == lead_small && j->t_submit < lead->t_submit)) {
+      lead = j.get();
+      lead_small = small;
+    }
+  }
+  each_set([&](auto& R, int idx) {
+    if (R.empty()) return;
+    bool reorder = false, any = false;
+    int K = 0;
+    for (auto& m : R.items) {
+      const bool is_lead = lead && (uint64_t)m.key == lead->ticket;
+      const int prio = is_lead ? 0 : 1;
+      if (m.prio != prio) {
+        m.prio = prio;
+        reorder = true;
+      }
+      m.budget = is_lead ? chunk_budget(m.it.p->nvox, idx, target_ms) : -1;
+      if (is_lead) {
+        any = true;
+        K = m.budget;
+      }
+    }
+    if (reorder) R.changed();  // the lead goes first in the launch tables
+    if (any) R.launch(K);
+  });
+}
+
 void Server::launch_sets() {
   int members = 0;
   each_set([&](auto& R, int) { members += (int)R.items.size(); });
   if (members == 0) return;
+  if (policy.load() == CBS_SERVE_LEAD) {
+    launch_lead();
+    return;
+  }
   const double share = target_ms / members;
   each_set([&](auto& R, int idx) {
     if (R.empty()) return;
-    const bool bicg = (idx & 2) != 0;
-    const double per_vox = 62.0 * ((idx & 4) ? 8.0 : 16.0) * (bicg ? 2.0 : 1.0);
     int K = 0;
+    bool reorder = false;
     for (auto& m : R.items) {
-      const double iter_ms = (double)m.it.p->nvox * per_vox / 3.0e9;  // 3 TB/s sustained
-      int k = (int)std::floor(share / std::max(iter_ms, 1e-6));
-      k = bicg ? std::max(1, std::min(32, k)) : std::max(2, std::min(64, k & ~1));
+      const int k = chunk_budget(m.it.p->nvox, idx, share);
       m.budget = k;
       K = std::max(K, k);
+      if (m.prio != 1) {
+        m.prio = 1;
+        reorder = true;
+      }
     }
+    if (reorder) R.changed();
     R.launch(K);
   });
 }
@@ -653,6 +722,16 @@ void Server::loop() {
 // ---------------------------------------------------------------------------
 // client side
 // ---------------------------------------------------------------------------
+static Server& running_server(Ctx& c);
+
+void server_policy(Ctx& c, int policy, double kappa) {
+  CBS_REQUIRE(policy == CBS_SERVE_SHARE || policy == CBS_SERVE_LEAD, "unknown serving policy");
+  CBS_REQUIRE(kappa >= 0.0 && std::isfinite(kappa), "kappa must be finite and >= 0");
+  Server& sv = running_server(c);
+  sv.policy.store(policy);
+  sv.kappa.store(kappa);
+}
+
 void server_start(Ctx& c, int max_active) {
   std::lock_guard<std::mutex> g(c.mu);
   if (!c.server) c.server = new Server();
@@ -760,6 +839,13 @@ static uint64_t enqueue(Server& sv, std::shared_ptr<Job> j) {
                                 std::to_string(sv.capacity));
   j->ticket = sv.next_ticket++;
   j->t_submit = Clock::now();
+  {
+    const double nv = (double)j->desc.n[0] * (double)j->desc.n[1] * (double)j->desc.n[2];
+    j->est = std::pow(nv, 4.0 / 3.0) * std::log10(1.0 / std::max(j->o.tol, 1e-12)) *
+             (j->o.precision == CBS_FP64 ? 2.0 : 1.0) * (j->o.method == CBS_BICGSTAB ? 0.5 : 1.0);
+    sv.est_sum += j->est;
+    sv.est_n += 1;
+  }
   sv.jobs[j->ticket] = j;
   sv.queue.push_back(j);
   sv.cv.notify_all();

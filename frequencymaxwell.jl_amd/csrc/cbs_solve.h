@@ -106,7 +106,8 @@ struct RunnerBase {
     SolveItem<T> it;
     long long launched;  // iterations launched for this item
     long long cap;       // safety bound on launched iterations
-    int budget = 0;      // iterations in the next chunk (0: the chunk length)
+    int budget = 0;      // iterations in the next chunk (0: the chunk length, < 0: none)
+    int prio = 1;        // launch-table order: lower first (a scheduler's lead member)
   };
   Ctx* ctx = nullptr;
   hipStream_t s = nullptr;
@@ -159,8 +160,9 @@ struct RunnerBase {
   // finished items (device done flag set, or cap reached) leave the set
   void collect(std::vector<Finished>& out, bool born);
   void remove(long long key);
-  // members in ascending cost (voxels), so that the members still running in
-  // any iteration of a chunk with per-member budgets form a prefix
+  // members by priority, then in ascending cost (voxels), so that the members
+  // still running in any iteration of a chunk with per-member budgets form a
+  // prefix
   void sort_items();
   // per-member iterations of a chunk of K (non-increasing along the items,
   // even when even_only)

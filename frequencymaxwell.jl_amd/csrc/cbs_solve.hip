@@ -182,6 +182,7 @@ void RunnerBase<T>::collect(std::vector<Finished>& out, bool born) {
 template <class T>
 void RunnerBase<T>::sort_items() {
   std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& b) {
+    if (a.prio != b.prio) return a.prio < b.prio;
     if (a.it.p->nvox != b.it.p->nvox) return a.it.p->nvox < b.it.p->nvox;
     return a.key < b.key;
   });
@@ -191,8 +192,8 @@ template <class T>
 std::vector<int> RunnerBase<T>::budgets(int K, bool even_only) const {
   std::vector<int> b(items.size());
   for (size_t i = 0; i < items.size(); ++i) {
-    int v = items[i].budget > 0 ? std::min(items[i].budget, K) : K;
-    if (even_only) v = std::max(2, v & ~1);
+    int v = items[i].budget < 0 ? 0 : (items[i].budget > 0 ? std::min(items[i].budget, K) : K);
+    if (even_only && v > 0) v = std::max(2, v & ~1);
     if (i > 0) v = std::min(v, b[i - 1]);
     b[i] = v;
   }
@@ -325,6 +326,7 @@ void BornRunner<T>::launch(int kk) {
   }
   const std::vector<int> bud = this->budgets(kk, true);
   const int K = bud[0];
+  if (K == 0) return;  // no member runs in this chunk
   if (this->gexec && K == this->k && bud[n - 1] == K) {
     CBS_HIP(hipGraphLaunch(this->gexec, this->s));
   } else {

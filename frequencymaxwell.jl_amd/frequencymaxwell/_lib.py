@@ -25,6 +25,8 @@ CBS_FP64, CBS_FP32, CBS_FP32_REFINE = 0, 1, 2
 CBS_BORN, CBS_BICGSTAB, CBS_GMRES = 0, 1, 2
 # stop rule
 CBS_STOP_RHS, CBS_STOP_KRYLOV = 0, 1
+# serving policy
+CBS_SERVE_SHARE, CBS_SERVE_LEAD = 0, 1
 
 EXPORTED = (
     "cbs_ctx_create", "cbs_ctx_destroy", "cbs_last_error", "cbs_version", "cbs_default_opts",
@@ -36,6 +38,7 @@ EXPORTED = (
     "cbs_solve_scene", "cbs_submit_scene", "cbs_field_create", "cbs_field_destroy", "cbs_field_get",
     "cbs_field_energy", "cbs_field_intensity", "cbs_field_poynting", "cbs_field_extract_plane",
     "cbs_solve_scene_field", "cbs_solve_scene_multi", "cbs_operator_apply_noalloc",
+    "cbs_server_policy",
 )
 
 
@@ -180,6 +183,7 @@ def _declare(lib):
                                       C.POINTER(C.c_uint64)]),
         "cbs_server_start": (s, [_P, C.c_int32]),
         "cbs_server_stop": (s, [_P]),
+        "cbs_server_policy": (s, [_P, C.c_int32, C.c_double]),
         "cbs_submit": (s, [_P, C.POINTER(Request), C.POINTER(C.c_uint64)]),
         "cbs_poll": (s, [_P, C.c_uint64, C.POINTER(C.c_int32), C.POINTER(SolveStats)]),
         "cbs_wait": (s, [_P, C.c_uint64, _D, _D, C.POINTER(SolveStats)]),

@@ -246,12 +246,20 @@ class CBSServer:
     by HBM footprint, share every sweep launch with the other resident solves
     (grids may differ) and leave as soon as their own stopping rule is met."""
 
-    def __init__(self, max_active: int = 8, ctx: L.Context | None = None):
+    def __init__(self, max_active: int = 8, ctx: L.Context | None = None, policy: str = "lead",
+                 kappa: float = 0.2):
+        """``policy``: "lead" (one request per chunk, first come first served with
+        requests of estimated work <= kappa x the mean passing) or "share" (equal
+        shares of every chunk in shared launches); see cbs_server_policy."""
         self.ctx = ctx or L.default_context()
         self._lib = L.load()
         self._shapes = {}
+        codes = {"share": L.CBS_SERVE_SHARE, "lead": L.CBS_SERVE_LEAD}
+        if policy not in codes:
+            raise ValueError(f"unknown serving policy {policy!r} (lead, share)")
         L.check(self._lib.cbs_server_start(self.ctx.handle, int(max_active)))
         self.running = True
+        L.check(self._lib.cbs_server_policy(self.ctx.handle, codes[policy], float(kappa)))
 
     def submit(self, V, spec: GreenSpec, S, opts: L.SolveOpts) -> int:
         """Queue one solve (V and S are copied); returns its ticket."""

@@ -359,6 +359,19 @@ cbs_status cbs_request_footprint(const cbs_problem_desc *desc, const cbs_solve_o
 cbs_status cbs_server_start(cbs_ctx *ctx, int32_t max_active);
 /* Stop the thread; pending and running requests end with CBS_ECANCELED. */
 cbs_status cbs_server_stop(cbs_ctx *ctx);
+/* How the resident requests share the GPU, chunk by chunk (after
+ * cbs_server_start; the default is CBS_SERVE_LEAD with kappa 0.2).
+ * CBS_SERVE_SHARE: every resident request gets an equal share of each chunk's
+ * GPU time, in shared launches (good median latency, but the largest requests
+ * stretch with the load).  CBS_SERVE_LEAD: each chunk runs one request, the
+ * oldest whose estimated work (padded voxels^(4/3) x log10(1/tol)) is at most
+ * kappa times the mean of the requests submitted so far, else the oldest:
+ * first come first served with small requests passing, which keeps the tail
+ * latency near one-at-a-time service and the median below it.  Admission,
+ * transfers, setup and the FP64 residual steps of every resident request
+ * overlap in both. */
+enum { CBS_SERVE_SHARE = 0, CBS_SERVE_LEAD = 1 };
+cbs_status cbs_server_policy(cbs_ctx *ctx, int32_t policy, double kappa);
 /* Queue a request (copies V and S); returns immediately with a ticket.
  * CBS_ENOMEM if the request can never fit the budget. */
 cbs_status cbs_submit(cbs_ctx *ctx, const cbs_request *req, uint64_t *ticket);

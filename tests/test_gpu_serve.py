@@ -47,8 +47,8 @@ def _mix(fm, oracle):
     return out
 
 
-def _serve(fm, reqs, max_active):
-    srv = fm.CBSServer(max_active=max_active)
+def _serve(fm, reqs, max_active, policy="lead"):
+    srv = fm.CBSServer(max_active=max_active, policy=policy)
     try:
         if max_active == 1:
             res = []
@@ -62,11 +62,14 @@ def _serve(fm, reqs, max_active):
         srv.stop()
 
 
-def test_serve_same_alone_and_in_mix(fm, oracle):
-    print("case serve alone vs mix", flush=True)
+@pytest.mark.parametrize("policy", ["lead", "share"])
+def test_serve_same_alone_and_in_mix(fm, oracle, policy):
+    """Each request's fields are the same bits alone and in a mix, under both
+    chunk schedules (one request per chunk, or equal shares in shared launches)."""
+    print(f"case serve alone vs mix, policy {policy}", flush=True)
     reqs = _mix(fm, oracle)
-    alone = _serve(fm, reqs, 1)
-    mixed = _serve(fm, reqs, 8)
+    alone = _serve(fm, reqs, 1, policy)
+    mixed = _serve(fm, reqs, 8, policy)
     for i, ((E1, H1, s1), (E2, H2, s2)) in enumerate(zip(alone, mixed)):
         print(f"  request {i}: iters {s1.iters} / {s2.iters}, resid {s1.resid:.3e}", flush=True)
         assert s1.iters == s2.iters and s1.converged == s2.converged
@@ -225,3 +228,18 @@ def test_serve_scenes_alone_mix_and_direct(fm, oracle):
     E1, H1, _ = res[1][0]
     assert rel(E1, ref.E) <= 1e-10
     assert rel(H1, ref.H) <= 1e-10
+
+
+def test_serve_policy_argument_checks(fm):
+    print("case serve policy checks", flush=True)
+    with pytest.raises(ValueError):
+        fm.CBSServer(max_active=2, policy="fastest")
+    srv = fm.CBSServer(max_active=2)
+    try:
+        with pytest.raises(ValueError):  # CBS_EINVAL
+            fm._lib.check(fm._lib.load().cbs_server_policy(srv.ctx.handle, 7, 0.2))
+        with pytest.raises(ValueError):
+            fm._lib.check(fm._lib.load().cbs_server_policy(srv.ctx.handle, 1, -1.0))
+        fm._lib.check(fm._lib.load().cbs_server_policy(srv.ctx.handle, 0, 0.0))
+    finally:
+        srv.stop()
