@@ -83,13 +83,14 @@ def random_spheres(fm, grid, rng):
     return fm.Medium(ph, eps_bg)
 
 
-def serve_request(fm, grid, rng, tol):
-    """One served solve as a scene: random-sphere phantom, plane wave at a random
-    angle within NA 1.0, s-polarised; the potential, incident field and source
-    term are built on the GPU."""
+def serve_request(fm, grid, rng, tol, med=None):
+    """One served solve as a scene: random-sphere phantom (or the given medium),
+    plane wave at a random angle within NA 1.0, s-polarised; the potential,
+    incident field and source term are built on the GPU."""
     from frequencymaxwell import solver as S
 
-    med = random_spheres(fm, grid, rng)
+    if med is None:
+        med = random_spheres(fm, grid, rng)
     sol = fm.ConvergentBornSolver(resolution=(50e-9,) * 3, grid_size=grid,
                                   boundary_conditions=boundary(fm), tolerance=tol,
                                   linear_solver="born", precision="fp32_refine")
@@ -101,18 +102,30 @@ def serve_request(fm, grid, rng, tol):
     return S.scene(sol, src, med)
 
 
-def serve_stream(fm, n_req, rate, max_active, seed, maxiters, field_every=0, policy="lead"):
+def serve_stream(fm, n_req, rate, max_active, seed, maxiters, field_every=0, policy="lead",
+                 pool=0):
     """Poisson arrivals (rate per second) of scene solves on grids drawn from
     SERVE_GRIDS with tolerance 1e-4 or 1e-6; returns latency percentiles and
-    throughput.  The requests are generated before the clock starts."""
+    throughput.  The requests are generated before the clock starts; with
+    pool > 0 the phantoms of each grid cycle through pool distinct media (the
+    plane wave and tolerance stay drawn per request), which bounds host memory
+    for long streams."""
     import threading
 
     rng = np.random.default_rng(seed)
-    reqs = []
+    reqs, media = [], {}
     for _ in range(n_req):
         g = SERVE_GRIDS[int(rng.integers(len(SERVE_GRIDS)))]
         tol = 1e-4 if rng.random() < 0.5 else 1e-6
-        reqs.append((serve_request(fm, g, rng, tol), tol))
+        med = None
+        if pool > 0:
+            lst = media.setdefault(g, [])
+            if len(lst) < pool:
+                lst.append(random_spheres(fm, g, rng))
+                med = lst[-1]
+            else:
+                med = lst[int(rng.integers(pool))]
+        reqs.append((serve_request(fm, g, rng, tol, med), tol))
     gaps = rng.exponential(1.0 / rate, n_req)
     srv = fm.CBSServer(max_active=max_active, policy=policy)
     tickets = [None] * n_req
@@ -143,12 +156,16 @@ def serve_stream(fm, n_req, rate, max_active, seed, maxiters, field_every=0, pol
             worst = max(worst, st.resid_true_fp64 / reqs[i][1])
             if field_every and i % field_every == 0:
                 fields[i] = E
+            if n_req >= 200 and (i + 1) % (n_req // 10) == 0:
+                print(f"[bench] {policy if max_active > 1 else 'one at a time'}: {i + 1}/{n_req} "
+                      f"requests done", file=sys.stderr, flush=True)
     finally:
         th.join()
         srv.stop()
     wall = time.perf_counter() - t0
     out = {
         "requests": n_req, "max_active": max_active, "policy": policy, "arrival_rate_per_s": rate,
+        "distinct_media_per_grid": pool if pool > 0 else "all",
         "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
         "mean_ms": float(np.mean(lat)), "voxel_iters_per_s": vi / wall,
         "requests_per_s": n_req / wall, "wall_s": wall, "converged": conv,
@@ -318,14 +335,45 @@ def batched_leg(fm, args):
             "one_at_a_time_voxel_iters_per_s": vi1 / dt1}
 
 
+def d2d_copy_ms(nbytes: int, reps: int = 5) -> float:
+    """Average duration (ms) of a device-to-device hipMemcpyAsync of nbytes
+    (torch's copy_ between two device buffers), timed with HIP events on
+    torch's stream: the runtime's copy beside the library's copy kernel."""
+    import torch
+
+    a = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    t1.record()
+    t1.synchronize()
+    ms = t0.elapsed_time(t1) / reps
+    del a, b
+    torch.cuda.empty_cache()
+    return ms
+
+
 def cpu_model() -> str:
+    """The host CPU as /proc/cpuinfo names it: model name, family/model/stepping
+    numbers and the nominal clock (a virtualised host may report a generic name)."""
+    info = {}
     try:
         for line in Path("/proc/cpuinfo").read_text().splitlines():
-            if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
+            if ":" in line:
+                k, v = (t.strip() for t in line.split(":", 1))
+                info.setdefault(k, v)
     except OSError:
-        pass
-    return "unknown"
+        return "unknown"
+    name = info.get("model name", "unknown")
+    fam = [f"{k} {info[k]}" for k in ("cpu family", "model", "stepping") if k in info]
+    if fam:
+        name += " (" + ", ".join(fam) + ")"
+    if "cpu MHz" in info:
+        name += f", {float(info['cpu MHz']):.0f} MHz"
+    return name
 
 
 def cpu_baseline(med, seconds: float):
@@ -384,23 +432,23 @@ def serving_leg(fm, args):
     rate = args.serve_rate
     if rate <= 0:
         # capacity: the same requests back to back, one at a time; arrivals at 80 % of it
-        probe = serve_stream(fm, args.serve_requests, 1e6, 1, 7, args.maxiters)
+        probe = serve_stream(fm, args.serve_requests, 1e6, 1, 7, args.maxiters, pool=args.serve_pool)
         cap = probe["requests_per_s"]
         rate = 0.8 * cap
         print(f"[bench] capacity {cap:.2f} requests/s -> arrivals at {rate:.2f}/s",
               file=sys.stderr, flush=True)
     if args.serve_mode in ("both", "batched"):
         cb = serve_stream(fm, args.serve_requests, rate, args.serve_active, 7, args.maxiters,
-                          field_every=args.serve_field_every, policy="lead")
+                          field_every=args.serve_field_every, policy="lead", pool=args.serve_pool)
         print(f"[bench] continuous batching, lead policy: p50 {cb['p50_ms']:.0f} ms "
               f"p99 {cb['p99_ms']:.0f} ms", file=sys.stderr, flush=True)
     if args.serve_mode in ("both", "share"):
         cbs = serve_stream(fm, args.serve_requests, rate, args.serve_active, 7, args.maxiters,
-                           policy="share")
+                           policy="share", pool=args.serve_pool)
         print(f"[bench] continuous batching, equal shares: p50 {cbs['p50_ms']:.0f} ms "
               f"p99 {cbs['p99_ms']:.0f} ms", file=sys.stderr, flush=True)
     if args.serve_mode in ("both", "sequential"):
-        seq = serve_stream(fm, args.serve_requests, rate, 1, 7, args.maxiters)
+        seq = serve_stream(fm, args.serve_requests, rate, 1, 7, args.maxiters, pool=args.serve_pool)
         print(f"[bench] one at a time: p50 {seq['p50_ms']:.0f} ms p99 {seq['p99_ms']:.0f} ms",
               file=sys.stderr, flush=True)
     return {"workload": "Poisson arrivals of scene requests (permittivity + plane wave in, cropped "
@@ -439,6 +487,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=0.0,
                     help="arrivals per second (0: 80 %% of the capacity measured first)")
     ap.add_argument("--serve-active", type=int, default=8)
+    ap.add_argument("--serve-pool", type=int, default=0,
+                    help="distinct phantoms per serving grid (0: one per request)")
     ap.add_argument("--serve-only", action="store_true")
     ap.add_argument("--serve-mode", default="both", choices=["both", "batched", "share", "sequential"])
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-illumination leg")
@@ -473,7 +523,8 @@ def main():
         ms, by = prob.time_sweeps("fp32", reps=args.profile_reps)
         cms = stream_copy_ms(1 << 30, reps=5)
         print(json.dumps({"sweeps_ms": dict(zip(KERNELS, ms)), "sweeps_bytes": dict(zip(KERNELS, by)),
-                          "copy_bytes": 1 << 30, "copy_ms": cms}), flush=True)
+                          "copy_bytes": 1 << 30, "copy_ms": cms,
+                          "hipmemcpy_d2d_ms": d2d_copy_ms(1 << 30, reps=5)}), flush=True)
         return
     fast = fm.make_opts(method=args.method, precision="fp32_refine", tol=1e-6, stop_rule="rhs",
                         maxiters=args.maxiters, use_graph=not args.no_graph,
