@@ -336,24 +336,41 @@ def batched_leg(fm, args):
 
 
 def d2d_copy_ms(nbytes: int, reps: int = 5) -> float:
-    """Average duration (ms) of a device-to-device hipMemcpyAsync of nbytes
-    (torch's copy_ between two device buffers), timed with HIP events on
-    torch's stream: the runtime's copy beside the library's copy kernel."""
-    import torch
+    """Average duration (ms) of a device-to-device hipMemcpyAsync of nbytes,
+    timed with HIP events on a stream of its own: the runtime's copy beside
+    the library's copy kernel (HIP runtime calls through ctypes)."""
+    import ctypes as C
 
-    a = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
-    b = torch.empty_like(a)
-    b.copy_(a)
-    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    t0.record()
-    for _ in range(reps):
-        b.copy_(a)
-    t1.record()
-    t1.synchronize()
-    ms = t0.elapsed_time(t1) / reps
-    del a, b
-    torch.cuda.empty_cache()
-    return ms
+    hip = C.CDLL("libamdhip64.so")
+
+    def ok(rc):
+        if rc != 0:
+            raise RuntimeError(f"HIP error {rc}")
+
+    a, b, s = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    e0, e1 = C.c_void_p(), C.c_void_p()
+    ok(hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)))
+    ok(hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)))
+    try:
+        ok(hip.hipStreamCreate(C.byref(s)))
+        ok(hip.hipEventCreate(C.byref(e0)))
+        ok(hip.hipEventCreate(C.byref(e1)))
+        d2d = 3  # hipMemcpyDeviceToDevice
+        ok(hip.hipMemcpyAsync(b, a, C.c_size_t(nbytes), d2d, s))
+        ok(hip.hipEventRecord(e0, s))
+        for _ in range(reps):
+            ok(hip.hipMemcpyAsync(b, a, C.c_size_t(nbytes), d2d, s))
+        ok(hip.hipEventRecord(e1, s))
+        ok(hip.hipEventSynchronize(e1))
+        ms = C.c_float()
+        ok(hip.hipEventElapsedTime(C.byref(ms), e0, e1))
+        ok(hip.hipEventDestroy(e0))
+        ok(hip.hipEventDestroy(e1))
+        ok(hip.hipStreamDestroy(s))
+    finally:
+        hip.hipFree(a)
+        hip.hipFree(b)
+    return ms.value / reps
 
 
 def cpu_model() -> str:

@@ -208,8 +208,11 @@ void Pipe<T>::build_impl(Ctx* c, void* dev, size_t cap, hipStream_t s) {
   L = PipeLaunch<T>{};
   L.S = h[0].S;
   L.fused = h[0].fuse_next;
+  L.bicg = h[0].pre >= PRE_BICG_P;
   for (int i = 0; i < np; ++i) {
     CBS_REQUIRE(h[i].S == L.S, "pipeline mixes averaging and non-averaging problems");
+    CBS_REQUIRE((h[i].pre >= PRE_BICG_P) == (L.bicg != 0),
+                "pipeline mixes BiCGStab and plain x forward pre-ops");
     CBS_REQUIRE(h[i].fuse_next == L.fused, "pipeline mixes fused and separate x sweeps");
     CBS_REQUIRE(!h[i].fuse_next || (h[i].post == POST_BORN && h[i].pre == PRE_V),
                 "fused x sweeps need the Born update");

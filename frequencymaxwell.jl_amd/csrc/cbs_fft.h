@@ -248,8 +248,14 @@ struct NoTileIO {};
 // instead of the tile (the first stage of a transform taking its data straight
 // from HBM); GOUT: it writes its outputs through st(q, m, v) instead of the tile
 // (the last stage writing straight to HBM; no barrier needed then).
+// TMAJ: butterfly b is (t = b mod N/R, q = b div N/R), so consecutive lanes
+// take consecutive butterflies of one pencil (pencils that lie along
+// contiguous memory: a first stage reads N/R consecutive elements of a pencil
+// per group of lanes, a last stage with N/R = NS writes NS consecutive ones);
+// otherwise (t = b div Q, q = b mod Q), consecutive lanes take consecutive
+// pencils (conflict-free on the padded tile rows).
 template <class T, int N, int Q, int QS, int DIR, int R, int NS, int NT, bool GIN, bool GOUT,
-          class LD, class ST>
+          class LD, class ST, bool TMAJ = false>
 __device__ __forceinline__ void stage_ct_io(Cx<T>* __restrict__ s, const Cx<T>* __restrict__ tw,
                                             int tid, const LD& ld, const ST& st) {
   constexpr int NB = N / R;
@@ -261,7 +267,7 @@ __device__ __forceinline__ void stage_ct_io(Cx<T>* __restrict__ s, const Cx<T>* 
   for (int k = 0; k < RD; ++k) {
     const int b = tid + k * NT;
     if (EXACT || b < TOTAL) {
-      const int q = b % Q, t = b / Q;
+      const int q = TMAJ ? b / NB : b % Q, t = TMAJ ? b % NB : b / Q;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         if constexpr (GIN) v[k][r] = ld(q, t + r * NB);
@@ -276,7 +282,7 @@ __device__ __forceinline__ void stage_ct_io(Cx<T>* __restrict__ s, const Cx<T>* 
   for (int k = 0; k < RD; ++k) {
     const int b = tid + k * NT;
     if (EXACT || b < TOTAL) {
-      const int q = b % Q, t = b / Q;
+      const int q = TMAJ ? b / NB : b % Q, t = TMAJ ? b % NB : b / Q;
       const int j = t % NS;
       if constexpr (NS > 1) {
 #pragma unroll
@@ -346,6 +352,40 @@ __device__ __forceinline__ void fft_ct_rev_head(Cx<T>* __restrict__ s,
     constexpr int R = plan_radix(N, nst - 1 - S, PK);
     stage_ct_io<T, N, Q, QS, DIR, R, NS, NT, GIN && S == 0, false>(s, tw, tid, ld, st);
     fft_ct_rev_head<T, N, Q, QS, DIR, NT, GIN, PK, S + 1, NS * R>(s, tw, tid, ld, st);
+  }
+}
+
+// Transforms of x tiles whose pencils lie along contiguous memory (the x
+// axis), for plans whose first radix leaves at least 16 butterflies per pencil
+// (x_direct): the inverse runs the plan in order and its first stage reads the
+// spectra through ld (t-major: 16 consecutive elements of a pencil per 16
+// lanes); the forward runs the plan in reverse order and its last stage, of
+// span N / plan_radix(N, 0), writes them through st (t-major again).  Two LDS
+// round trips and two barriers fewer than staging the pencils through the tile.
+template <int N>
+__host__ __device__ constexpr bool x_direct() {
+  return plan_nst(N) >= 2 && N / plan_radix(N, 0) >= 16;
+}
+template <class T, int N, int Q, int QS, int DIR, int NT, int S = 0, int NS = 1, class LD>
+__device__ __forceinline__ void fft_x_in(Cx<T>* __restrict__ s, const Cx<T>* __restrict__ tw,
+                                         int tid, const LD& ld) {
+  if constexpr (S < plan_nst(N)) {
+    constexpr int R = plan_radix(N, S);
+    stage_ct_io<T, N, Q, QS, DIR, R, NS, NT, S == 0, false, LD, NoTileIO, S == 0>(
+        s, tw, tid, ld, NoTileIO{});
+    fft_x_in<T, N, Q, QS, DIR, NT, S + 1, NS * R>(s, tw, tid, ld);
+  }
+}
+template <class T, int N, int Q, int QS, int DIR, int NT, int S = 0, int NS = 1, class ST>
+__device__ __forceinline__ void fft_x_out(Cx<T>* __restrict__ s, const Cx<T>* __restrict__ tw,
+                                          int tid, const ST& st) {
+  constexpr int nst = plan_nst(N);
+  if constexpr (S < nst) {
+    constexpr int R = plan_radix(N, nst - 1 - S);
+    constexpr bool last = S + 1 == nst;
+    stage_ct_io<T, N, Q, QS, DIR, R, NS, NT, false, last, NoTileIO, ST, last>(
+        s, tw, tid, NoTileIO{}, st);
+    fft_x_out<T, N, Q, QS, DIR, NT, S + 1, NS * R>(s, tw, tid, st);
   }
 }
 

@@ -127,6 +127,23 @@ __device__ __forceinline__ void gst(Cx<T>* p, const Cx<T>& c) {
 __device__ __forceinline__ double gld(const double* p) {
   return *(const __attribute__((address_space(1))) double*)(p);
 }
+// Streaming (nontemporal) accesses, for data read or written once per sweep
+// and not again before other traffic has evicted it (the Born iterate): they
+// leave the caches to the spectra that the next pass re-reads.
+template <class T>
+__device__ __forceinline__ Cx<T> gld_nt(const Cx<T>* p) {
+  typedef typename GVec2<T>::type V;
+  const V v = __builtin_nontemporal_load((const __attribute__((address_space(1))) V*)(p));
+  return mk<T>(v.x, v.y);
+}
+template <class T>
+__device__ __forceinline__ void gst_nt(Cx<T>* p, const Cx<T>& c) {
+  typedef typename GVec2<T>::type V;
+  V v;
+  v.x = c.re;
+  v.y = c.im;
+  __builtin_nontemporal_store(v, (__attribute__((address_space(1))) V*)(p));
+}
 
 // ---------------------------------------------------------------------------
 // radix plans: n = r0 * r1 * ... ; powers of two use radix 8 with one or two

@@ -29,6 +29,7 @@ struct PipeLaunch {
   int S;                            // pipelines per RHS (uniform over the table)
   int fused;                        // the xinv pass also runs the next sweep's x forward
                                     // (Born); the pipeline then starts at the y pass
+  int bicg;                         // the x forward forms BiCGStab vectors (PRE_BICG_*)
   int ng[4];                        // groups of the xfwd, y, z and xinv passes
   PassLaunch<T> g[4][kMaxGroups];
 };

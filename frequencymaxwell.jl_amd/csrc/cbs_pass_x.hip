@@ -11,7 +11,8 @@ constexpr int x_nt() {
   return NC > 0 ? xthreads(NC, (int)sizeof(Cx<T>), S) : kThreads;
 }
 
-template <class T, int S, int NC>
+// B: the table's pre-op is a BiCGStab vector update (PRE_BICG_*)
+template <class T, int S, int NC, bool B>
 __global__ void __launch_bounds__((x_nt<T, S, NC>()))
 k_xfwd(const DevProb<T>* __restrict__ probs, const int* __restrict__ bstart, int nprob) {
   const int pi = find_prob(bstart, nprob, blockIdx.x);
@@ -20,14 +21,14 @@ k_xfwd(const DevProb<T>* __restrict__ probs, const int* __restrict__ bstart, int
   const int tile = blockIdx.x - bstart[pi];
   const int tid = threadIdx.x;
   if constexpr (NC != 0) {
-    xfwd_tile<T, NC, S>(pr, tile, tid);
+    xfwd_tile<T, NC, S, B>(pr, tile, tid);
   } else {
     switch (pr.rt[0] ? 0 : pr.n[0]) {
 #define CBS_CASE_(L) \
-      case L: xfwd_tile<T, L, S>(pr, tile, tid); break;
+      case L: xfwd_tile<T, L, S, B>(pr, tile, tid); break;
       CBS_CT_LENGTHS(CBS_CASE_)
 #undef CBS_CASE_
-      default: xfwd_tile<T, 0, S>(pr, tile, tid); break;
+      default: xfwd_tile<T, 0, S, B>(pr, tile, tid); break;
     }
   }
 }
@@ -55,9 +56,13 @@ k_xinv(const DevProb<T>* __restrict__ probs, const int* __restrict__ bstart, int
 }
 
 template <class T, int S, int NC>
-static void go_xfwd(const PassLaunch<T>& G, hipStream_t s) {
-  hipLaunchKernelGGL((k_xfwd<T, S, NC>), dim3(G.blocks), dim3(x_nt<T, S, NC>()), G.lds, s, G.probs,
-                     G.bstart, G.nprob);
+static void go_xfwd(const PassLaunch<T>& G, bool bicg, hipStream_t s) {
+  if (bicg)
+    hipLaunchKernelGGL((k_xfwd<T, S, NC, true>), dim3(G.blocks), dim3(x_nt<T, S, NC>()), G.lds, s,
+                       G.probs, G.bstart, G.nprob);
+  else
+    hipLaunchKernelGGL((k_xfwd<T, S, NC, false>), dim3(G.blocks), dim3(x_nt<T, S, NC>()), G.lds, s,
+                       G.probs, G.bstart, G.nprob);
 }
 template <class T, int S, int NC>
 static void go_xinv(const PassLaunch<T>& G, bool fused, hipStream_t s) {
@@ -70,13 +75,13 @@ static void go_xinv(const PassLaunch<T>& G, bool fused, hipStream_t s) {
 }
 
 template <class T, int S>
-static void xfwd_len(const PassLaunch<T>& G, hipStream_t s) {
+static void xfwd_len(const PassLaunch<T>& G, bool bicg, hipStream_t s) {
   switch (G.len) {
 #define CBS_CASE_(L) \
-    case L: go_xfwd<T, S, L>(G, s); break;
+    case L: go_xfwd<T, S, L>(G, bicg, s); break;
       CBS_CT_LENGTHS(CBS_CASE_)
 #undef CBS_CASE_
-    default: go_xfwd<T, S, 0>(G, s); break;
+    default: go_xfwd<T, S, 0>(G, bicg, s); break;
   }
 }
 template <class T, int S>
@@ -93,8 +98,8 @@ static void xinv_len(const PassLaunch<T>& G, bool fused, hipStream_t s) {
 template <class T>
 void launch_xfwd(const PipeLaunch<T>& L, hipStream_t s) {
   for (int i = 0; i < L.ng[0]; ++i) {
-    if (L.S == 2) xfwd_len<T, 2>(L.g[0][i], s);
-    else xfwd_len<T, 1>(L.g[0][i], s);
+    if (L.S == 2) xfwd_len<T, 2>(L.g[0][i], L.bicg != 0, s);
+    else xfwd_len<T, 1>(L.g[0][i], L.bicg != 0, s);
   }
 }
 template <class T>

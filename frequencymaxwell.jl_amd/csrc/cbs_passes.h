@@ -42,13 +42,42 @@ __device__ __forceinline__ int find_prob(const int* __restrict__ bstart, int npr
 // rounded the same way in every kernel (no contraction into FMAs), so the
 // x forward of a fused x pass and of sweep A apply the same ramp
 template <class T>
-__device__ __forceinline__ Cx<T> ramp_at(const DevProb<T>& pr, int x, int y, int z) {
+__device__ __forceinline__ Cx<T> ramp_mul(const Cx<T>& a, const Cx<T>& b, const Cx<T>& c) {
 #pragma clang fp contract(off)
-  // generic loads: through global-address-space pointers the x inverse
-  // measured 5-7 % slower on MI355X
-  const Cx<T> a = pr.ramp[0][x], b = pr.ramp[1][y], c = pr.ramp[2][z];
   const Cx<T> ab = mk<T>(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
   return mk<T>(ab.re * c.re - ab.im * c.im, ab.re * c.im + ab.im * c.re);
+}
+template <class T>
+__device__ __forceinline__ Cx<T> ramp_at(const DevProb<T>& pr, int x, int y, int z) {
+  return ramp_mul(gld(pr.ramp[0] + x), gld(pr.ramp[1] + y), gld(pr.ramp[2] + z));
+}
+
+// Element k of a thread of a specialised x tile (e = tid + k NT: m = e % NC
+// along x, line p = e / NC of the tile).  Lanes of a slot beyond the tile's np
+// lines get the indices of the tile's first element, so the loads of a tile
+// are issued back to back, before the first result is used, with no branch
+// but the one that skips the slots beyond the tile's P lines in the last
+// element (whole waves there): per-element branches around the loads would
+// split them into basic blocks that the compiler schedules one at a time,
+// one HBM round trip each.
+struct XElem {
+  bool live;  // a slot of the tile (e < P NC)
+  bool ok;    // inside the tile (live and p < np)
+  int m, y, z;
+  long long g;  // voxel index (l y-z line, m)
+};
+template <int NC>
+__device__ __forceinline__ XElem x_elem(int e, int Pc, int l0, int np, int ny) {
+  XElem r;
+  const int m = e % NC, p = e / NC;
+  r.live = e < Pc * NC;
+  r.ok = r.live && p < np;
+  const int l = l0 + (r.ok ? p : 0);
+  r.m = r.ok ? m : 0;
+  r.y = l % ny;
+  r.z = l / ny;
+  r.g = (long long)l * NC + r.m;
+  return r;
 }
 
 // complex a b without FMA contraction: every kernel rounds it the same way,
@@ -63,6 +92,21 @@ __device__ __forceinline__ Cx<T> cmul_nc(const Cx<T>& a, const Cx<T>& b) {
 template <class T>
 __device__ __forceinline__ bool skip_prob(const DevProb<T>& pr) {
   return pr.st && (pr.st->done | pr.st->cyc);
+}
+
+// Offset of pencil q of an x tile in the spectrum workspace: q = (3 j + c) P
+// + p is component c of pipeline s0 + j on line l0 + p (lines beyond the
+// tile's np take line l0: their values are read but never used, so the loads
+// need no branch).
+template <class T>
+__device__ __forceinline__ long long x_pencil_off(const DevProb<T>& pr, int q, int P, int s0,
+                                                  int l0, int np, bool& in) {
+  const int p = q % P, jc = q / P;
+  in = p < np;
+  const int l = l0 + (in ? p : 0);
+  const int ny = pr.n[1];
+  return (long long)(s0 * 3 + jc) * pr.ws[2] + (long long)(l / ny) * pr.ws[1] +
+         (long long)(l % ny) * pr.ws[0];
 }
 
 // two adjacent complex FP32 values as one 16-byte access
@@ -118,6 +162,98 @@ __device__ __forceinline__ void xfwd_operand(const DevProb<T>& pr, long long g,
   }
 }
 
+// The x-forward operands of the E elements of a thread of a specialised x
+// tile (xfwd_operand for each), with every load issued before any value is
+// used; lanes outside the tile load the tile's first voxel and store nothing.
+template <class T, int E, bool BICG>
+__device__ __forceinline__ void xfwd_operands_tile(const DevProb<T>& pr, const XElem (&xe)[E],
+                                                   Cx<T> (&a)[E][3]) {
+  const long long nv = pr.nvox;
+  const Cx<T> zero = mk<T>(T(0), T(0));
+  if (!BICG) {
+#pragma unroll
+    for (int k = 0; k < E; ++k)
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        a[k][c] = (k < E - 1 || xe[k].live) ? gld(pr.in0 + c * nv + xe[k].g) : zero;
+  } else if (pr.pre == PRE_BICG_P) {
+    // one component at a time (its loads for every element issued together):
+    // all five vectors of all three components would cost the tile occupancy
+    const RState* st = pr.st;
+    const Cx<T> al = mk<T>((T)st->alpha_re, (T)st->alpha_im);
+    const Cx<T> om = mk<T>((T)st->omega_re, (T)st->omega_im);
+    const Cx<T> be = mk<T>((T)st->beta_re, (T)st->beta_im);
+    const bool pend = st->pend_x != 0;
+    Cx<T>* pp = const_cast<Cx<T>*>(pr.in0);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      Cx<T> po[E], r[E], vq[E];
+      if (pend) {  // the previous iteration's x and r (Krylov.jl: x += alpha p; x += omega s)
+        Cx<T> sv[E], tv[E], xv[E];
+#pragma unroll
+        for (int k = 0; k < E; ++k) {
+          const long long j = c * nv + xe[k].g;
+          po[k] = sv[k] = tv[k] = xv[k] = vq[k] = zero;
+          if (k < E - 1 || xe[k].live) {
+            po[k] = gld(pp + j);
+            sv[k] = gld(pr.kv[2] + j);
+            tv[k] = gld(pr.kv[3] + j);
+            xv[k] = gld(pr.kv[1] + j);
+            vq[k] = gld(pr.kv[4] + j);
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < E; ++k) {
+          const long long j = c * nv + xe[k].g;
+          r[k] = sv[k] - om * tv[k];
+          const Cx<T> x1 = xv[k] + al * po[k];
+          if (xe[k].ok) {
+            gst(pr.kv[1] + j, x1 + om * sv[k]);
+            gst(pr.kv[0] + j, r[k]);
+          }
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < E; ++k) {
+          const long long j = c * nv + xe[k].g;
+          po[k] = r[k] = vq[k] = zero;
+          if (k < E - 1 || xe[k].live) {
+            po[k] = gld(pp + j);
+            r[k] = gld(pr.kv[0] + j);
+            vq[k] = gld(pr.kv[4] + j);
+          }
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < E; ++k) {
+        const Cx<T> pn = r[k] + be * (po[k] - om * vq[k]);
+        if (xe[k].ok) gst(pp + c * nv + xe[k].g, pn);
+        a[k][c] = pn;
+      }
+    }
+  } else {  // PRE_BICG_S
+    const Cx<T> al = mk<T>((T)pr.st->alpha_re, (T)pr.st->alpha_im);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      Cx<T> rv[E], vq[E];
+#pragma unroll
+      for (int k = 0; k < E; ++k) {
+        rv[k] = vq[k] = zero;
+        if (k < E - 1 || xe[k].live) {
+          rv[k] = gld(pr.kv[0] + c * nv + xe[k].g);
+          vq[k] = gld(pr.kv[4] + c * nv + xe[k].g);
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < E; ++k) {
+        const Cx<T> sv = rv[k] - al * vq[k];
+        if (xe[k].ok) gst(pr.kv[2] + c * nv + xe[k].g, sv);
+        a[k][c] = sv;
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // sweep A: x forward transform, pre-op u = V x (or x), shift ramps.
 // Every path transforms the 3 S P pencils of its P lines (the S ramped copies
@@ -153,19 +289,30 @@ __device__ __forceinline__ void xfwd_ct_store(const DevProb<T>& pr, Cx<T>* __res
       }
     }
     __syncthreads();
-    fft_ct<T, NC, Qc, QSc, -1, NT>(lds, lds_twiddles<T>(), tid);
+    if constexpr (x_direct<NC>()) {
+      // the last stage writes the spectra straight to HBM
+      const auto st = [&](int q, int m, const Cx<T>& v) {
+        bool in;
+        const long long off = x_pencil_off(pr, q, Pc, s0, l0, np, in);
+        if (in) gst(pr.w + off + m, v);
+      };
+      fft_x_out<T, NC, Qc, QSc, -1, NT>(lds, lds_twiddles<T>(), tid, st);
+    } else {
+      fft_ct<T, NC, Qc, QSc, -1, NT>(lds, lds_twiddles<T>(), tid);
 #pragma unroll
-    for (int k = 0; k < E; ++k) {
-      const int e = tid + k * NT;
-      const int m = e % NC, p = e / NC;
-      if (e < Pc * NC && p < np) {
-        const int l = l0 + p;
-        const long long wo = (long long)(l / ny) * pr.ws[1] + (long long)(l % ny) * pr.ws[0] + m;
+      for (int k = 0; k < E; ++k) {
+        const int e = tid + k * NT;
+        const int m = e % NC, p = e / NC;
+        if (e < Pc * NC && p < np) {
+          const int l = l0 + p;
+          const long long wo =
+              (long long)(l / ny) * pr.ws[1] + (long long)(l % ny) * pr.ws[0] + m;
 #pragma unroll
-        for (int j = 0; j < J; ++j) {
+          for (int j = 0; j < J; ++j) {
 #pragma unroll
-          for (int c = 0; c < 3; ++c)
-            gst(pr.w + ((s0 + j) * 3 + c) * pr.ws[2] + wo, lds[m * QSc + (3 * j + c) * Pc + p]);
+            for (int c = 0; c < 3; ++c)
+              gst(pr.w + ((s0 + j) * 3 + c) * pr.ws[2] + wo, lds[m * QSc + (3 * j + c) * Pc + p]);
+          }
         }
       }
     }
@@ -207,7 +354,9 @@ __device__ __forceinline__ void xfwd_rt_store(const DevProb<T>& pr, Cx<T>* a, Cx
   }
 }
 
-template <class T, int NC, int S>
+// BICG: the tile's pre-op is PRE_BICG_P or PRE_BICG_S (their register-heavy
+// operand loads are compiled only into the BiCGStab kernels)
+template <class T, int NC, int S, bool BICG>
 __device__ __forceinline__ void xfwd_tile(const DevProb<T>& pr, int tile, int tid) {
   constexpr bool CT = NC > 0;
   constexpr int eb = (int)sizeof(Cx<T>);
@@ -223,28 +372,38 @@ __device__ __forceinline__ void xfwd_tile(const DevProb<T>& pr, int tile, int ti
   if constexpr (CT) {
     constexpr int Pc = xlines(CT ? NC : 8, eb, S, false);
     constexpr int E = (Pc * NC + NT - 1) / NT;
+    // every load of the tile (V, the operands, the ramp factors) is issued
+    // before the first value is used: one HBM round trip per tile
+    XElem xe[E];
+#pragma unroll
+    for (int k = 0; k < E; ++k) xe[k] = x_elem<NC>(tid + k * NT, Pc, l0, np, ny);
+    Cx<T> vv[E];
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      vv[k] = mk<T>(T(1), T(0));
+      if (useV && (k < E - 1 || xe[k].live)) vv[k] = gld(pr.V + xe[k].g);
+    }
+    Cx<T> ra[E][3];
+    if (pr.has_ramp) {
+#pragma unroll
+      for (int k = 0; k < E; ++k) {
+        if (k < E - 1 || xe[k].live) {
+          ra[k][0] = gld(pr.ramp[0] + xe[k].m);
+          ra[k][1] = gld(pr.ramp[1] + xe[k].y);
+          ra[k][2] = gld(pr.ramp[2] + xe[k].z);
+        }
+      }
+    }
+    Cx<T> xv[E][3];
+    xfwd_operands_tile<T, E, BICG>(pr, xe, xv);
     Cx<T> u[E][3];
     Cx<T> rp[E];
 #pragma unroll
     for (int k = 0; k < E; ++k) {
-      const int e = tid + k * NT;
-      const int m = e % NC, p = e / NC;
-      rp[k] = mk<T>(T(1), T(0));
-      if (e < Pc * NC && p < np) {
-        const long long g = (long long)(l0 + p) * NC + m;
-        const Cx<T> vv = useV ? gld(pr.V + g) : mk<T>(T(1), T(0));
-        Cx<T> xv[3];
-        xfwd_operand(pr, g, xv);
+      rp[k] = pr.has_ramp && xe[k].ok ? ramp_mul(ra[k][0], ra[k][1], ra[k][2]) : mk<T>(T(1), T(0));
 #pragma unroll
-        for (int c = 0; c < 3; ++c) u[k][c] = useV ? cmul_nc(vv, xv[c]) : xv[c];
-        if (pr.has_ramp) {
-          const int l = l0 + p;
-          rp[k] = ramp_at(pr, m, l % ny, l / ny);
-        }
-      } else {
-#pragma unroll
-        for (int c = 0; c < 3; ++c) u[k][c] = mk<T>(T(0), T(0));
-      }
+      for (int c = 0; c < 3; ++c)
+        u[k][c] = !xe[k].ok ? mk<T>(T(0), T(0)) : useV ? cmul_nc(vv[k], xv[k][c]) : xv[k][c];
     }
     xfwd_ct_store<T, NC, S, NT, E, Pc>(pr, lds, u, rp, l0, np, tid);
   } else {
@@ -627,6 +786,58 @@ __device__ __forceinline__ void xinv_operands(const DevProb<T>& pr, long long g,
   }
 }
 
+// The same operands for the E elements of a thread of a specialised x tile,
+// all loads issued before any is used (lanes outside the tile read the tile's
+// first voxel; their values are never used).  BORN: the fused Born pass
+// (V, x = in0, b = in1).
+template <class T, int E, bool BORN>
+__device__ __forceinline__ void xinv_operands_tile(const DevProb<T>& pr, const XElem (&xe)[E],
+                                                   Cx<T> (&vv)[E], Cx<T> (&a0)[E][3],
+                                                   Cx<T> (&a1)[E][3]) {
+  const long long nv = pr.nvox;
+  const int post = BORN ? (int)POST_BORN : pr.post;
+  const bool use_b = post == POST_OPERATOR && (pr.b_re != 0.0 || pr.b_im != 0.0);
+  const bool need0 = post != POST_STORE;
+  const bool need1 = post == POST_BORN || post == POST_RESID || use_b ||
+                     (post == POST_PMATVEC && pr.in1 != nullptr);
+  const Cx<T>* src1 = post == POST_OPERATOR ? pr.out0 : pr.in1;
+  const Cx<T> zero = mk<T>(T(0), T(0));
+#pragma unroll
+  for (int k = 0; k < E; ++k) vv[k] = zero;
+#pragma unroll
+  for (int k = 0; k < E; ++k)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) a0[k][c] = a1[k][c] = zero;
+  if (post >= POST_BORN) {
+#pragma unroll
+    for (int k = 0; k < E; ++k)
+      if (k < E - 1 || xe[k].live) vv[k] = gld(pr.V + xe[k].g);
+  }
+  if (post == POST_BORN) {  // the iterate x streams (read once per sweep)
+#pragma unroll
+    for (int k = 0; k < E; ++k)
+      if (k < E - 1 || xe[k].live) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) a0[k][c] = gld_nt(pr.in0 + c * nv + xe[k].g);
+      }
+  } else if (need0) {
+#pragma unroll
+    for (int k = 0; k < E; ++k)
+      if (k < E - 1 || xe[k].live) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) a0[k][c] = gld(pr.in0 + c * nv + xe[k].g);
+      }
+  }
+  if (need1) {
+#pragma unroll
+    for (int k = 0; k < E; ++k)
+      if (k < E - 1 || xe[k].live) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) a1[k][c] = gld(src1 + c * nv + xe[k].g);
+      }
+  }
+}
+
 // The fused epilogue of voxel g from w = G u and the operands above;
 // acc collects the FP64 partial sums of the post-op.  For POST_BORN w
 // returns the new iterate written to out0.  BORN: the caller runs POST_BORN
@@ -666,7 +877,7 @@ __device__ __forceinline__ void xinv_finish(const DevProb<T>& pr, long long g, C
         const Cx<T> t = mk<T>(a1[c].re - a0[c].re + w[c].re, a1[c].im - a0[c].im + w[c].im);
         const Cx<T> r = cmul_nc(gam, t);
         w[c] = mk<T>(a0[c].re + r.re, a0[c].im + r.im);
-        gst(pr.out0 + c * nv + g, w[c]);
+        gst_nt(pr.out0 + c * nv + g, w[c]);
         acc[0] += (double)(r.re * r.re + r.im * r.im);
       }
     } else if (post == POST_RESID) {
@@ -717,49 +928,98 @@ __device__ __forceinline__ void xinv_tile(const DevProb<T>& pr, int tile, int ti
   if constexpr (CT) {
     constexpr int Pc = FUSE ? xlines_f(CT ? NC : 8, eb, S, false) : xlines(CT ? NC : 8, eb, S, false);
     constexpr int E = (Pc * NC + NT - 1) / NT;
-    // the spectra and ramps of the tile are loaded before the transforms; the
-    // epilogue operands after them (LATE; with the two-stage transforms their
-    // registers would cost the tile occupancy) or, LATE false, before
-    constexpr bool LATE = CBS_XOPS_LATE;
-    Cx<T> w[S][E][3];
-    Cx<T> rp[E];
-    Cx<T> vv[E], a0[E][3], a1[E][3];
-#pragma unroll
-    for (int k = 0; k < E; ++k) {
-      const int e = tid + k * NT;
-      const int m = e % NC, p = e / NC;
-      const bool valid = e < Pc * NC && p < np;
-      const int l = l0 + p;
-      const long long wo = (long long)(l / ny) * pr.ws[1] + (long long)(l % ny) * pr.ws[0] + m;
-#pragma unroll
-      for (int s = 0; s < S; ++s) {
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-          w[s][k][c] = valid ? gld(pr.w + (s * 3 + c) * pr.ws[2] + wo) : mk<T>(T(0), T(0));
-      }
-      rp[k] = pr.has_ramp && valid ? ramp_at(pr, m, l % ny, l / ny) : mk<T>(T(1), T(0));
-      if constexpr (!LATE) xinv_operands(pr, (long long)l * NC + m, valid, vv[k], a0[k], a1[k]);
-    }
+    // The tile's spectra and ramp factors load before the transforms, the
+    // epilogue operands after them (with the two-stage transforms their
+    // registers would cost the tile occupancy).  Each group is issued as a
+    // whole before its first value is used: two HBM round trips per tile.
+    // x_direct lengths read the spectra in the first stage of the transform
+    // (no staging through registers and the tile); the others load them into
+    // registers first.
+    constexpr bool XD = x_direct<NC>();
     constexpr int J = xjoint(S, false);  // pipelines per transform
     constexpr int Qc = 3 * J * Pc;
     constexpr int QSc = xpad_stride(Qc);
+    constexpr int SW = XD ? 1 : S;  // staged spectra (unused when XD)
+    Cx<T> w[SW][E][3];
+    Cx<T> xw[E][3];  // the averaged inverse transform of the tile's elements
+    Cx<T> rp[E];
+    Cx<T> vv[E], a0[E][3], a1[E][3];
+    XElem xe[E];
 #pragma unroll
-    for (int s0 = 0; s0 < S; s0 += J) {
-      if (s0 > 0) __syncthreads();  // the next pipeline reuses the tile
+    for (int k = 0; k < E; ++k) xe[k] = x_elem<NC>(tid + k * NT, Pc, l0, np, ny);
+    if constexpr (!XD) {
 #pragma unroll
       for (int k = 0; k < E; ++k) {
-        const int e = tid + k * NT;
-        if (e < Pc * NC) {
-          const int m = e % NC, p = e / NC;
+        const long long wo =
+            (long long)xe[k].z * pr.ws[1] + (long long)xe[k].y * pr.ws[0] + xe[k].m;
 #pragma unroll
-          for (int j = 0; j < J; ++j) {
+        for (int s = 0; s < S; ++s) {
 #pragma unroll
-            for (int c = 0; c < 3; ++c) lds[m * QSc + (3 * j + c) * Pc + p] = w[s0 + j][k][c];
+          for (int c = 0; c < 3; ++c) {
+            w[s][k][c] = mk<T>(T(0), T(0));
+            if (k < E - 1 || xe[k].live) w[s][k][c] = gld(pr.w + (s * 3 + c) * pr.ws[2] + wo);
           }
         }
       }
-      __syncthreads();
-      fft_ct<T, NC, Qc, QSc, 1, NT>(lds, lds_twiddles<T>(), tid);
+    }
+    // ramp factors (multiplied once the transforms have run)
+    Cx<T> ra[E][3];
+#pragma unroll
+    for (int k = 0; k < E; ++k) ra[k][0] = ra[k][1] = ra[k][2] = mk<T>(T(1), T(0));
+    if (pr.has_ramp) {
+#pragma unroll
+      for (int k = 0; k < E; ++k) {
+        if (k < E - 1 || xe[k].live) {
+          ra[k][0] = gld(pr.ramp[0] + xe[k].m);
+          ra[k][1] = gld(pr.ramp[1] + xe[k].y);
+          ra[k][2] = gld(pr.ramp[2] + xe[k].z);
+        }
+      }
+    }
+    if constexpr (!XD) {
+#pragma unroll
+      for (int k = 0; k < E; ++k) {
+        if (!xe[k].ok) {
+#pragma unroll
+          for (int s = 0; s < S; ++s)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) w[s][k][c] = mk<T>(T(0), T(0));
+        }
+      }
+    }
+    if constexpr (!CBS_XOPS_LATE) xinv_operands_tile<T, E, FUSE>(pr, xe, vv, a0, a1);
+#pragma unroll
+    for (int s0 = 0; s0 < S; s0 += J) {
+      if (s0 > 0) __syncthreads();  // the next pipeline reuses the tile
+      if constexpr (XD) {
+        const auto ld = [&](int q, int m) -> Cx<T> {
+          bool in;
+          const long long off = x_pencil_off(pr, q, Pc, s0, l0, np, in);
+          const Cx<T> v = gld(pr.w + off + m);
+          return in ? v : mk<T>(T(0), T(0));
+        };
+        fft_x_in<T, NC, Qc, QSc, 1, NT>(lds, lds_twiddles<T>(), tid, ld);
+      } else {
+#pragma unroll
+        for (int k = 0; k < E; ++k) {
+          const int e = tid + k * NT;
+          if (e < Pc * NC) {
+            const int m = e % NC, p = e / NC;
+#pragma unroll
+            for (int j = 0; j < J; ++j) {
+#pragma unroll
+              for (int c = 0; c < 3; ++c) lds[m * QSc + (3 * j + c) * Pc + p] = w[s0 + j][k][c];
+            }
+          }
+        }
+        __syncthreads();
+        fft_ct<T, NC, Qc, QSc, 1, NT>(lds, lds_twiddles<T>(), tid);
+      }
+      if (s0 == 0) {
+#pragma unroll
+        for (int k = 0; k < E; ++k)
+          rp[k] = pr.has_ramp ? ramp_mul(ra[k][0], ra[k][1], ra[k][2]) : mk<T>(T(1), T(0));
+      }
 #pragma unroll
       for (int k = 0; k < E; ++k) {
         const int e = tid + k * NT;
@@ -774,25 +1034,21 @@ __device__ __forceinline__ void xinv_tile(const DevProb<T>& pr, int tile, int ti
             for (int c = 0; c < 3; ++c) {
               const Cx<T> v = lds[m * QSc + (3 * j + c) * Pc + p];
               const Cx<T> t = pr.has_ramp ? cmul_nc(r, v) : v;
-              w[0][k][c] = s == 0 ? t : w[0][k][c] + t;
+              xw[k][c] = s == 0 ? t : xw[k][c] + t;
             }
           }
         }
       }
     }
+    if constexpr (CBS_XOPS_LATE) xinv_operands_tile<T, E, FUSE>(pr, xe, vv, a0, a1);
 #pragma unroll
     for (int k = 0; k < E; ++k) {
-      const int e = tid + k * NT;
-      const int m = e % NC, p = e / NC;
-      if constexpr (LATE) {
-        xinv_operands(pr, (long long)(l0 + p) * NC + m, e < Pc * NC && p < np, vv[k], a0[k], a1[k]);
-      }
-      if (e < Pc * NC && p < np) {
+      if (xe[k].ok) {
         if (S == 2) {
 #pragma unroll
-          for (int c = 0; c < 3; ++c) w[0][k][c] = T(0.5) * w[0][k][c];
+          for (int c = 0; c < 3; ++c) xw[k][c] = T(0.5) * xw[k][c];
         }
-        xinv_finish<T, FUSE>(pr, (long long)(l0 + p) * NC + m, w[0][k], vv[k], a0[k], a1[k], acc);
+        xinv_finish<T, FUSE>(pr, xe[k].g, xw[k], vv[k], a0[k], a1[k], acc);
       }
     }
     if constexpr (FUSE) {
@@ -800,10 +1056,9 @@ __device__ __forceinline__ void xinv_tile(const DevProb<T>& pr, int tile, int ti
       Cx<T> u[E][3];
 #pragma unroll
       for (int k = 0; k < E; ++k) {
-        const int e = tid + k * NT;
-        const bool valid = e < Pc * NC && (e / NC) < np;
 #pragma unroll
-        for (int c = 0; c < 3; ++c) u[k][c] = valid ? cmul_nc(vv[k], w[0][k][c]) : mk<T>(T(0), T(0));
+        for (int c = 0; c < 3; ++c)
+          u[k][c] = xe[k].ok ? cmul_nc(vv[k], xw[k][c]) : mk<T>(T(0), T(0));
       }
       __syncthreads();  // other waves may still read the inverse tile
       xfwd_ct_store<T, NC, S, NT, E, Pc>(pr, lds, u, rp, l0, np, tid);
