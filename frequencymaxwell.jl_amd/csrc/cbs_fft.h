@@ -366,14 +366,18 @@ template <int N>
 __host__ __device__ constexpr bool x_direct() {
   return plan_nst(N) >= 2 && N / plan_radix(N, 0) >= 16;
 }
-template <class T, int N, int Q, int QS, int DIR, int NT, int S = 0, int NS = 1, class LD>
+// mid() runs after the first stage (a caller issues loads there that the
+// remaining stages hide)
+template <class T, int N, int Q, int QS, int DIR, int NT, int S = 0, int NS = 1, class LD,
+          class MID>
 __device__ __forceinline__ void fft_x_in(Cx<T>* __restrict__ s, const Cx<T>* __restrict__ tw,
-                                         int tid, const LD& ld) {
+                                         int tid, const LD& ld, const MID& mid) {
   if constexpr (S < plan_nst(N)) {
     constexpr int R = plan_radix(N, S);
     stage_ct_io<T, N, Q, QS, DIR, R, NS, NT, S == 0, false, LD, NoTileIO, S == 0>(
         s, tw, tid, ld, NoTileIO{});
-    fft_x_in<T, N, Q, QS, DIR, NT, S + 1, NS * R>(s, tw, tid, ld);
+    if constexpr (S == 0) mid();
+    fft_x_in<T, N, Q, QS, DIR, NT, S + 1, NS * R>(s, tw, tid, ld, mid);
   }
 }
 template <class T, int N, int Q, int QS, int DIR, int NT, int S = 0, int NS = 1, class ST>

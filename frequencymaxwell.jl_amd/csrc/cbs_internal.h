@@ -27,6 +27,12 @@ constexpr int kLdsBudget = 65536; // LDS bytes a pass tile may use (two blocks p
 #ifndef CBS_XOPS_LATE
 #define CBS_XOPS_LATE 1
 #endif
+// CBS_XOPS_MID: the complex-FP32 fused x pass of lengths whose spectra the
+// first FFT stage reads (x_direct) loads its epilogue operands between the two
+// stages of the inverse
+#ifndef CBS_XOPS_MID
+#define CBS_XOPS_MID 1
+#endif
 #ifndef CBS_XJOINT
 #define CBS_XJOINT 1
 #endif

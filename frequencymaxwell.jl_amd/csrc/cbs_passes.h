@@ -936,6 +936,10 @@ __device__ __forceinline__ void xinv_tile(const DevProb<T>& pr, int tile, int ti
     // (no staging through registers and the tile); the others load them into
     // registers first.
     constexpr bool XD = x_direct<NC>();
+    // the fused pass loads its complex-FP32 operands between the two inverse
+    // stages (in FP64, or in the plain x inverse with its wider post-ops,
+    // their registers cost the tile occupancy more than the overlap gains)
+    constexpr bool XMID = XD && FUSE && CBS_XOPS_MID && sizeof(T) == 4;
     constexpr int J = xjoint(S, false);  // pipelines per transform
     constexpr int Qc = 3 * J * Pc;
     constexpr int QSc = xpad_stride(Qc);
@@ -998,7 +1002,17 @@ __device__ __forceinline__ void xinv_tile(const DevProb<T>& pr, int tile, int ti
           const Cx<T> v = gld(pr.w + off + m);
           return in ? v : mk<T>(T(0), T(0));
         };
-        fft_x_in<T, NC, Qc, QSc, 1, NT>(lds, lds_twiddles<T>(), tid, ld);
+        // the epilogue operands load after the first stage (its registers are
+        // free again) and arrive during the second
+        const auto mid = [&]() {
+          if (s0 == 0) {  // the ramp factors have arrived with the spectra
+#pragma unroll
+            for (int k = 0; k < E; ++k)
+              rp[k] = pr.has_ramp ? ramp_mul(ra[k][0], ra[k][1], ra[k][2]) : mk<T>(T(1), T(0));
+          }
+          if (XMID && s0 + J >= S) xinv_operands_tile<T, E, FUSE>(pr, xe, vv, a0, a1);
+        };
+        fft_x_in<T, NC, Qc, QSc, 1, NT>(lds, lds_twiddles<T>(), tid, ld, mid);
       } else {
 #pragma unroll
         for (int k = 0; k < E; ++k) {
@@ -1015,7 +1029,7 @@ __device__ __forceinline__ void xinv_tile(const DevProb<T>& pr, int tile, int ti
         __syncthreads();
         fft_ct<T, NC, Qc, QSc, 1, NT>(lds, lds_twiddles<T>(), tid);
       }
-      if (s0 == 0) {
+      if (!XD && s0 == 0) {
 #pragma unroll
         for (int k = 0; k < E; ++k)
           rp[k] = pr.has_ramp ? ramp_mul(ra[k][0], ra[k][1], ra[k][2]) : mk<T>(T(1), T(0));
@@ -1040,7 +1054,7 @@ __device__ __forceinline__ void xinv_tile(const DevProb<T>& pr, int tile, int ti
         }
       }
     }
-    if constexpr (CBS_XOPS_LATE) xinv_operands_tile<T, E, FUSE>(pr, xe, vv, a0, a1);
+    if constexpr (CBS_XOPS_LATE && !XMID) xinv_operands_tile<T, E, FUSE>(pr, xe, vv, a0, a1);
 #pragma unroll
     for (int k = 0; k < E; ++k) {
       if (xe[k].ok) {
