@@ -157,7 +157,11 @@ struct Pipe {
 // device arrays describing the RHS of a Born/Krylov solve (for finalize kernels)
 struct StateSet {
   DevBuf sts, parts, ntiles, sums;
+  // chunk sums of the partial rows (kChunkRows rows each) for finalizes that
+  // first sum their partials on many CUs: cparts[i] holds cnt[i] rows
+  DevBuf cbuf, cparts, cnt;
   int n = 0;
+  int maxchunks = 0;
   void build(Ctx* c, const std::vector<RState*>& st, const std::vector<double*>& parts_h,
              const std::vector<int>& nt, hipStream_t s);
 };

@@ -256,8 +256,11 @@ void Pipe<T>::build_impl(Ctx* c, void* dev, size_t cap, hipStream_t s) {
       }
       if (g.n == 0) continue;
       hb.push_back(g.blocks);
-      // the xinv epilogue reuses the tile LDS for its block reduction
+      // the xinv epilogue reuses the tile LDS for its block reduction (through
+      // LDS for blocks of up to kThreads threads, else wave shuffles)
       g.lds = std::max(g.lds, (size_t)((kMaxXThreads / 64) * kNumPart * sizeof(double)));
+      if (pass == 3)
+        g.lds = std::max(g.lds, (size_t)(kNumPart * kXRedStride<kThreads>() * sizeof(double)));
       if (g.lds > 160 * 1024)
         throw Error(CBS_EINVAL, "grid axis too long for an LDS tile (" + std::to_string(g.lds) +
                                     " bytes)");
@@ -327,9 +330,22 @@ void StateSet::build(Ctx* c, const std::vector<RState*>& st, const std::vector<d
   parts.ensure(c, n * sizeof(double*));
   ntiles.ensure(c, n * sizeof(int));
   sums.ensure(c, n * kNumPart * sizeof(double));
+  maxchunks = 1;
+  std::vector<int> nc(n);
+  for (int i = 0; i < n; ++i) {
+    nc[i] = std::max(1, (nt[i] + kChunkRows - 1) / kChunkRows);
+    maxchunks = std::max(maxchunks, nc[i]);
+  }
+  cbuf.ensure(c, (size_t)n * maxchunks * kNumPart * sizeof(double));
+  std::vector<double*> cp(n);
+  for (int i = 0; i < n; ++i) cp[i] = cbuf.as<double>() + (size_t)i * maxchunks * kNumPart;
+  cparts.ensure(c, n * sizeof(double*));
+  cnt.ensure(c, n * sizeof(int));
   CBS_HIP(hipMemcpyAsync(sts.p, st.data(), n * sizeof(RState*), hipMemcpyHostToDevice, s));
   CBS_HIP(hipMemcpyAsync(parts.p, ph.data(), n * sizeof(double*), hipMemcpyHostToDevice, s));
   CBS_HIP(hipMemcpyAsync(ntiles.p, nt.data(), n * sizeof(int), hipMemcpyHostToDevice, s));
+  CBS_HIP(hipMemcpyAsync(cparts.p, cp.data(), n * sizeof(double*), hipMemcpyHostToDevice, s));
+  CBS_HIP(hipMemcpyAsync(cnt.p, nc.data(), n * sizeof(int), hipMemcpyHostToDevice, s));
   CBS_HIP(hipStreamSynchronize(s));
 }
 

@@ -107,7 +107,8 @@ __device__ __forceinline__ void push_hist(RState* st, double r2) {
 // a non-finite or overflowing squared norm
 __device__ __forceinline__ bool bad(double v) { return !(v <= 1e300); }
 
-// One block per RHS; ntiles[i] = partial blocks of the producing kernel.
+// One block per RHS; ntiles[i] = partial rows (the chunk sums of the partial
+// blocks of the producing kernel).
 // Iteration k of Krylov.jl's bicgstab: kit counts it (at its first matvec),
 // iters counts the operator applications (two per iteration, plus the warm
 // start's r0 = gamma (b - A x0)).  BF_OMEGA takes everything the end of the
@@ -224,9 +225,43 @@ static void set_kv(DevProb<T>& d, const BiVec<T>& q) {
   d.kv[4] = q.v;
 }
 
+// First level of a finalize: block (b, i) sums rows [b kChunkRows, (b + 1)
+// kChunkRows) of the partials of right-hand side i, column by column, into
+// row b of its chunk sums (rows g, g + 8, ... in ascending order per lane
+// group, then an 8-lane tree): the one-block finalize then reads a few dozen
+// rows instead of one per tile of the pass.
+__global__ void __launch_bounds__(kThreads)
+k_part_chunks(double* const* __restrict__ parts, const int* __restrict__ ntiles,
+              double* const* __restrict__ cparts) {
+  __shared__ double sh[kChunkRows * (kNumPart + 1)];
+  const int r = blockIdx.y, b = blockIdx.x;
+  const int nt = ntiles[r];
+  const int row0 = b * kChunkRows;
+  if (row0 >= nt) return;
+  const int nrow = min(kChunkRows, nt - row0);
+  const double* src = parts[r] + (long long)row0 * kNumPart;
+  for (int i = threadIdx.x; i < kChunkRows * kNumPart; i += kThreads) {
+    const int row = i / kNumPart, k = i % kNumPart;
+    sh[row * (kNumPart + 1) + k] = row < nrow ? src[i] : 0.0;
+  }
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    const int k = threadIdx.x / 8, g = threadIdx.x % 8;
+    double v = 0.0;
+#pragma unroll
+    for (int j = 0; j < kChunkRows / 8; ++j) v += sh[(g + 8 * j) * (kNumPart + 1) + k];
+    v += __shfl_down(v, 4, 8);
+    v += __shfl_down(v, 2, 8);
+    v += __shfl_down(v, 1, 8);
+    if (g == 0) cparts[r][(long long)b * kNumPart + k] = v;
+  }
+}
+
 static void bicg_fin(StateSet& f, int n, int mode, int init_iters, hipStream_t s) {
+  hipLaunchKernelGGL(k_part_chunks, dim3(f.maxchunks, n), dim3(kThreads), 0, s,
+                     f.parts.as<double*>(), f.ntiles.as<int>(), f.cparts.as<double*>());
   hipLaunchKernelGGL(k_bicg_fin, dim3(n), dim3(kThreads), 0, s, f.sts.as<RState*>(),
-                     f.parts.as<double*>(), f.ntiles.as<int>(), mode, init_iters);
+                     f.cparts.as<double*>(), f.cnt.as<int>(), mode, init_iters);
 }
 
 // x0, r0 = gamma (b - A x0), rho_1 = <b, r0> and the first state of every new member

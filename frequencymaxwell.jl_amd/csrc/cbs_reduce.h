@@ -43,6 +43,49 @@ __device__ __forceinline__ void block_reduce_store(double (&acc)[kNumPart], doub
   }
 }
 
+// The x tiles' reduction: the first nred of kNumPart doubles per thread summed
+// over a block of NT threads through LDS instead of wave shuffles (each value
+// of a 64-lane shuffle reduction costs twelve LDS-pipe permutes).  Every
+// thread writes its values as a column of `scratch` (kXRedStride(NT) doubles
+// per part), then lane g of group k of the first wave sums rows g, g + 8, ...
+// of part k in ascending order and an 8-lane shuffle combines the groups;
+// thread 8 k writes part k to dst[k] (parts from nred on are written as 0.0).
+// Fixed order: the sums depend only on the block's data.
+template <int NT>
+__host__ __device__ constexpr int kXRedStride() {
+  return NT + 8;  // 8 doubles of pad: the groups of different parts start on different banks
+}
+template <int NT>
+__device__ __forceinline__ void block_reduce_store_lds(const double (&acc)[kNumPart], double* dst,
+                                                       double* scratch, int tid,
+                                                       int nred = kNumPart) {
+  static_assert(NT % 64 == 0 && kNumPart * 8 <= 64, "one wave sums the groups");
+  constexpr int LS = kXRedStride<NT>();
+  constexpr int G = 8;  // groups per part
+  __syncthreads();  // the caller's LDS tile may still be read by other waves
+#pragma unroll
+  for (int k = 0; k < kNumPart; ++k)
+    if (k < nred) scratch[k * LS + tid] = acc[k];
+  __syncthreads();
+  if (tid < 64) {
+    const int k = tid / G, g = tid % G;
+    double v = 0.0;
+    if (k < nred) {
+      const double* col = scratch + k * LS + g;
+#pragma unroll 8
+      for (int i = 0; i < NT / G; ++i) v += col[i * G];
+    }
+    v += __shfl_down(v, 4, G);
+    v += __shfl_down(v, 2, G);
+    v += __shfl_down(v, 1, G);
+    if (g == 0) dst[k] = k < nred ? v : 0.0;
+  }
+}
+
+// Partial rows summed per block by k_part_chunks (the first level of a
+// finalize whose one block would otherwise read every row of a pass alone)
+constexpr int kChunkRows = 128;
+
 // Sum column k of kNumPart-strided partials [0, ntile); every thread gets the total.
 __device__ __forceinline__ double block_sum_parts(const double* __restrict__ part, int ntile,
                                                   int k, double* red) {
