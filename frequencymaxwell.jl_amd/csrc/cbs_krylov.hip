@@ -456,7 +456,7 @@ struct GmVec {
   long long nv;
   double eps_inv;
   double* part;   // kNumPart per vector-kernel block
-  double* gpart;  // dot partials [(m+1)][nwp][2], nwp = 4 per block (one per wave)
+  double* gpart;  // dot partials [(m+1)][nchunk][2], one per block of k_gm_dots
   double* sm;     // small dense data, layout GmLayout
   RState* st;
   int nchunk;
@@ -521,84 +521,95 @@ __global__ void __launch_bounds__(kThreads) k_gm_scale(const GmVec<T>* __restric
   }
 }
 
-// per-wave partials of <V_j, w> (w = V_{k+1}) for j = 0..k
+// per-block partials of <V_j, w> (w = V_{k+1}) for j = 0..k: the coefficients
+// run in groups of kGmJB, so each group reads its chunk of w once and keeps
+// kGmJB load streams in flight; each group's sums are reduced over the block
+// through LDS (gpart[(j nchunk + block) 2 + re/im])
+#ifndef CBS_GM_JB
+#define CBS_GM_JB 2
+#endif
+constexpr int kGmJB = CBS_GM_JB;
 template <class T>
 __global__ void __launch_bounds__(kThreads) k_gm_dots(const GmVec<T>* __restrict__ vs, int k) {
+  __shared__ double scratch[2 * kGmJB * (kThreads + 8)];
   const GmVec<T>& q = vs[blockIdx.y];
   if (halted(q.st)) return;
   const long long start = (long long)blockIdx.x * kChunk;
   if (start >= q.nv) return;
   const long long end = min(start + kChunk, q.nv), nv = q.nv;
   const Cx<T>* w = q.K + (long long)(k + 1) * 3 * nv;
-  const int nwp = q.nchunk * (kThreads / 64);
-  const int pidx = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
-  for (int j = 0; j <= k; ++j) {
-    const Cx<T>* vj = q.K + (long long)j * 3 * nv;
-    double ar = 0.0, ai = 0.0;
-    for (long long i = start + threadIdx.x; i < end; i += kThreads) {
+  for (int j0 = 0; j0 <= k; j0 += kGmJB) {
+    double acc[2 * kGmJB];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const Cx<T> d = cdot(vj[c * nv + i], w[c * nv + i]);
-        ar += (double)d.re;
-        ai += (double)d.im;
+    for (int jj = 0; jj < 2 * kGmJB; ++jj) acc[jj] = 0.0;
+    for (long long i = start + threadIdx.x; i < end; i += kThreads) {
+      Cx<T> wv[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) wv[c] = w[c * nv + i];
+#pragma unroll
+      for (int jj = 0; jj < kGmJB; ++jj) {
+        if (j0 + jj <= k) {
+          const Cx<T>* vj = q.K + (long long)(j0 + jj) * 3 * nv;
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            const Cx<T> d = cdot(vj[c * nv + i], wv[c]);
+            acc[2 * jj] += (double)d.re;
+            acc[2 * jj + 1] += (double)d.im;
+          }
+        }
       }
     }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      ar += __shfl_down(ar, off, 64);
-      ai += __shfl_down(ai, off, 64);
-    }
-    if ((threadIdx.x & 63) == 0) {
-      q.gpart[((long long)j * nwp + pidx) * 2 + 0] = ar;
-      q.gpart[((long long)j * nwp + pidx) * 2 + 1] = ai;
-    }
+    int which;
+    const double t = block_sum_lds<kThreads, 2 * kGmJB>(acc, scratch, threadIdx.x, which);
+    if (which >= 0 && j0 + which / 2 <= k)
+      q.gpart[((long long)(j0 + which / 2) * q.nchunk + blockIdx.x) * 2 + (which & 1)] = t;
   }
 }
 
-// coef_j = sum of the partials; H(j, k) = coef (pass 0) or += coef (pass 1)
+// coef_j = sum of the partials; H(j, k) = coef (pass 0) or += coef (pass 1).
+// Block (j, i) sums coefficient j of right-hand side i, so the k + 1 sums of an
+// Arnoldi step run on k + 1 CUs at once (fixed order per coefficient).
 __global__ void __launch_bounds__(kThreads)
 k_gm_fin_dots(RState* const* __restrict__ sts, double* const* __restrict__ gparts,
               double* const* __restrict__ sms, const int* __restrict__ nchunks, int m, int k,
               int pass) {
   __shared__ double red[2][kThreads / 64];
-  const int rid = blockIdx.x;
+  const int rid = blockIdx.y, j = blockIdx.x;
+  if (j > k) return;
   RState* st = sts[rid];
   if (halted(st)) return;
   const double* gp = gparts[rid];
   double* sm = sms[rid];
-  const int nwp = nchunks[rid] * (kThreads / 64);
+  const int nwp = nchunks[rid];  // one partial per block of k_gm_dots
   const GmLayout L{m};
-  for (int j = 0; j <= k; ++j) {
-    double ar = 0.0, ai = 0.0;
-    for (int i = threadIdx.x; i < nwp; i += kThreads) {
-      ar += gp[((long long)j * nwp + i) * 2 + 0];
-      ai += gp[((long long)j * nwp + i) * 2 + 1];
-    }
+  double ar = 0.0, ai = 0.0;
+#pragma unroll 4
+  for (int i = threadIdx.x; i < nwp; i += kThreads) {
+    ar += gp[((long long)j * nwp + i) * 2 + 0];
+    ai += gp[((long long)j * nwp + i) * 2 + 1];
+  }
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      ar += __shfl_down(ar, off, 64);
-      ai += __shfl_down(ai, off, 64);
+  for (int off = 32; off > 0; off >>= 1) {
+    ar += __shfl_down(ar, off, 64);
+    ai += __shfl_down(ai, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = ar;
+    red[1][threadIdx.x >> 6] = ai;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double sr = 0.0, si = 0.0;
+    for (int w = 0; w < kThreads / 64; ++w) {
+      sr += red[0][w];
+      si += red[1][w];
     }
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) {
-      red[0][threadIdx.x >> 6] = ar;
-      red[1][threadIdx.x >> 6] = ai;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double sr = 0.0, si = 0.0;
-      for (int w = 0; w < kThreads / 64; ++w) {
-        sr += red[0][w];
-        si += red[1][w];
-      }
-      const Cx<double> cj = mk<double>(sr, si);
-      st2(sm, L.c(j), cj);
-      st2(sm, L.H(j, k), pass ? ld(sm, L.H(j, k)) + cj : cj);
-    }
+    const Cx<double> cj = mk<double>(sr, si);
+    st2(sm, L.c(j), cj);
+    st2(sm, L.H(j, k), pass ? ld(sm, L.H(j, k)) + cj : cj);
   }
 }
 
-// w -= sum_j coef_j V_j ; partial ||w||^2
 template <class T>
 __global__ void __launch_bounds__(kThreads) k_gm_update(const GmVec<T>* __restrict__ vs, int m,
                                                         int k) {
@@ -880,7 +891,7 @@ void gmres_solve(Ctx* ctx, std::vector<SolveItem<T>>& it, const IterOpts& io, hi
       mk_[k].launch(s);
       for (int pass = 0; pass < 2; ++pass) {
         hipLaunchKernelGGL((k_gm_dots<T>), vgrid, dim3(kThreads), 0, s, d_vs, k);
-        hipLaunchKernelGGL(k_gm_fin_dots, dim3(n), dim3(kThreads), 0, s, d_sts,
+        hipLaunchKernelGGL(k_gm_fin_dots, dim3(k + 1, n), dim3(kThreads), 0, s, d_sts,
                            dg.as<double*>(), d_sms, dnc.as<int>(), m, k, pass);
         hipLaunchKernelGGL((k_gm_update<T>), vgrid, dim3(kThreads), 0, s, d_vs, m, k);
       }

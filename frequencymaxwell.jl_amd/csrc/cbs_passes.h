@@ -1131,7 +1131,8 @@ __device__ __forceinline__ void xinv_tile(const DevProb<T>& pr, int tile, int ti
     double* scratch = reinterpret_cast<double*>(g_smem);
     // POST_BORN and POST_RESID accumulate only part 0
     const int nred = (!FUSE && pr.post == POST_PMATVEC) ? kNumPart : 1;
-    if constexpr (NT <= kThreads)  // the launch reserves kXRedStride(kThreads) parts of LDS
+    // the launch reserves kNumPart * kXRedStride<kThreads>() doubles of LDS for it
+    if constexpr (NT <= kThreads)
       block_reduce_store_lds<NT>(acc, pr.part + (long long)tile * kNumPart, scratch, tid, nred);
     else
       block_reduce_store<NT>(acc, pr.part + (long long)tile * kNumPart, scratch, tid, nred);

@@ -82,6 +82,36 @@ __device__ __forceinline__ void block_reduce_store_lds(const double (&acc)[kNumP
   }
 }
 
+// NV doubles per thread summed over a block of NT threads through LDS (NV
+// divides 64): every thread writes its values as LDS columns (NT + 8 doubles
+// apart), lane g of group v of the first wave sums rows g, g + G, ... of
+// column v in ascending order (G = 64 / NV lanes per column), a G-lane tree
+// ends it, and lane 0 of group v returns the total of value v (the other
+// lanes return 0).  scratch holds NV * (NT + 8) doubles.  Fixed order.
+template <int NT, int NV>
+__device__ __forceinline__ double block_sum_lds(const double (&v)[NV], double* scratch, int tid,
+                                                int& which) {
+  static_assert(64 % NV == 0 && NT % (64 / NV) == 0, "whole lane groups");
+  constexpr int LS = NT + 8;
+  constexpr int G = 64 / NV;
+  __syncthreads();  // scratch may still be read by an earlier sum
+#pragma unroll
+  for (int k = 0; k < NV; ++k) scratch[k * LS + tid] = v[k];
+  __syncthreads();
+  double t = 0.0;
+  which = -1;
+  if (tid < 64) {
+    const int k = tid / G, g = tid % G;
+    const double* col = scratch + k * LS + g;
+#pragma unroll 8
+    for (int i = 0; i < NT / G; ++i) t += col[i * G];
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) t += __shfl_down(t, off, G);
+    if (g == 0) which = k;
+  }
+  return which >= 0 ? t : 0.0;
+}
+
 // Partial rows summed per block by k_part_chunks (the first level of a
 // finalize whose one block would otherwise read every row of a pass alone)
 constexpr int kChunkRows = 128;
