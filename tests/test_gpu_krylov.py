@@ -137,23 +137,27 @@ def test_multi_rhs_equals_single(fm, oracle):
             assert np.array_equal(Es[k], E1), method
 
 
+@pytest.mark.parametrize("method", ["bicgstab", "gmres"])
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
-def test_bicgstab_batch_of_grids_many_partial_chunks(fm, oracle, precision):
-    """Problems of different grids in one BiCGStab set, each with hundreds of
-    x-inverse tiles: the finalize first sums the tiles' partials in 128-row
-    chunks on many CUs, a different number of chunks per problem.  Every
-    result equals its single solve bit for bit, and in FP64 the largest
-    matches the oracle's iterates at a fixed budget."""
-    print(f"case bicgstab batch of grids {precision}", flush=True)
+def test_krylov_batch_of_grids_many_partial_chunks(fm, oracle, method, precision):
+    """Problems of different grids in one Krylov set, each with hundreds of
+    x-inverse tiles and vector-kernel blocks: the BiCGStab finalize first sums
+    the tiles' partials in 128-row chunks on many CUs, the GMRES dot finalize
+    sums each coefficient over the blocks in a block of its own, with a
+    different number of chunks and blocks per problem.  Every result equals
+    its single solve bit for bit, and in FP64 the largest matches the
+    oracle's iterates at a fixed budget."""
+    print(f"case {method} batch of grids {precision}", flush=True)
     grids = [(64, 48, 40), (32, 40, 24)]
-    budget = 5
+    budget = 5 if method == "bicgstab" else 9
     probs, Ss, refs = [], [], []
     for g, seed in zip(grids, (41, 42)):
         init, S, rhs, prob = _setup(fm, oracle, g, oracle.PeriodicBC(), seed, precision=precision)
         probs.append(prob)
         Ss.append(S)
         refs.append((init, rhs))
-    o = fm.make_opts(method="bicgstab", precision=precision, tol=0.0, maxiters=budget)
+    o = fm.make_opts(method=method, precision=precision, tol=0.0, maxiters=budget,
+                     gmres_restart=4 if method == "gmres" else 0)
     Es, _, sts = fm.solve_problems(probs, Ss, o, want_h=False)
     for k, (p, S) in enumerate(zip(probs, Ss)):
         E1, _, st1 = p.solve(S, o, want_h=False)
@@ -161,10 +165,14 @@ def test_bicgstab_batch_of_grids_many_partial_chunks(fm, oracle, precision):
         assert np.array_equal(Es[k], E1), grids[k]
     if precision == "fp64":
         init, rhs = refs[0]
-        x_o, st_o = oracle.solve_bicgstab(rhs, rhs, init.potential, init.eps_imag, init.green,
-                                          0.0, budget)
+        if method == "bicgstab":
+            x_o, st_o = oracle.solve_bicgstab(rhs, rhs, init.potential, init.eps_imag,
+                                              init.green, 0.0, budget)
+        else:
+            x_o, st_o = oracle.solve_gmres(rhs, rhs, init.potential, init.eps_imag, init.green,
+                                           0.0, budget, restart=4)
         assert rel(Es[0], x_o) <= 1e-9
-        assert sts[0].resid == pytest.approx(st_o.resid, rel=1e-8)
+        assert sts[0].resid == pytest.approx(st_o.resid, rel=1e-8 if method == "bicgstab" else 1e-7)
 
 
 def test_gmres_basis_limit(fm, oracle):
