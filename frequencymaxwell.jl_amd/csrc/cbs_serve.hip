@@ -29,6 +29,25 @@ namespace cbs {
 
 using Clock = std::chrono::steady_clock;
 
+// host copy split over a few threads (pageable <-> pinned staging)
+static void par_copy(void* dst, const void* src, size_t bytes) {
+  const size_t kPiece = (size_t)16 << 20;
+  const int nt = (int)std::min<size_t>(8, (bytes + kPiece - 1) / kPiece);
+  if (nt <= 1) {
+    std::memcpy(dst, src, bytes);
+    return;
+  }
+  std::vector<std::thread> th;
+  const size_t per = (bytes + nt - 1) / nt;
+  for (int t = 0; t < nt; ++t) {
+    const size_t o = (size_t)t * per;
+    if (o >= bytes) break;
+    const size_t n = std::min(per, bytes - o);
+    th.emplace_back([=] { std::memcpy((char*)dst + o, (const char*)src + o, n); });
+  }
+  for (auto& x : th) x.join();
+}
+
 enum JobState { J_QUEUED = 0, J_ACTIVE = 1, J_DONE = 2 };
 enum Phase {
   PH_WAIT_LOAD = 0,  // device buffers allocated, fields not yet sent
@@ -39,7 +58,20 @@ enum Phase {
   PH_RETIRE = 5,     // solved: H and download to start
   PH_WAIT_DOWN = 6,  // H launched, download not started
   PH_UNLOADING = 7,  // fields on the transfer stream
-  PH_DOWN_DONE = 8   // fields in the pinned buffer
+  PH_DOWN_DONE = 8,  // fields in the pinned buffer
+  PH_FREE = 9        // fields with the client: device buffers go after the chunk
+};
+
+// host result array, left uninitialised until the copy from the pinned
+// buffer writes it (the copy threads first-touch its pages in parallel)
+struct HostArr {
+  std::unique_ptr<double[]> p;
+  size_t n = 0;
+  void take(const char* src, size_t bytes) {
+    n = bytes / sizeof(double);
+    p.reset(new double[n]);
+    par_copy(p.get(), src, bytes);
+  }
 };
 
 struct Job {
@@ -55,7 +87,7 @@ struct Job {
   cbs_status status = CBS_OK;
   std::string err;
   cbs_solve_stats stats{};
-  std::vector<double> E, H;
+  HostArr E, H;
   // server thread only
   std::unique_ptr<Problem> p;
   DevBuf Sd, X, Hd;  // FP64 source term, iterate / result, curl result
@@ -98,25 +130,6 @@ struct Pinned {
     if (p) (void)hipHostFree(p);
   }
 };
-
-// host copy split over a few threads (pageable <-> pinned staging)
-static void par_copy(void* dst, const void* src, size_t bytes) {
-  const size_t kPiece = (size_t)64 << 20;
-  const int nt = (int)std::min<size_t>(8, (bytes + kPiece - 1) / kPiece);
-  if (nt <= 1) {
-    std::memcpy(dst, src, bytes);
-    return;
-  }
-  std::vector<std::thread> th;
-  const size_t per = (bytes + nt - 1) / nt;
-  for (int t = 0; t < nt; ++t) {
-    const size_t o = (size_t)t * per;
-    if (o >= bytes) break;
-    const size_t n = std::min(per, bytes - o);
-    th.emplace_back([=] { std::memcpy((char*)dst + o, (const char*)src + o, n); });
-  }
-  for (auto& x : th) x.join();
-}
 
 struct Server {
   Ctx* ctx = nullptr;
@@ -226,7 +239,7 @@ void Server::end_job(const std::shared_ptr<Job>& j, cbs_status st, const std::st
   release(*j);
   std::lock_guard<std::mutex> lk(mu);
   reserved -= j->footprint;
-  finish_locked(*j, st, err);
+  if (j->state != J_DONE) finish_locked(*j, st, err);  // PH_FREE jobs keep their result
   for (size_t i = 0; i < active.size(); ++i)
     if (active[i] == j) {
       active.erase(active.begin() + i);
@@ -634,20 +647,17 @@ void Server::tick() {
   launch_sets();
   // host side of the transfers while the chunk runs
   if (landed) {
+    // the client gets the fields now; the device buffers go after the chunk
+    // launched above (hipFree waits for the whole device)
     const size_t fb = landed->out_bytes();
-    std::vector<double> E(fb / sizeof(double)), H;
-    par_copy(E.data(), down.p, fb);
-    if (landed->o.compute_h) {
-      H.resize(fb / sizeof(double));
-      par_copy(H.data(), down.p + fb, fb);
-    }
-    {
-      std::lock_guard<std::mutex> lk(mu);
-      landed->E.swap(E);
-      landed->H.swap(H);
-    }
-    auto sp = find_active(landed);
-    if (sp) end_job(sp, CBS_OK, "");
+    HostArr E, H;
+    E.take(down.p, fb);
+    if (landed->o.compute_h) H.take(down.p + fb, fb);
+    std::lock_guard<std::mutex> lk(mu);
+    landed->E = std::move(E);
+    landed->H = std::move(H);
+    finish_locked(*landed, CBS_OK, "");
+    landed->phase = PH_FREE;
   }
   if (!down_job)
     for (auto& j : active)
@@ -662,6 +672,10 @@ void Server::tick() {
         break;
       }
   wait_sets();
+  std::vector<std::shared_ptr<Job>> freed;
+  for (auto& j : active)
+    if (j->phase == PH_FREE) freed.push_back(j);
+  for (auto& j : freed) end_job(j, CBS_OK, "");
 }
 
 void Server::loop() {
@@ -680,8 +694,13 @@ void Server::loop() {
           ++it;
         }
       }
-      // arrival order; the head waits for memory rather than being overtaken
-      while (!queue.empty() && (int)active.size() < max_active) {
+      // arrival order; the head waits for memory rather than being overtaken.
+      // max_active bounds the requests being loaded or solved: a solved
+      // request's curl, download and host copy overlap the next admission
+      int solving = 0;
+      for (auto& j : active)
+        if (j->phase < PH_WAIT_DOWN) ++solving;
+      while (!queue.empty() && solving < max_active) {
         auto& j = queue.front();
         if (!active.empty() && reserved + j->footprint > capacity) break;
         reserved += j->footprint;
@@ -689,6 +708,7 @@ void Server::loop() {
         active.push_back(j);
         adm.push_back(j);
         queue.pop_front();
+        ++solving;
       }
       for (auto& j : active)
         if (j->cancel) cancel.push_back(j);
@@ -895,8 +915,11 @@ void server_wait(Ctx& c, uint64_t ticket, double* E, double* H, cbs_solve_stats*
   sv.jobs.erase(ticket);
   if (stats) *stats = j->stats;
   if (j->status != CBS_OK) throw Error(j->status, j->err);
-  if (E) std::memcpy(E, j->E.data(), j->E.size() * sizeof(double));
-  if (H && !j->H.empty()) std::memcpy(H, j->H.data(), j->H.size() * sizeof(double));
+  // the copy into the caller's arrays runs without the server's lock
+  HostArr Ej = std::move(j->E), Hj = std::move(j->H);
+  lk.unlock();
+  if (E) par_copy(E, Ej.p.get(), Ej.n * sizeof(double));
+  if (H && Hj.n) par_copy(H, Hj.p.get(), Hj.n * sizeof(double));
 }
 
 void server_cancel(Ctx& c, uint64_t ticket) {

@@ -248,7 +248,9 @@ class CBSServer:
 
     def __init__(self, max_active: int = 8, ctx: L.Context | None = None, policy: str = "lead",
                  kappa: float = 0.2):
-        """``policy``: "lead" (one request per chunk, first come first served with
+        """``max_active``: requests being loaded or solved at once (a solved
+        request's download does not hold a slot).
+        ``policy``: "lead" (one request per chunk, first come first served with
         requests of estimated work <= kappa x the mean passing) or "share" (equal
         shares of every chunk in shared launches); see cbs_server_policy."""
         self.ctx = ctx or L.default_context()

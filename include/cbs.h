@@ -354,7 +354,8 @@ typedef struct {
 /* Bytes of HBM a request of this description holds while resident. */
 cbs_status cbs_request_footprint(const cbs_problem_desc *desc, const cbs_solve_opts *opts,
                                  uint64_t *bytes);
-/* Start the server thread; at most max_active requests resident at once.
+/* Start the server thread; at most max_active requests being loaded or
+ * solved at once (a solved request's download does not hold a slot).
  * Admission also respects the context's HBM budget (cbs_ctx_create). */
 cbs_status cbs_server_start(cbs_ctx *ctx, int32_t max_active);
 /* Stop the thread; pending and running requests end with CBS_ECANCELED. */
