@@ -444,36 +444,64 @@ void bicgstab_solve(Ctx* ctx, std::vector<SolveItem<T>>& it, const IterOpts& io,
 }
 
 // ---------------------------------------------------------------------------
-// GMRES (restarted when io.restart > 0), classical Gram-Schmidt with one
-// re-orthogonalisation (CGS2), Givens rotations on the device
+// GMRES (restarted when io.restart > 0), Givens rotations on the device.
+//
+// Orthogonalisation: classical Gram-Schmidt with one re-orthogonalisation,
+// the second pass lagged by one Arnoldi step (delayed CGS2; Bielich, Langou,
+// Thomas, Swirydowicz, Yamazaki, Boman, "Low-synch Gram-Schmidt with delayed
+// reorthogonalization for Krylov solvers", Parallel Computing 2022), so that
+// every step reads the basis twice instead of four times.  Step k holds the
+// final vectors q_0 .. q_(k-1) and u_k, the step-(k-1) vector projected once
+// and not normalised (u_0 = the residual):
+//   z = A u_k                                        (the operator sweeps)
+//   s = Q^H u_k, t = Q^H z, alpha = |u_k|^2, gamma = u_k^H z   (one pass)
+//   beta = sqrt(alpha - |s|^2) = |u_k - Q s|, q_k = (u_k - Q s) / beta;
+//   column k-1 of the Hessenberg matrix is final: += s, row k = beta
+//   (rotations, residual estimate, stopping decision);
+//   A q_k = (z - A Q s) / beta = (z - [Q q_k] Hbar s) / beta, so its
+//   coefficients are h = ([t; q_k^H z] - Hbar s) / beta with
+//   q_k^H z = (gamma - s^H t) / beta, and the once-projected next vector is
+//   u_(k+1) = A q_k - [Q q_k] h = (z - Q t - q_k q_k^H z) / beta  (one pass,
+//   which also writes q_k).
+// The last column of a cycle is closed by one more reduce of its pending
+// vector (no operator application).  In exact arithmetic every quantity is
+// the one of CGS2 (and of Krylov.jl's modified Gram-Schmidt); a cycle that
+// meets the tolerance learns it one step late, with one operator application
+// that the iteration count leaves out.
 // ---------------------------------------------------------------------------
 template <class T>
 struct GmVec {
   Cx<T>* x;
-  Cx<T>* K;  // basis V_0 .. V_m, each 3 nv
+  Cx<T>* K;  // slots 0 .. m: q_0 .. q_(k-1), u_k, z
   const Cx<T>* V;
   const Cx<T>* b;
   long long nv;
   double eps_inv;
   double* part;   // kNumPart per vector-kernel block
-  double* gpart;  // dot partials [(m+1)][nchunk][2], one per block of k_gm_dots
+  double* gpart;  // reduce partials [(m+1)][nchunk][4], one row per block of k_gm_lred
   double* sm;     // small dense data, layout GmLayout
   RState* st;
   int nchunk;
 };
 
+// H: the rotated Hessenberg matrix (R after the rotations), Hr: the same
+// columns before the rotations (the steps read Hbar s from them); s, t:
+// the reduce coefficients of the current step (t(k) = q_k^H z)
 struct GmLayout {
   int m;
   __host__ __device__ int H(int i, int j) const { return 8 + 2 * (i + j * (m + 1)); }
-  __host__ __device__ int cs(int i) const { return 8 + 2 * (m + 1) * m + i; }
-  __host__ __device__ int sn(int i) const { return 8 + 2 * (m + 1) * m + m + 2 * i; }
-  __host__ __device__ int g(int i) const { return 8 + 2 * (m + 1) * m + 3 * m + 2 * i; }
-  __host__ __device__ int y(int i) const { return 8 + 2 * (m + 1) * m + 3 * m + 2 * (m + 1) + 2 * i; }
-  __host__ __device__ int c(int i) const { return 8 + 2 * (m + 1) * m + 5 * m + 2 * (m + 1) + 2 * i; }
-  __host__ __device__ int size() const { return c(m + 1); }
+  __host__ __device__ int Hr(int i, int j) const { return 8 + 2 * (m + 1) * m + 2 * (i + j * (m + 1)); }
+  __host__ __device__ int cs(int i) const { return 8 + 4 * (m + 1) * m + i; }
+  __host__ __device__ int sn(int i) const { return 8 + 4 * (m + 1) * m + m + 2 * i; }
+  __host__ __device__ int g(int i) const { return 8 + 4 * (m + 1) * m + 3 * m + 2 * i; }
+  __host__ __device__ int y(int i) const { return 8 + 4 * (m + 1) * m + 3 * m + 2 * (m + 1) + 2 * i; }
+  __host__ __device__ int s(int i) const { return 8 + 4 * (m + 1) * m + 5 * m + 2 * (m + 1) + 2 * i; }
+  __host__ __device__ int t(int i) const { return 8 + 4 * (m + 1) * m + 5 * m + 4 * (m + 1) + 2 * i; }
+  __host__ __device__ int size() const { return t(m + 1); }
 };
-// header slots of sm
-enum { GM_KDONE = 0, GM_SCALE = 1 };
+// header slots of sm: final columns of the cycle, 1/beta of the step, steps
+// of the cycle, alpha and gamma of the reduce
+enum { GM_KDONE = 0, GM_SCALE = 1, GM_KSTEP = 2, GM_ALPHA = 3, GM_GAMMA = 4 };
 
 __device__ __forceinline__ Cx<double> ld(const double* sm, int o) { return mk<double>(sm[o], sm[o + 1]); }
 __device__ __forceinline__ void st2(double* sm, int o, Cx<double> v) {
@@ -481,8 +509,11 @@ __device__ __forceinline__ void st2(double* sm, int o, Cx<double> v) {
   sm[o + 1] = v.im;
 }
 __device__ __forceinline__ bool halted(const RState* st) { return st->done | st->cyc; }
+// cyc: 0 stepping, 1 the cycle has ended (its columns are final), 2 no more
+// steps, the last column still to be closed by the final reduce
+__device__ __forceinline__ bool closed(const RState* st) { return st->done || st->cyc == 1; }
 
-// r = gamma b - (use_v ? V_0 : 0) into V_0; partial ||r||^2
+// r = gamma b - (use_v ? V_0 : 0) into slot 0; partial ||r||^2
 template <class T>
 __global__ void __launch_bounds__(kThreads) k_gm_resid(const GmVec<T>* __restrict__ vs, int use_v) {
   __shared__ double scratch[(kThreads / 64) * kNumPart];
@@ -505,143 +536,296 @@ __global__ void __launch_bounds__(kThreads) k_gm_resid(const GmVec<T>* __restric
   block_reduce_store(acc, q.part + (long long)blockIdx.x * kNumPart, scratch, threadIdx.x);
 }
 
-// V_j *= sm[GM_SCALE]
-template <class T>
-__global__ void __launch_bounds__(kThreads) k_gm_scale(const GmVec<T>* __restrict__ vs, int j) {
-  const GmVec<T>& q = vs[blockIdx.y];
-  if (halted(q.st)) return;
-  const long long start = (long long)blockIdx.x * kChunk;
-  if (start >= q.nv) return;
-  const long long end = min(start + kChunk, q.nv), nv = q.nv;
-  const T sc = (T)q.sm[GM_SCALE];
-  Cx<T>* v = q.K + (long long)j * 3 * nv;
-  for (long long i = start + threadIdx.x; i < end; i += kThreads) {
-#pragma unroll
-    for (int c = 0; c < 3; ++c) v[c * nv + i] = sc * v[c * nv + i];
-  }
-}
-
-// per-block partials of <V_j, w> (w = V_{k+1}) for j = 0..k: the coefficients
-// run in groups of kGmJB, so each group reads its chunk of w once and keeps
-// kGmJB load streams in flight; each group's sums are reduced over the block
-// through LDS (gpart[(j nchunk + block) 2 + re/im])
+// Reduce of step k (k < 0: the cycle's final reduce, K = steps taken, no z):
+// per-block partials of s_j = <q_j, u_K>, t_j = <q_j, z> (j < K) and of
+// alpha = |u_K|^2, gamma = <u_K, z> (row K), u_K in slot K and z in slot
+// K + 1.  The rows run in groups of kGmJB, each group reading its chunk of
+// u and z once and keeping kGmJB basis streams in flight; each group's sums
+// are reduced over the block through LDS.
 #ifndef CBS_GM_JB
-#define CBS_GM_JB 2
+#define CBS_GM_JB 4
 #endif
 constexpr int kGmJB = CBS_GM_JB;
 template <class T>
-__global__ void __launch_bounds__(kThreads) k_gm_dots(const GmVec<T>* __restrict__ vs, int k) {
-  __shared__ double scratch[2 * kGmJB * (kThreads + 8)];
+__global__ void __launch_bounds__(kThreads) k_gm_lred(const GmVec<T>* __restrict__ vs, int k) {
+  __shared__ double scratch[4 * kGmJB * (kThreads + 8)];
   const GmVec<T>& q = vs[blockIdx.y];
-  if (halted(q.st)) return;
+  const bool last = k < 0;
+  if (last ? closed(q.st) : halted(q.st)) return;
+  const int K = last ? (int)q.sm[GM_KSTEP] : k;
   const long long start = (long long)blockIdx.x * kChunk;
   if (start >= q.nv) return;
   const long long end = min(start + kChunk, q.nv), nv = q.nv;
-  const Cx<T>* w = q.K + (long long)(k + 1) * 3 * nv;
-  for (int j0 = 0; j0 <= k; j0 += kGmJB) {
-    double acc[2 * kGmJB];
+  const Cx<T>* u = q.K + (long long)K * 3 * nv;
+  const Cx<T>* z = u + 3 * nv;
+  for (int j0 = 0; j0 <= K; j0 += kGmJB) {
+    double acc[4 * kGmJB];
 #pragma unroll
-    for (int jj = 0; jj < 2 * kGmJB; ++jj) acc[jj] = 0.0;
+    for (int e = 0; e < 4 * kGmJB; ++e) acc[e] = 0.0;
     for (long long i = start + threadIdx.x; i < end; i += kThreads) {
-      Cx<T> wv[3];
+      Cx<T> uv[3], zv[3];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) wv[c] = w[c * nv + i];
+      for (int c = 0; c < 3; ++c) {
+        uv[c] = u[c * nv + i];
+        zv[c] = last ? mk<T>(T(0), T(0)) : z[c * nv + i];
+      }
 #pragma unroll
       for (int jj = 0; jj < kGmJB; ++jj) {
-        if (j0 + jj <= k) {
-          const Cx<T>* vj = q.K + (long long)(j0 + jj) * 3 * nv;
+        const int j = j0 + jj;
+        if (j < K) {
+          const Cx<T>* qj = q.K + (long long)j * 3 * nv;
 #pragma unroll
           for (int c = 0; c < 3; ++c) {
-            const Cx<T> d = cdot(vj[c * nv + i], wv[c]);
-            acc[2 * jj] += (double)d.re;
-            acc[2 * jj + 1] += (double)d.im;
+            const Cx<T> qv = qj[c * nv + i];
+            const Cx<T> d0 = cdot(qv, uv[c]), d1 = cdot(qv, zv[c]);
+            acc[4 * jj] += (double)d0.re;
+            acc[4 * jj + 1] += (double)d0.im;
+            acc[4 * jj + 2] += (double)d1.re;
+            acc[4 * jj + 3] += (double)d1.im;
+          }
+        } else if (j == K) {
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            const Cx<T> d1 = cdot(uv[c], zv[c]);
+            acc[4 * jj] += (double)abs2(uv[c]);
+            acc[4 * jj + 2] += (double)d1.re;
+            acc[4 * jj + 3] += (double)d1.im;
           }
         }
       }
     }
     int which;
-    const double t = block_sum_lds<kThreads, 2 * kGmJB>(acc, scratch, threadIdx.x, which);
-    if (which >= 0 && j0 + which / 2 <= k)
-      q.gpart[((long long)(j0 + which / 2) * q.nchunk + blockIdx.x) * 2 + (which & 1)] = t;
+    const double t = block_sum_lds<kThreads, 4 * kGmJB>(acc, scratch, threadIdx.x, which);
+    if (which >= 0 && j0 + which / 4 <= K)
+      q.gpart[((long long)(j0 + which / 4) * q.nchunk + blockIdx.x) * 4 + (which & 3)] = t;
   }
 }
 
-// coef_j = sum of the partials; H(j, k) = coef (pass 0) or += coef (pass 1).
-// Block (j, i) sums coefficient j of right-hand side i, so the k + 1 sums of an
-// Arnoldi step run on k + 1 CUs at once (fixed order per coefficient).
+// Row j of the reduce summed over the blocks: s_j, t_j (j < K) or alpha,
+// gamma (j = K).  Block (j, i) sums row j of right-hand side i, so the rows
+// of a step run on K + 1 CUs at once (fixed order per row).
 __global__ void __launch_bounds__(kThreads)
-k_gm_fin_dots(RState* const* __restrict__ sts, double* const* __restrict__ gparts,
-              double* const* __restrict__ sms, const int* __restrict__ nchunks, int m, int k,
-              int pass) {
-  __shared__ double red[2][kThreads / 64];
+k_gm_fin_lred(RState* const* __restrict__ sts, double* const* __restrict__ gparts,
+              double* const* __restrict__ sms, const int* __restrict__ nchunks, int m, int k) {
+  __shared__ double red[4][kThreads / 64];
   const int rid = blockIdx.y, j = blockIdx.x;
-  if (j > k) return;
   RState* st = sts[rid];
-  if (halted(st)) return;
-  const double* gp = gparts[rid];
+  const bool last = k < 0;
+  if (last ? closed(st) : halted(st)) return;
   double* sm = sms[rid];
-  const int nwp = nchunks[rid];  // one partial per block of k_gm_dots
+  const int K = last ? (int)sm[GM_KSTEP] : k;
+  if (j > K) return;
+  const double* gp = gparts[rid];
+  const int nwp = nchunks[rid];
   const GmLayout L{m};
-  double ar = 0.0, ai = 0.0;
-#pragma unroll 4
+  double a[4] = {0.0, 0.0, 0.0, 0.0};
   for (int i = threadIdx.x; i < nwp; i += kThreads) {
-    ar += gp[((long long)j * nwp + i) * 2 + 0];
-    ai += gp[((long long)j * nwp + i) * 2 + 1];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) a[e] += gp[((long long)j * nwp + i) * 4 + e];
   }
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    ar += __shfl_down(ar, off, 64);
-    ai += __shfl_down(ai, off, 64);
+  for (int e = 0; e < 4; ++e) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) a[e] += __shfl_down(a[e], off, 64);
   }
   if ((threadIdx.x & 63) == 0) {
-    red[0][threadIdx.x >> 6] = ar;
-    red[1][threadIdx.x >> 6] = ai;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) red[e][threadIdx.x >> 6] = a[e];
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    double sr = 0.0, si = 0.0;
+    double r[4] = {0.0, 0.0, 0.0, 0.0};
     for (int w = 0; w < kThreads / 64; ++w) {
-      sr += red[0][w];
-      si += red[1][w];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) r[e] += red[e][w];
     }
-    const Cx<double> cj = mk<double>(sr, si);
-    st2(sm, L.c(j), cj);
-    st2(sm, L.H(j, k), pass ? ld(sm, L.H(j, k)) + cj : cj);
+    if (j < K) {
+      st2(sm, L.s(j), mk<double>(r[0], r[1]));
+      st2(sm, L.t(j), mk<double>(r[2], r[3]));
+    } else {
+      sm[GM_ALPHA] = r[0];
+      st2(sm, GM_GAMMA, mk<double>(r[2], r[3]));
+    }
   }
 }
 
+// Close column j: the raw column gets the re-orthogonalisation coefficients
+// s_0 .. s_j and beta below them; its rotated copy gets the earlier rotations
+// and a new one, and the right-hand side g its update.  Returns the residual
+// estimate |g_(j+1)|^2 (one thread).
+__device__ double gm_close_column(double* sm, const GmLayout& L, int j, double beta) {
+  for (int i = 0; i <= j; ++i) st2(sm, L.Hr(i, j), ld(sm, L.Hr(i, j)) + ld(sm, L.s(i)));
+  st2(sm, L.Hr(j + 1, j), mk<double>(beta, 0.0));
+  for (int i = 0; i <= j + 1; ++i) st2(sm, L.H(i, j), ld(sm, L.Hr(i, j)));
+  for (int i = 0; i < j; ++i) {  // previous rotations on column j
+    const double c = sm[L.cs(i)];
+    const Cx<double> s = ld(sm, L.sn(i));
+    const Cx<double> a = ld(sm, L.H(i, j)), b = ld(sm, L.H(i + 1, j));
+    st2(sm, L.H(i, j), c * a + s * b);
+    st2(sm, L.H(i + 1, j), c * b - conj(s) * a);
+  }
+  const Cx<double> a = ld(sm, L.H(j, j)), b = ld(sm, L.H(j + 1, j));
+  const double na = sqrt(abs2(a)), den = sqrt(abs2(a) + abs2(b));
+  double c;
+  Cx<double> s;
+  if (den == 0.0) {
+    c = 1.0;
+    s = mk<double>(0.0, 0.0);
+  } else if (na == 0.0) {
+    c = 0.0;
+    s = mk<double>(1.0, 0.0);
+  } else {
+    c = na / den;
+    s = (1.0 / (na * den)) * (a * conj(b));
+  }
+  sm[L.cs(j)] = c;
+  st2(sm, L.sn(j), s);
+  st2(sm, L.H(j, j), c * a + s * b);
+  st2(sm, L.H(j + 1, j), mk<double>(0.0, 0.0));
+  const Cx<double> gj = ld(sm, L.g(j));
+  st2(sm, L.g(j + 1), (-1.0) * (conj(s) * gj));
+  st2(sm, L.g(j), c * gj);
+  sm[GM_KDONE] = (double)(j + 1);
+  return abs2(ld(sm, L.g(j + 1)));
+}
+
+// beta = |u_K - Q s| from alpha and s (one thread)
+__device__ __forceinline__ double gm_beta(const double* sm, const GmLayout& L, int K, double& b2) {
+  double ss = 0.0;
+  for (int j = 0; j < K; ++j) ss += abs2(ld(sm, L.s(j)));
+  b2 = sm[GM_ALPHA] - ss;
+  return b2 > 0.0 ? sqrt(b2) : 0.0;
+}
+
+// Step k after its reduce: beta, column k-1 closed (Krylov.jl's gmres stopping:
+// solved when the estimate meets the threshold, breakdown when beta <= btol,
+// tired after maxit Arnoldi steps in all cycles), then column k of the raw
+// Hessenberg matrix h = ([t; q_k^H z] - Hbar s) / beta (rows over the threads)
+__global__ void __launch_bounds__(kThreads)
+k_gm_fin_lstep(RState* const* __restrict__ sts, double* const* __restrict__ sms, int m, int k,
+               double btol) {
+  __shared__ int go;
+  const int rid = blockIdx.x;
+  RState* st = sts[rid];
+  if (halted(st)) return;
+  double* sm = sms[rid];
+  const GmLayout L{m};
+  if (threadIdx.x == 0) {
+    double b2;
+    const double beta = gm_beta(sm, L, k, b2);
+    st->iters += 1;  // this step's operator application
+    st->kit += 1;
+    sm[GM_KSTEP] = (double)(k + 1);
+    bool stop = false;
+    if (k == 0) {
+      st2(sm, L.g(0), mk<double>(beta, 0.0));
+      if (bad(b2) || beta == 0.0) {
+        st->cyc = 1;
+        st->pend = bad(b2) ? 3 : 1;
+        stop = true;
+      }
+    } else {
+      const double res2 = gm_close_column(sm, L, k - 1, beta);
+      st->r2 = res2;
+      push_hist(st, res2);
+      if (bad(res2) || bad(b2)) {
+        st->cyc = 1;
+        st->pend = 3;
+        stop = true;
+      } else if (res2 <= st->thr2 || beta <= btol) {
+        // solved (or broken down) with k columns: this step's application
+        // is not an iteration of the method
+        st->cyc = 1;
+        st->pend = res2 <= st->thr2 ? 1 : 4;
+        st->kit -= 1;
+        stop = true;
+      }
+    }
+    if (!stop) {
+      if (st->kit >= st->maxit) st->cyc = 2;  // last step: the final reduce closes it
+      sm[GM_SCALE] = 1.0 / beta;
+      // q_k^H z = (gamma - s^H t) / beta, kept as t_k
+      Cx<double> st_ = mk<double>(0.0, 0.0);
+      for (int j = 0; j < k; ++j) st_ = st_ + conj(ld(sm, L.s(j))) * ld(sm, L.t(j));
+      st2(sm, L.t(k), (1.0 / beta) * (ld(sm, GM_GAMMA) - st_));
+    }
+    go = stop ? 0 : 1;
+  }
+  __syncthreads();
+  if (!go) return;
+  const double ib = sm[GM_SCALE];
+  for (int i = threadIdx.x; i <= k; i += kThreads) {
+    Cx<double> hs = mk<double>(0.0, 0.0);  // (Hbar s)_i, Hbar upper Hessenberg
+    for (int j = i > 0 ? i - 1 : 0; j < k; ++j) hs = hs + ld(sm, L.Hr(i, j)) * ld(sm, L.s(j));
+    st2(sm, L.Hr(i, k), ib * (ld(sm, L.t(i)) - hs));
+  }
+}
+
+// The cycle's last column closed after the final reduce
+__global__ void k_gm_fin_lfinal(RState* const* __restrict__ sts, double* const* __restrict__ sms,
+                                int m, double btol) {
+  const int rid = blockIdx.x;
+  RState* st = sts[rid];
+  if (closed(st) || threadIdx.x) return;
+  double* sm = sms[rid];
+  const GmLayout L{m};
+  const int K = (int)sm[GM_KSTEP];
+  double b2;
+  const double beta = gm_beta(sm, L, K, b2);
+  const double res2 = gm_close_column(sm, L, K - 1, beta);
+  st->r2 = res2;
+  push_hist(st, res2);
+  st->cyc = 1;
+  if (bad(res2) || bad(b2)) st->pend = 3;
+  else if (res2 <= st->thr2) st->pend = 1;
+  else if (beta <= btol) st->pend = 4;  // breakdown short of the tolerance
+  else if (st->kit >= st->maxit) st->pend = 2;
+  else st->pend = 0;
+}
+
+// Update of step k: q_k = (u_k - Q s) / beta into slot k and
+// u_(k+1) = (z - Q t - q_k t_k) / beta into slot k + 1 (one read of Q)
 template <class T>
-__global__ void __launch_bounds__(kThreads) k_gm_update(const GmVec<T>* __restrict__ vs, int m,
-                                                        int k) {
-  __shared__ double scratch[(kThreads / 64) * kNumPart];
+__global__ void __launch_bounds__(kThreads) k_gm_lupdate(const GmVec<T>* __restrict__ vs, int m,
+                                                         int k) {
   const GmVec<T>& q = vs[blockIdx.y];
-  if (halted(q.st)) return;
+  if (closed(q.st) || (int)q.sm[GM_KSTEP] != k + 1) return;
   const long long start = (long long)blockIdx.x * kChunk;
   if (start >= q.nv) return;
   const long long end = min(start + kChunk, q.nv), nv = q.nv;
   const GmLayout L{m};
-  Cx<T>* w = q.K + (long long)(k + 1) * 3 * nv;
-  double acc[kNumPart] = {0, 0, 0, 0, 0};
+  const T ib = (T)q.sm[GM_SCALE];
+  const Cx<T> tk = mk<T>((T)q.sm[L.t(k)], (T)q.sm[L.t(k) + 1]);
+  Cx<T>* u = q.K + (long long)k * 3 * nv;
+  Cx<T>* z = u + 3 * nv;
   for (long long i = start + threadIdx.x; i < end; i += kThreads) {
-    Cx<T> a[3];
+    Cx<T> a[3], w[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) a[c] = w[c * nv + i];
-    for (int j = 0; j <= k; ++j) {
-      const Cx<T> cj = mk<T>((T)q.sm[L.c(j)], (T)q.sm[L.c(j) + 1]);
-      const Cx<T>* vj = q.K + (long long)j * 3 * nv;
+    for (int c = 0; c < 3; ++c) {
+      a[c] = u[c * nv + i];
+      w[c] = z[c * nv + i];
+    }
+    for (int j = 0; j < k; ++j) {
+      const Cx<T> sj = mk<T>((T)q.sm[L.s(j)], (T)q.sm[L.s(j) + 1]);
+      const Cx<T> tj = mk<T>((T)q.sm[L.t(j)], (T)q.sm[L.t(j) + 1]);
+      const Cx<T>* qj = q.K + (long long)j * 3 * nv;
 #pragma unroll
-      for (int c = 0; c < 3; ++c) a[c] = a[c] - cj * vj[c * nv + i];
+      for (int c = 0; c < 3; ++c) {
+        const Cx<T> v = qj[c * nv + i];
+        a[c] = a[c] - sj * v;
+        w[c] = w[c] - tj * v;
+      }
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      w[c * nv + i] = a[c];
-      acc[0] += (double)abs2(a[c]);
+      const Cx<T> qk = ib * a[c];
+      u[c * nv + i] = qk;
+      z[c * nv + i] = ib * (w[c] - tk * qk);
     }
   }
-  block_reduce_store(acc, q.part + (long long)blockIdx.x * kNumPart, scratch, threadIdx.x);
 }
 
-// start of a cycle: beta = ||r||, thresholds, g = (beta, 0, ...), scale = 1/beta
+// start of a cycle: ||r||, thresholds, g = 0 (g_0 = beta is set by step 0)
 __global__ void __launch_bounds__(kThreads)
 k_gm_fin_start(RState* const* __restrict__ sts, double* const* __restrict__ parts,
                const int* __restrict__ ntiles, double* const* __restrict__ sms, int m, int count) {
@@ -663,6 +847,7 @@ k_gm_fin_start(RState* const* __restrict__ sts, double* const* __restrict__ part
   st->cyc = 0;
   st->pend = 0;
   sm[GM_KDONE] = 0.0;
+  sm[GM_KSTEP] = 0.0;
   if (bad(r2)) {
     st->done = 3;
   } else if (r2 == 0.0 || r2 <= st->thr2) {
@@ -670,80 +855,7 @@ k_gm_fin_start(RState* const* __restrict__ sts, double* const* __restrict__ part
   } else if (st->kit >= st->maxit) {
     st->done = 2;
   } else {
-    const double beta = sqrt(r2);
     for (int i = 0; i <= m; ++i) st2(sm, L.g(i), mk<double>(0.0, 0.0));
-    st2(sm, L.g(0), mk<double>(beta, 0.0));
-    sm[GM_SCALE] = 1.0 / beta;
-  }
-}
-
-// finish Arnoldi step k: h = ||w||, rotations, residual estimate, stop decision
-// (Krylov.jl's gmres: solved when the estimate |g_(k+1)| meets the threshold,
-// breakdown when h <= btol, tired after maxit Arnoldi steps in all cycles)
-__global__ void __launch_bounds__(kThreads)
-k_gm_fin_arnoldi(RState* const* __restrict__ sts, double* const* __restrict__ parts,
-                 const int* __restrict__ ntiles, double* const* __restrict__ sms, int m, int k,
-                 double btol) {
-  __shared__ double red[kThreads / 64];
-  const int rid = blockIdx.x;
-  RState* st = sts[rid];
-  if (halted(st)) return;
-  const double h2 = block_sum_parts(parts[rid], ntiles[rid], 0, red);
-  if (threadIdx.x) return;
-  double* sm = sms[rid];
-  const GmLayout L{m};
-  const double h = sqrt(h2);
-  st2(sm, L.H(k + 1, k), mk<double>(h, 0.0));
-  for (int i = 0; i < k; ++i) {  // previous rotations on column k
-    const double c = sm[L.cs(i)];
-    const Cx<double> s = ld(sm, L.sn(i));
-    const Cx<double> a = ld(sm, L.H(i, k)), b = ld(sm, L.H(i + 1, k));
-    st2(sm, L.H(i, k), c * a + s * b);
-    st2(sm, L.H(i + 1, k), c * b - conj(s) * a);
-  }
-  const Cx<double> a = ld(sm, L.H(k, k)), b = ld(sm, L.H(k + 1, k));
-  const double na = sqrt(abs2(a)), den = sqrt(abs2(a) + abs2(b));
-  double c;
-  Cx<double> s;
-  if (den == 0.0) {
-    c = 1.0;
-    s = mk<double>(0.0, 0.0);
-  } else if (na == 0.0) {
-    c = 0.0;
-    s = mk<double>(1.0, 0.0);
-  } else {
-    c = na / den;
-    s = (1.0 / (na * den)) * (a * conj(b));
-  }
-  sm[L.cs(k)] = c;
-  st2(sm, L.sn(k), s);
-  st2(sm, L.H(k, k), c * a + s * b);
-  st2(sm, L.H(k + 1, k), mk<double>(0.0, 0.0));
-  const Cx<double> gk = ld(sm, L.g(k));
-  st2(sm, L.g(k + 1), (-1.0) * (conj(s) * gk));
-  st2(sm, L.g(k), c * gk);
-  const double res2 = abs2(ld(sm, L.g(k + 1)));
-  st->iters += 1;
-  st->kit += 1;
-  sm[GM_KDONE] = (double)(k + 1);
-  st->r2 = res2;
-  push_hist(st, res2);
-  sm[GM_SCALE] = h > 0.0 ? 1.0 / h : 0.0;
-  if (bad(res2) || bad(h2)) {
-    st->cyc = 1;
-    st->pend = 3;
-  } else if (res2 <= st->thr2) {
-    st->cyc = 1;
-    st->pend = 1;
-  } else if (h <= btol) {
-    st->cyc = 1;
-    st->pend = 4;  // breakdown short of the tolerance
-  } else if (st->kit >= st->maxit) {
-    st->cyc = 1;
-    st->pend = 2;
-  } else if (k + 1 == m) {
-    st->cyc = 1;
-    st->pend = 0;
   }
 }
 
@@ -828,7 +940,7 @@ void gmres_solve(Ctx* ctx, std::vector<SolveItem<T>>& it, const IterOpts& io, hi
     q.eps_inv = x.p->eps_inv;
     q.part = x.w->part.template as<double>();
     q.nchunk = (int)((q.nv + kChunk - 1) / kChunk);
-    x.w->D.ensure(ctx, (size_t)(m + 1) * q.nchunk * (kThreads / 64) * 2 * sizeof(double));
+    x.w->D.ensure(ctx, (size_t)(m + 1) * q.nchunk * 4 * sizeof(double));
     x.w->small.ensure(ctx, (size_t)lay.size() * sizeof(double));
     q.gpart = x.w->D.template as<double>();
     q.sm = x.w->small.template as<double>();
@@ -852,7 +964,7 @@ void gmres_solve(Ctx* ctx, std::vector<SolveItem<T>>& it, const IterOpts& io, hi
   CBS_HIP(hipMemcpyAsync(dnc.p, nchunks.data(), n * sizeof(int), hipMemcpyHostToDevice, s));
   const GmVec<T>* d_vs = dv.as<GmVec<T>>();
 
-  // pipelines: x -> V_0 (restart residual) and V_k -> V_{k+1}
+  // pipelines: x -> slot 0 (restart residual) and u_k in slot k -> z = A u_k in slot k + 1
   Pipe<T> mx;
   std::vector<Pipe<T>> mk_(m);
   std::vector<RState*> sts;
@@ -886,20 +998,19 @@ void gmres_solve(Ctx* ctx, std::vector<SolveItem<T>>& it, const IterOpts& io, hi
     hipLaunchKernelGGL((k_gm_resid<T>), vgrid, dim3(kThreads), 0, s, d_vs, use_v ? 1 : 0);
     hipLaunchKernelGGL(k_gm_fin_start, dim3(n), dim3(kThreads), 0, s, d_sts, d_parts, d_nt,
                        d_sms, m, use_v ? 1 : 0);
-    hipLaunchKernelGGL((k_gm_scale<T>), vgrid, dim3(kThreads), 0, s, d_vs, 0);
     for (int k = 0; k < m; ++k) {
-      mk_[k].launch(s);
-      for (int pass = 0; pass < 2; ++pass) {
-        hipLaunchKernelGGL((k_gm_dots<T>), vgrid, dim3(kThreads), 0, s, d_vs, k);
-        hipLaunchKernelGGL(k_gm_fin_dots, dim3(k + 1, n), dim3(kThreads), 0, s, d_sts,
-                           dg.as<double*>(), d_sms, dnc.as<int>(), m, k, pass);
-        hipLaunchKernelGGL((k_gm_update<T>), vgrid, dim3(kThreads), 0, s, d_vs, m, k);
-      }
-      hipLaunchKernelGGL(k_gm_fin_arnoldi, dim3(n), dim3(kThreads), 0, s, d_sts, d_parts, d_nt,
-                         d_sms, m, k, btol);
-      if (k + 1 < m)
-        hipLaunchKernelGGL((k_gm_scale<T>), vgrid, dim3(kThreads), 0, s, d_vs, k + 1);
+      mk_[k].launch(s);  // z = A u_k into slot k + 1
+      hipLaunchKernelGGL((k_gm_lred<T>), vgrid, dim3(kThreads), 0, s, d_vs, k);
+      hipLaunchKernelGGL(k_gm_fin_lred, dim3(k + 1, n), dim3(kThreads), 0, s, d_sts,
+                         dg.as<double*>(), d_sms, dnc.as<int>(), m, k);
+      hipLaunchKernelGGL(k_gm_fin_lstep, dim3(n), dim3(kThreads), 0, s, d_sts, d_sms, m, k, btol);
+      hipLaunchKernelGGL((k_gm_lupdate<T>), vgrid, dim3(kThreads), 0, s, d_vs, m, k);
     }
+    // the last column of the cycle: reduce of its pending vector
+    hipLaunchKernelGGL((k_gm_lred<T>), vgrid, dim3(kThreads), 0, s, d_vs, -1);
+    hipLaunchKernelGGL(k_gm_fin_lred, dim3(m + 1, n), dim3(kThreads), 0, s, d_sts,
+                       dg.as<double*>(), d_sms, dnc.as<int>(), m, -1);
+    hipLaunchKernelGGL(k_gm_fin_lfinal, dim3(n), dim3(64), 0, s, d_sts, d_sms, m, btol);
     hipLaunchKernelGGL(k_gm_solve, dim3(n), dim3(64), 0, s, d_sts, d_sms, m, btol);
     hipLaunchKernelGGL((k_gm_xupdate<T>), vgrid, dim3(kThreads), 0, s, d_vs, m);
     hipLaunchKernelGGL(k_gm_fin_cycle, dim3(n), dim3(64), 0, s, d_sts);
