@@ -546,9 +546,10 @@ __global__ void __launch_bounds__(kThreads) k_gm_resid(const GmVec<T>* __restric
 #define CBS_GM_JB 4
 #endif
 constexpr int kGmJB = CBS_GM_JB;
+constexpr int kGmRed = 4 * kGmJB < 16 ? 4 * kGmJB : 16;  // sums per block reduction
 template <class T>
 __global__ void __launch_bounds__(kThreads) k_gm_lred(const GmVec<T>* __restrict__ vs, int k) {
-  __shared__ double scratch[4 * kGmJB * (kThreads + 8)];
+  __shared__ double scratch[kGmRed * (kThreads + 8)];
   const GmVec<T>& q = vs[blockIdx.y];
   const bool last = k < 0;
   if (last ? closed(q.st) : halted(q.st)) return;
@@ -594,10 +595,17 @@ __global__ void __launch_bounds__(kThreads) k_gm_lred(const GmVec<T>* __restrict
         }
       }
     }
-    int which;
-    const double t = block_sum_lds<kThreads, 4 * kGmJB>(acc, scratch, threadIdx.x, which);
-    if (which >= 0 && j0 + which / 4 <= K)
-      q.gpart[((long long)(j0 + which / 4) * q.nchunk + blockIdx.x) * 4 + (which & 3)] = t;
+#pragma unroll
+    for (int h = 0; h < 4 * kGmJB; h += kGmRed) {
+      double part[kGmRed];
+#pragma unroll
+      for (int e = 0; e < kGmRed; ++e) part[e] = acc[h + e];
+      int which;
+      const double t = block_sum_lds<kThreads, kGmRed>(part, scratch, threadIdx.x, which);
+      const int e = h + which;
+      if (which >= 0 && j0 + e / 4 <= K)
+        q.gpart[((long long)(j0 + e / 4) * q.nchunk + blockIdx.x) * 4 + (e & 3)] = t;
+    }
   }
 }
 
@@ -649,22 +657,67 @@ k_gm_fin_lred(RState* const* __restrict__ sts, double* const* __restrict__ gpart
   }
 }
 
-// Close column j: the raw column gets the re-orthogonalisation coefficients
-// s_0 .. s_j and beta below them; its rotated copy gets the earlier rotations
-// and a new one, and the right-hand side g its update.  Returns the residual
-// estimate |g_(j+1)|^2 (one thread).
-__device__ double gm_close_column(double* sm, const GmLayout& L, int j, double beta) {
-  for (int i = 0; i <= j; ++i) st2(sm, L.Hr(i, j), ld(sm, L.Hr(i, j)) + ld(sm, L.s(i)));
-  st2(sm, L.Hr(j + 1, j), mk<double>(beta, 0.0));
-  for (int i = 0; i <= j + 1; ++i) st2(sm, L.H(i, j), ld(sm, L.Hr(i, j)));
-  for (int i = 0; i < j; ++i) {  // previous rotations on column j
-    const double c = sm[L.cs(i)];
-    const Cx<double> s = ld(sm, L.sn(i));
-    const Cx<double> a = ld(sm, L.H(i, j)), b = ld(sm, L.H(i + 1, j));
-    st2(sm, L.H(i, j), c * a + s * b);
-    st2(sm, L.H(i + 1, j), c * b - conj(s) * a);
+// LDS copies of a step's small dense data, so that the one thread that closes
+// a column runs its sequential loops on LDS rather than on chains of
+// dependent global loads
+struct GmStage {
+  double s[2 * (kGmresMaxBasis + 1)];
+  double t[2 * (kGmresMaxBasis + 1)];
+  double raw[2 * (kGmresMaxBasis + 2)];  // column j before the rotations
+  double col[2 * (kGmresMaxBasis + 2)];  // column j rotated
+  double cs[kGmresMaxBasis];
+  double sn[2 * kGmresMaxBasis];
+};
+__device__ __forceinline__ Cx<double> lds2(const double* a, int i) { return mk<double>(a[2 * i], a[2 * i + 1]); }
+__device__ __forceinline__ void sts2(double* a, int i, Cx<double> v) {
+  a[2 * i] = v.re;
+  a[2 * i + 1] = v.im;
+}
+
+// every thread: s (and t) of the K reduce rows, raw column j (rows 0..j) and
+// the rotations 0..j-1 into LDS
+__device__ void gm_stage(GmStage& G, const double* sm, const GmLayout& L, int K, int j,
+                         bool with_t) {
+  for (int i = threadIdx.x; i < K; i += blockDim.x) {
+    sts2(G.s, i, ld(sm, L.s(i)));
+    if (with_t) sts2(G.t, i, ld(sm, L.t(i)));
   }
-  const Cx<double> a = ld(sm, L.H(j, j)), b = ld(sm, L.H(j + 1, j));
+  for (int i = threadIdx.x; i <= j; i += blockDim.x) sts2(G.raw, i, ld(sm, L.Hr(i, j)));
+  for (int i = threadIdx.x; i < j; i += blockDim.x) {
+    G.cs[i] = sm[L.cs(i)];
+    sts2(G.sn, i, ld(sm, L.sn(i)));
+  }
+  __syncthreads();
+}
+
+// beta = |u_K - Q s| from alpha and s (one thread)
+__device__ __forceinline__ double gm_beta(const GmStage& G, const double* sm, int K, double& b2) {
+  double ss = 0.0;
+  for (int j = 0; j < K; ++j) ss += abs2(lds2(G.s, j));
+  b2 = sm[GM_ALPHA] - ss;
+  return b2 > 0.0 ? sqrt(b2) : 0.0;
+}
+
+// Close column j (one thread): the raw column gets the re-orthogonalisation
+// coefficients s_0 .. s_j and beta below them; its rotated copy gets the
+// earlier rotations and a new one, and the right-hand side g its update.
+// Returns the residual estimate |g_(j+1)|^2.
+__device__ double gm_close_column(GmStage& G, double* sm, const GmLayout& L, int j, double beta) {
+  for (int i = 0; i <= j; ++i) sts2(G.raw, i, lds2(G.raw, i) + lds2(G.s, i));
+  sts2(G.raw, j + 1, mk<double>(beta, 0.0));
+  for (int i = 0; i <= j + 1; ++i) {
+    const Cx<double> v = lds2(G.raw, i);
+    st2(sm, L.Hr(i, j), v);
+    sts2(G.col, i, v);
+  }
+  for (int i = 0; i < j; ++i) {  // previous rotations on column j
+    const double c = G.cs[i];
+    const Cx<double> s = lds2(G.sn, i);
+    const Cx<double> a = lds2(G.col, i), b = lds2(G.col, i + 1);
+    sts2(G.col, i, c * a + s * b);
+    sts2(G.col, i + 1, c * b - conj(s) * a);
+  }
+  const Cx<double> a = lds2(G.col, j), b = lds2(G.col, j + 1);
   const double na = sqrt(abs2(a)), den = sqrt(abs2(a) + abs2(b));
   double c;
   Cx<double> s;
@@ -678,23 +731,17 @@ __device__ double gm_close_column(double* sm, const GmLayout& L, int j, double b
     c = na / den;
     s = (1.0 / (na * den)) * (a * conj(b));
   }
+  sts2(G.col, j, c * a + s * b);
+  sts2(G.col, j + 1, mk<double>(0.0, 0.0));
+  for (int i = 0; i <= j + 1; ++i) st2(sm, L.H(i, j), lds2(G.col, i));
   sm[L.cs(j)] = c;
   st2(sm, L.sn(j), s);
-  st2(sm, L.H(j, j), c * a + s * b);
-  st2(sm, L.H(j + 1, j), mk<double>(0.0, 0.0));
   const Cx<double> gj = ld(sm, L.g(j));
-  st2(sm, L.g(j + 1), (-1.0) * (conj(s) * gj));
+  const Cx<double> gn = (-1.0) * (conj(s) * gj);
+  st2(sm, L.g(j + 1), gn);
   st2(sm, L.g(j), c * gj);
   sm[GM_KDONE] = (double)(j + 1);
-  return abs2(ld(sm, L.g(j + 1)));
-}
-
-// beta = |u_K - Q s| from alpha and s (one thread)
-__device__ __forceinline__ double gm_beta(const double* sm, const GmLayout& L, int K, double& b2) {
-  double ss = 0.0;
-  for (int j = 0; j < K; ++j) ss += abs2(ld(sm, L.s(j)));
-  b2 = sm[GM_ALPHA] - ss;
-  return b2 > 0.0 ? sqrt(b2) : 0.0;
+  return abs2(gn);
 }
 
 // Step k after its reduce: beta, column k-1 closed (Krylov.jl's gmres stopping:
@@ -704,15 +751,18 @@ __device__ __forceinline__ double gm_beta(const double* sm, const GmLayout& L, i
 __global__ void __launch_bounds__(kThreads)
 k_gm_fin_lstep(RState* const* __restrict__ sts, double* const* __restrict__ sms, int m, int k,
                double btol) {
+  __shared__ GmStage G;
   __shared__ int go;
+  __shared__ double ibeta;
   const int rid = blockIdx.x;
   RState* st = sts[rid];
   if (halted(st)) return;
   double* sm = sms[rid];
   const GmLayout L{m};
+  gm_stage(G, sm, L, k, k - 1, true);
   if (threadIdx.x == 0) {
     double b2;
-    const double beta = gm_beta(sm, L, k, b2);
+    const double beta = gm_beta(G, sm, k, b2);
     st->iters += 1;  // this step's operator application
     st->kit += 1;
     sm[GM_KSTEP] = (double)(k + 1);
@@ -725,7 +775,7 @@ k_gm_fin_lstep(RState* const* __restrict__ sts, double* const* __restrict__ sms,
         stop = true;
       }
     } else {
-      const double res2 = gm_close_column(sm, L, k - 1, beta);
+      const double res2 = gm_close_column(G, sm, L, k - 1, beta);
       st->r2 = res2;
       push_hist(st, res2);
       if (bad(res2) || bad(b2)) {
@@ -746,33 +796,43 @@ k_gm_fin_lstep(RState* const* __restrict__ sts, double* const* __restrict__ sms,
       sm[GM_SCALE] = 1.0 / beta;
       // q_k^H z = (gamma - s^H t) / beta, kept as t_k
       Cx<double> st_ = mk<double>(0.0, 0.0);
-      for (int j = 0; j < k; ++j) st_ = st_ + conj(ld(sm, L.s(j))) * ld(sm, L.t(j));
-      st2(sm, L.t(k), (1.0 / beta) * (ld(sm, GM_GAMMA) - st_));
+      for (int j = 0; j < k; ++j) st_ = st_ + conj(lds2(G.s, j)) * lds2(G.t, j);
+      const Cx<double> tk = (1.0 / beta) * (ld(sm, GM_GAMMA) - st_);
+      st2(sm, L.t(k), tk);
+      sts2(G.t, k, tk);
+      ibeta = 1.0 / beta;
     }
     go = stop ? 0 : 1;
   }
   __syncthreads();
   if (!go) return;
-  const double ib = sm[GM_SCALE];
+  const double ib = ibeta;
   for (int i = threadIdx.x; i <= k; i += kThreads) {
-    Cx<double> hs = mk<double>(0.0, 0.0);  // (Hbar s)_i, Hbar upper Hessenberg
-    for (int j = i > 0 ? i - 1 : 0; j < k; ++j) hs = hs + ld(sm, L.Hr(i, j)) * ld(sm, L.s(j));
-    st2(sm, L.Hr(i, k), ib * (ld(sm, L.t(i)) - hs));
+    // (Hbar s)_i over the closed columns 0..k-1 (upper Hessenberg); column
+    // k-1 from LDS, the earlier ones from the small buffer
+    Cx<double> hs = mk<double>(0.0, 0.0);
+    for (int j = i > 0 ? i - 1 : 0; j < k - 1; ++j) hs = hs + ld(sm, L.Hr(i, j)) * lds2(G.s, j);
+    if (k >= 1 && i <= k) hs = hs + lds2(G.raw, i) * lds2(G.s, k - 1);
+    st2(sm, L.Hr(i, k), ib * (lds2(G.t, i) - hs));
   }
 }
 
 // The cycle's last column closed after the final reduce
-__global__ void k_gm_fin_lfinal(RState* const* __restrict__ sts, double* const* __restrict__ sms,
-                                int m, double btol) {
+__global__ void __launch_bounds__(kThreads)
+k_gm_fin_lfinal(RState* const* __restrict__ sts, double* const* __restrict__ sms, int m,
+                double btol) {
+  __shared__ GmStage G;
   const int rid = blockIdx.x;
   RState* st = sts[rid];
-  if (closed(st) || threadIdx.x) return;
+  if (closed(st)) return;
   double* sm = sms[rid];
   const GmLayout L{m};
   const int K = (int)sm[GM_KSTEP];
+  gm_stage(G, sm, L, K, K - 1, false);
+  if (threadIdx.x) return;
   double b2;
-  const double beta = gm_beta(sm, L, K, b2);
-  const double res2 = gm_close_column(sm, L, K - 1, beta);
+  const double beta = gm_beta(G, sm, K, b2);
+  const double res2 = gm_close_column(G, sm, L, K - 1, beta);
   st->r2 = res2;
   push_hist(st, res2);
   st->cyc = 1;
@@ -1010,7 +1070,7 @@ void gmres_solve(Ctx* ctx, std::vector<SolveItem<T>>& it, const IterOpts& io, hi
     hipLaunchKernelGGL((k_gm_lred<T>), vgrid, dim3(kThreads), 0, s, d_vs, -1);
     hipLaunchKernelGGL(k_gm_fin_lred, dim3(m + 1, n), dim3(kThreads), 0, s, d_sts,
                        dg.as<double*>(), d_sms, dnc.as<int>(), m, -1);
-    hipLaunchKernelGGL(k_gm_fin_lfinal, dim3(n), dim3(64), 0, s, d_sts, d_sms, m, btol);
+    hipLaunchKernelGGL(k_gm_fin_lfinal, dim3(n), dim3(kThreads), 0, s, d_sts, d_sms, m, btol);
     hipLaunchKernelGGL(k_gm_solve, dim3(n), dim3(64), 0, s, d_sts, d_sms, m, btol);
     hipLaunchKernelGGL((k_gm_xupdate<T>), vgrid, dim3(kThreads), 0, s, d_vs, m);
     hipLaunchKernelGGL(k_gm_fin_cycle, dim3(n), dim3(64), 0, s, d_sts);
