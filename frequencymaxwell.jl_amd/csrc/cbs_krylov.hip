@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <type_traits>
 
 #include "cbs_reduce.h"
 #include "cbs_solve.h"
@@ -536,17 +537,143 @@ __global__ void __launch_bounds__(kThreads) k_gm_resid(const GmVec<T>* __restric
   block_reduce_store(acc, q.part + (long long)blockIdx.x * kNumPart, scratch, threadIdx.x);
 }
 
+// E consecutive complex values of a basis vector: E = 2 for complex-FP32
+// vectors of even component length (one 16-byte global access, aligned since
+// every component starts at an even element), else one at a time
+template <int E, class T>
+__device__ __forceinline__ void gm_lde(const Cx<T>* p, Cx<T> (&v)[E]) {
+  if constexpr (E == 2 && sizeof(T) == 4) {
+    typedef float V __attribute__((ext_vector_type(4)));
+    const V w = *(const __attribute__((address_space(1))) V*)(p);
+    v[0] = mk<T>(w.x, w.y);
+    v[1] = mk<T>(w.z, w.w);
+  } else {
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = gld(p + e);
+  }
+}
+template <int E, class T>
+__device__ __forceinline__ void gm_ste(Cx<T>* p, const Cx<T> (&v)[E]) {
+  if constexpr (E == 2 && sizeof(T) == 4) {
+    typedef float V __attribute__((ext_vector_type(4)));
+    V w;
+    w.x = v[0].re;
+    w.y = v[0].im;
+    w.z = v[1].re;
+    w.w = v[1].im;
+    *(__attribute__((address_space(1))) V*)(p) = w;
+  } else {
+#pragma unroll
+    for (int e = 0; e < E; ++e) gst(p + e, v[e]);
+  }
+}
+// elements per thread and access of the GMRES vector kernels for one block
+template <class T>
+__device__ __forceinline__ bool gm_paired(long long nv) {
+  return sizeof(T) == 4 && (nv & 1) == 0;
+}
+
+// the reduce sums of elements i .. i + E - 1 for rows j0 .. j0 + kGmJB - 1
+template <int E, class T, int NA>
+__device__ __forceinline__ void gm_lred_elems(const Cx<T>* __restrict__ K_, const Cx<T>* u,
+                                              const Cx<T>* z, long long nv, long long i, int j0,
+                                              int K, bool last, double (&acc)[NA]) {
+  Cx<T> uv[3][E], zv[3][E];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    gm_lde<E>(u + c * nv + i, uv[c]);
+    if (last) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) zv[c][e] = mk<T>(T(0), T(0));
+    } else {
+      gm_lde<E>(z + c * nv + i, zv[c]);
+    }
+  }
+#pragma unroll
+  for (int jj = 0; jj < NA / 4; ++jj) {
+    const int j = j0 + jj;
+    if (j < K) {
+      const Cx<T>* qj = K_ + (long long)j * 3 * nv;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        Cx<T> qv[E];
+        gm_lde<E>(qj + c * nv + i, qv);
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const Cx<T> d0 = cdot(qv[e], uv[c][e]), d1 = cdot(qv[e], zv[c][e]);
+          acc[4 * jj] += (double)d0.re;
+          acc[4 * jj + 1] += (double)d0.im;
+          acc[4 * jj + 2] += (double)d1.re;
+          acc[4 * jj + 3] += (double)d1.im;
+        }
+      }
+    } else if (j == K) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const Cx<T> d1 = cdot(uv[c][e], zv[c][e]);
+          acc[4 * jj] += (double)abs2(uv[c][e]);
+          acc[4 * jj + 2] += (double)d1.re;
+          acc[4 * jj + 3] += (double)d1.im;
+        }
+      }
+    }
+  }
+}
+
 // Reduce of step k (k < 0: the cycle's final reduce, K = steps taken, no z):
 // per-block partials of s_j = <q_j, u_K>, t_j = <q_j, z> (j < K) and of
 // alpha = |u_K|^2, gamma = <u_K, z> (row K), u_K in slot K and z in slot
-// K + 1.  The rows run in groups of kGmJB, each group reading its chunk of
-// u and z once and keeping kGmJB basis streams in flight; each group's sums
-// are reduced over the block through LDS.
+// K + 1.  The rows run in groups (kGmJB, or kGmJB2 where each thread sums two
+// elements through 16-byte accesses), each group reading its chunk of u and z
+// once and keeping a stream per row in flight; each group's sums are reduced
+// over the block through LDS.
 #ifndef CBS_GM_JB
 #define CBS_GM_JB 4
 #endif
+// rows per group when two elements per thread are summed (16-byte accesses)
+#ifndef CBS_GM_JB2
+#define CBS_GM_JB2 2
+#endif
+#ifndef CBS_GM_PAIR_RED
+#define CBS_GM_PAIR_RED 1
+#endif
+#ifndef CBS_GM_PAIR_UPD
+#define CBS_GM_PAIR_UPD 1
+#endif
 constexpr int kGmJB = CBS_GM_JB;
-constexpr int kGmRed = 4 * kGmJB < 16 ? 4 * kGmJB : 16;  // sums per block reduction
+constexpr int kGmJB2 = CBS_GM_JB2;
+constexpr int kGmRed = 16;  // sums per block reduction (at most)
+
+template <int E, int JB, class T>
+__device__ __forceinline__ void gm_lred_run(const GmVec<T>& q, int K, bool last, long long start,
+                                            long long end, double* scratch) {
+  constexpr int NA = 4 * JB;
+  constexpr int NR = NA < kGmRed ? NA : kGmRed;
+  const long long nv = q.nv;
+  const Cx<T>* u = q.K + (long long)K * 3 * nv;
+  const Cx<T>* z = u + 3 * nv;
+  for (int j0 = 0; j0 <= K; j0 += JB) {
+    double acc[NA];
+#pragma unroll
+    for (int e = 0; e < NA; ++e) acc[e] = 0.0;
+    for (long long i = start + E * threadIdx.x; i < end; i += E * kThreads)
+      gm_lred_elems<E>(q.K, u, z, nv, i, j0, K, last, acc);
+#pragma unroll
+    for (int h = 0; h < NA; h += NR) {
+      double part[NR];
+#pragma unroll
+      for (int e = 0; e < NR; ++e) part[e] = acc[h + e];
+      int which;
+      const double t = block_sum_lds<kThreads, NR>(part, scratch, threadIdx.x, which);
+      const int e = h + which;
+      if (which >= 0 && j0 + e / 4 <= K)
+        q.gpart[((long long)(j0 + e / 4) * q.nchunk + blockIdx.x) * 4 + (e & 3)] = t;
+    }
+  }
+}
+
 template <class T>
 __global__ void __launch_bounds__(kThreads) k_gm_lred(const GmVec<T>* __restrict__ vs, int k) {
   __shared__ double scratch[kGmRed * (kThreads + 8)];
@@ -556,57 +683,11 @@ __global__ void __launch_bounds__(kThreads) k_gm_lred(const GmVec<T>* __restrict
   const int K = last ? (int)q.sm[GM_KSTEP] : k;
   const long long start = (long long)blockIdx.x * kChunk;
   if (start >= q.nv) return;
-  const long long end = min(start + kChunk, q.nv), nv = q.nv;
-  const Cx<T>* u = q.K + (long long)K * 3 * nv;
-  const Cx<T>* z = u + 3 * nv;
-  for (int j0 = 0; j0 <= K; j0 += kGmJB) {
-    double acc[4 * kGmJB];
-#pragma unroll
-    for (int e = 0; e < 4 * kGmJB; ++e) acc[e] = 0.0;
-    for (long long i = start + threadIdx.x; i < end; i += kThreads) {
-      Cx<T> uv[3], zv[3];
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        uv[c] = u[c * nv + i];
-        zv[c] = last ? mk<T>(T(0), T(0)) : z[c * nv + i];
-      }
-#pragma unroll
-      for (int jj = 0; jj < kGmJB; ++jj) {
-        const int j = j0 + jj;
-        if (j < K) {
-          const Cx<T>* qj = q.K + (long long)j * 3 * nv;
-#pragma unroll
-          for (int c = 0; c < 3; ++c) {
-            const Cx<T> qv = qj[c * nv + i];
-            const Cx<T> d0 = cdot(qv, uv[c]), d1 = cdot(qv, zv[c]);
-            acc[4 * jj] += (double)d0.re;
-            acc[4 * jj + 1] += (double)d0.im;
-            acc[4 * jj + 2] += (double)d1.re;
-            acc[4 * jj + 3] += (double)d1.im;
-          }
-        } else if (j == K) {
-#pragma unroll
-          for (int c = 0; c < 3; ++c) {
-            const Cx<T> d1 = cdot(uv[c], zv[c]);
-            acc[4 * jj] += (double)abs2(uv[c]);
-            acc[4 * jj + 2] += (double)d1.re;
-            acc[4 * jj + 3] += (double)d1.im;
-          }
-        }
-      }
-    }
-#pragma unroll
-    for (int h = 0; h < 4 * kGmJB; h += kGmRed) {
-      double part[kGmRed];
-#pragma unroll
-      for (int e = 0; e < kGmRed; ++e) part[e] = acc[h + e];
-      int which;
-      const double t = block_sum_lds<kThreads, kGmRed>(part, scratch, threadIdx.x, which);
-      const int e = h + which;
-      if (which >= 0 && j0 + e / 4 <= K)
-        q.gpart[((long long)(j0 + e / 4) * q.nchunk + blockIdx.x) * 4 + (e & 3)] = t;
-    }
-  }
+  const long long end = min(start + kChunk, q.nv);
+  // chunk starts and lengths are even when the vectors pair
+  if (CBS_GM_PAIR_RED && gm_paired<T>(q.nv))
+    gm_lred_run<2, kGmJB2>(q, K, last, start, end, scratch);
+  else gm_lred_run<1, kGmJB>(q, K, last, start, end, scratch);
 }
 
 // Row j of the reduce summed over the blocks: s_j, t_j (j < K) or alpha,
@@ -858,12 +939,13 @@ __global__ void __launch_bounds__(kThreads) k_gm_lupdate(const GmVec<T>* __restr
   const Cx<T> tk = mk<T>((T)q.sm[L.t(k)], (T)q.sm[L.t(k) + 1]);
   Cx<T>* u = q.K + (long long)k * 3 * nv;
   Cx<T>* z = u + 3 * nv;
-  for (long long i = start + threadIdx.x; i < end; i += kThreads) {
-    Cx<T> a[3], w[3];
+  auto elems = [&](auto E_, long long i) {
+    constexpr int E = decltype(E_)::value;
+    Cx<T> a[3][E], w[3][E];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      a[c] = u[c * nv + i];
-      w[c] = z[c * nv + i];
+      gm_lde<E>(u + c * nv + i, a[c]);
+      gm_lde<E>(z + c * nv + i, w[c]);
     }
     for (int j = 0; j < k; ++j) {
       const Cx<T> sj = mk<T>((T)q.sm[L.s(j)], (T)q.sm[L.s(j) + 1]);
@@ -871,17 +953,33 @@ __global__ void __launch_bounds__(kThreads) k_gm_lupdate(const GmVec<T>* __restr
       const Cx<T>* qj = q.K + (long long)j * 3 * nv;
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        const Cx<T> v = qj[c * nv + i];
-        a[c] = a[c] - sj * v;
-        w[c] = w[c] - tj * v;
+        Cx<T> v[E];
+        gm_lde<E>(qj + c * nv + i, v);
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          a[c][e] = a[c][e] - sj * v[e];
+          w[c][e] = w[c][e] - tj * v[e];
+        }
       }
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const Cx<T> qk = ib * a[c];
-      u[c * nv + i] = qk;
-      z[c * nv + i] = ib * (w[c] - tk * qk);
+      Cx<T> qk[E], un[E];
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        qk[e] = ib * a[c][e];
+        un[e] = ib * (w[c][e] - tk * qk[e]);
+      }
+      gm_ste<E>(u + c * nv + i, qk);
+      gm_ste<E>(z + c * nv + i, un);
     }
+  };
+  if (CBS_GM_PAIR_UPD && gm_paired<T>(nv)) {  // chunk starts and lengths are even
+    for (long long i = start + 2 * threadIdx.x; i < end; i += 2 * kThreads)
+      elems(std::integral_constant<int, 2>{}, i);
+  } else {
+    for (long long i = start + threadIdx.x; i < end; i += kThreads)
+      elems(std::integral_constant<int, 1>{}, i);
   }
 }
 
