@@ -1045,18 +1045,31 @@ __global__ void __launch_bounds__(kThreads) k_gm_xupdate(const GmVec<T>* __restr
   const long long end = min(start + kChunk, q.nv), nv = q.nv;
   const GmLayout L{m};
   const int kd = (int)q.sm[GM_KDONE];
-  for (long long i = start + threadIdx.x; i < end; i += kThreads) {
-    Cx<T> a[3];
+  auto elems = [&](auto E_, long long i) {
+    constexpr int E = decltype(E_)::value;
+    Cx<T> a[3][E];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) a[c] = q.x[c * nv + i];
+    for (int c = 0; c < 3; ++c) gm_lde<E>(q.x + c * nv + i, a[c]);
     for (int j = 0; j < kd; ++j) {
       const Cx<T> yj = mk<T>((T)q.sm[L.y(j)], (T)q.sm[L.y(j) + 1]);
       const Cx<T>* vj = q.K + (long long)j * 3 * nv;
 #pragma unroll
-      for (int c = 0; c < 3; ++c) a[c] = a[c] + yj * vj[c * nv + i];
+      for (int c = 0; c < 3; ++c) {
+        Cx<T> v[E];
+        gm_lde<E>(vj + c * nv + i, v);
+#pragma unroll
+        for (int e = 0; e < E; ++e) a[c][e] = a[c][e] + yj * v[e];
+      }
     }
 #pragma unroll
-    for (int c = 0; c < 3; ++c) q.x[c * nv + i] = a[c];
+    for (int c = 0; c < 3; ++c) gm_ste<E>(q.x + c * nv + i, a[c]);
+  };
+  if (gm_paired<T>(nv)) {  // chunk starts and lengths are even
+    for (long long i = start + 2 * threadIdx.x; i < end; i += 2 * kThreads)
+      elems(std::integral_constant<int, 2>{}, i);
+  } else {
+    for (long long i = start + threadIdx.x; i < end; i += kThreads)
+      elems(std::integral_constant<int, 1>{}, i);
   }
 }
 
