@@ -187,3 +187,22 @@ def test_gmres_basis_limit(fm, oracle):
     _, _, st = prob.solve(S, fm.make_opts(method="gmres", tol=1e-6, maxiters=1000,
                                           gmres_restart=400), want_h=False)
     assert st.converged == 1
+
+
+@pytest.mark.parametrize("grid", [(16, 12, 8), (15, 13, 9)])
+def test_gmres_fp32_even_and_odd_voxel_counts(fm, oracle, grid):
+    """Complex-FP32 GMRES moves two values per 16-byte access when a grid's voxel
+    count is even and one at a time when it is odd: both layouts follow the
+    oracle's iterates at a fixed budget over restarted cycles to FP32 accuracy,
+    with Krylov.jl's iteration and application counts."""
+    nvox = int(np.prod(grid))
+    print(f"case gmres fp32 grid {grid} ({'odd' if nvox % 2 else 'even'} voxel count)", flush=True)
+    init, S, rhs, prob = _setup(fm, oracle, grid, oracle.PeriodicBC(), seed=37, precision="fp32")
+    budget = 11
+    x_o, st_o = oracle.solve_gmres(rhs, rhs, init.potential, init.eps_imag, init.green, 0.0,
+                                   budget, restart=4)
+    E, _, st = prob.solve(S, fm.make_opts(method="gmres", precision="fp32", tol=0.0,
+                                          maxiters=budget, gmres_restart=4), want_h=False)
+    assert st.iters == st_o.iters == budget
+    assert st.applications == st_o.applications == budget + 3  # cycles of 4, 4, 3 steps
+    assert rel(E, x_o) <= 2e-5
