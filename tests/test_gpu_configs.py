@@ -6,6 +6,8 @@ checked against the oracle where they are credited.
   every axis -> padded 160x160x96, averaged Green: 12 fixed Born sweeps
   through the whole solve path, E and H against the oracle (FP64 <= 1e-10,
   FP32 <= 2e-5);
+* the same workload with GMRES (restarted inside the budget) and BiCGStab at
+  a fixed budget, FP64 <= 1e-9 and FP32 <= 2e-5 against the oracle's iterates;
 * 64x64x32 bead (r 8 px, phantom_bead, src/medium/phantoms.jl:58-106),
   periodic: 20 fixed FP64 sweeps <= 1e-10; the same bead with 8 px absorbing
   padding converged to 1e-6 (FP32+refine) against the oracle converged to
@@ -74,6 +76,39 @@ def test_bench_workload_fixed_sweeps(fm, oracle):
     assert ref.init.potential.shape == (160, 160, 96)
     for precision, tol in (("fp64", TOL64), ("fp32", TOL32)):
         out = _fixed_solve(fm, oracle, med, bcs, 12, precision)
+        eE, eH = rel(out.E, ref.E), rel(out.H, ref.H)
+        print(f"  {precision}: E {eE:.2e} H {eH:.2e}", flush=True)
+        assert eE <= tol and eH <= tol, precision
+
+
+@pytest.mark.timeout(300)
+@pytest.mark.parametrize("method, budget, restart", [("gmres", 7, 3), ("bicgstab", 3, 0)])
+def test_bench_workload_krylov_fixed_budget(fm, oracle, method, budget, restart):
+    """The Krylov solvers on the benchmark workload (160x160x96, averaged) at a
+    fixed budget through the whole solve path: GMRES with a restart inside the
+    budget (delayed CGS2, two cycles closed by the final reduce and one
+    cut short) and BiCGStab, E and H against the oracle's iterates (FP64
+    <= 1e-9, FP32 <= 2e-5), Krylov.jl's iteration counts."""
+    print(f"case bench workload {method} budget {budget} restart {restart}", flush=True)
+    med = fm.phantom_bead((128, 128, 64), [1.5 ** 2], 16.0, permittivity_bg=N_BG ** 2)
+    bf, bo = _absorbing(fm, oracle)
+    s = oracle.Solver(resolution=RES, grid_size=med.permittivity.shape, boundary_conditions=bo,
+                      tolerance=0.0)
+    src_o = oracle.PlaneWave(wavelength=500e-9, polarization=[1, 0, 0], k_vector=[0, 0, 1])
+    ref = oracle.solve(s, src_o, med.permittivity, N_BG ** 2, method=method, rule="rhs",
+                       maxiters=budget, restart=restart)
+    assert ref.stats.iters == budget
+    import warnings
+    for precision, tol in (("fp64", 1e-9), ("fp32", TOL32)):
+        solver = fm.ConvergentBornSolver(resolution=RES, grid_size=med.permittivity.shape,
+                                         boundary_conditions=bf, tolerance=0.0,
+                                         linear_solver=method, stop_rule="rhs", maxiters=budget,
+                                         gmres_restart=restart, precision=precision)
+        src = fm.PlaneWaveSource(wavelength=500e-9, polarization=[1, 0, 0], k_vector=[0, 0, 1])
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore")  # a fixed budget ends unconverged
+            out = fm.solve(solver, src, med)
+        assert solver.iteration_count == budget
         eE, eH = rel(out.E, ref.E), rel(out.H, ref.H)
         print(f"  {precision}: E {eE:.2e} H {eH:.2e}", flush=True)
         assert eE <= tol and eH <= tol, precision
