@@ -84,6 +84,7 @@ struct Work {
   DevBuf X[2], B, W, R, part, st, D;
   DevBuf K;      // Krylov vectors, nk fields of 3 nvox, contiguous
   DevBuf small;  // Krylov small dense data (Hessenberg, rotations, coefficients)
+  DevBuf gm;     // GMRES block partials (reduce rows, residual norms)
   size_t nv = 0;
   int S = 1;
   int nk = 0;

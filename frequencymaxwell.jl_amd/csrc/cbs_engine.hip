@@ -524,7 +524,8 @@ uint64_t footprint_bytes(const cbs_problem_desc& d, const cbs_solve_opts& o) {
   if (o.method == CBS_BICGSTAB) b += fk * 6;
   if (o.method == CBS_GMRES) {
     const uint64_t m = o.gmres_restart > 0 ? (uint64_t)o.gmres_restart : (uint64_t)o.maxiters;
-    b += fk * (std::min<uint64_t>(m, 400) + 3);
+    const uint64_t mb = std::min<uint64_t>(m, 400), nch = (nv + kGmChunk - 1) / kGmChunk;
+    b += fk * (mb + 3) + ((mb + 1) * nch * 4 + nch * kNumPart) * 8;  // basis, block partials
   }
   if (o.compute_h) b += f64 + (o.precision == CBS_FP32 ? W64 : 0);  // curl output, FP64 workspace
   return b + (4u << 20);      // tables, partial sums, solver state

@@ -42,6 +42,13 @@ constexpr int kLdsBudget = 65536; // LDS bytes a pass tile may use (two blocks p
 #define CBS_ZMERGE 1
 #endif
 constexpr int kNumPart = 8;       // doubles of partial sums written per xinv tile
+// voxels per block of the GMRES vector kernels (more, smaller blocks than the
+// other vector kernels: the reduce and update passes need the waves in flight)
+#ifndef CBS_GM_CHUNK
+#define CBS_GM_CHUNK 512
+#endif
+constexpr long long kGmChunk = CBS_GM_CHUNK;
+static_assert(kGmChunk % (2 * kThreads) == 0, "whole pairs per thread");
 constexpr int kXLines = 4;        // x lines per x-pass tile where no tuned width exists
 
 // Axis lengths with compile-time specialised transforms (one kernel per

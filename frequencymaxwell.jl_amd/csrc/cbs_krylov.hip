@@ -478,7 +478,7 @@ struct GmVec {
   const Cx<T>* b;
   long long nv;
   double eps_inv;
-  double* part;   // kNumPart per vector-kernel block
+  double* part;   // residual-norm partials, kNumPart per vector-kernel block
   double* gpart;  // reduce partials [(m+1)][nchunk][4], one row per block of k_gm_lred
   double* sm;     // small dense data, layout GmLayout
   RState* st;
@@ -520,9 +520,9 @@ __global__ void __launch_bounds__(kThreads) k_gm_resid(const GmVec<T>* __restric
   __shared__ double scratch[(kThreads / 64) * kNumPart];
   const GmVec<T>& q = vs[blockIdx.y];
   if (q.st->done) return;
-  const long long start = (long long)blockIdx.x * kChunk;
+  const long long start = (long long)blockIdx.x * kGmChunk;
   if (start >= q.nv) return;
-  const long long end = min(start + kChunk, q.nv), nv = q.nv;
+  const long long end = min(start + kGmChunk, q.nv), nv = q.nv;
   double acc[kNumPart] = {0, 0, 0, 0, 0};
   for (long long i = start + threadIdx.x; i < end; i += kThreads) {
     const Cx<T> gam = mul_i((T)q.eps_inv * q.V[i]);
@@ -681,9 +681,9 @@ __global__ void __launch_bounds__(kThreads) k_gm_lred(const GmVec<T>* __restrict
   const bool last = k < 0;
   if (last ? closed(q.st) : halted(q.st)) return;
   const int K = last ? (int)q.sm[GM_KSTEP] : k;
-  const long long start = (long long)blockIdx.x * kChunk;
+  const long long start = (long long)blockIdx.x * kGmChunk;
   if (start >= q.nv) return;
-  const long long end = min(start + kChunk, q.nv);
+  const long long end = min(start + kGmChunk, q.nv);
   // chunk starts and lengths are even when the vectors pair
   if (CBS_GM_PAIR_RED && gm_paired<T>(q.nv))
     gm_lred_run<2, kGmJB2>(q, K, last, start, end, scratch);
@@ -931,9 +931,9 @@ __global__ void __launch_bounds__(kThreads) k_gm_lupdate(const GmVec<T>* __restr
                                                          int k) {
   const GmVec<T>& q = vs[blockIdx.y];
   if (closed(q.st) || (int)q.sm[GM_KSTEP] != k + 1) return;
-  const long long start = (long long)blockIdx.x * kChunk;
+  const long long start = (long long)blockIdx.x * kGmChunk;
   if (start >= q.nv) return;
-  const long long end = min(start + kChunk, q.nv), nv = q.nv;
+  const long long end = min(start + kGmChunk, q.nv), nv = q.nv;
   const GmLayout L{m};
   const T ib = (T)q.sm[GM_SCALE];
   const Cx<T> tk = mk<T>((T)q.sm[L.t(k)], (T)q.sm[L.t(k) + 1]);
@@ -1040,9 +1040,9 @@ template <class T>
 __global__ void __launch_bounds__(kThreads) k_gm_xupdate(const GmVec<T>* __restrict__ vs, int m) {
   const GmVec<T>& q = vs[blockIdx.y];
   if (q.st->done) return;
-  const long long start = (long long)blockIdx.x * kChunk;
+  const long long start = (long long)blockIdx.x * kGmChunk;
   if (start >= q.nv) return;
-  const long long end = min(start + kChunk, q.nv), nv = q.nv;
+  const long long end = min(start + kGmChunk, q.nv), nv = q.nv;
   const GmLayout L{m};
   const int kd = (int)q.sm[GM_KDONE];
   auto elems = [&](auto E_, long long i) {
@@ -1109,11 +1109,13 @@ void gmres_solve(Ctx* ctx, std::vector<SolveItem<T>>& it, const IterOpts& io, hi
     q.b = x.b;
     q.nv = x.p->nvox;
     q.eps_inv = x.p->eps_inv;
-    q.part = x.w->part.template as<double>();
-    q.nchunk = (int)((q.nv + kChunk - 1) / kChunk);
-    x.w->D.ensure(ctx, (size_t)(m + 1) * q.nchunk * 4 * sizeof(double));
+    q.nchunk = (int)((q.nv + kGmChunk - 1) / kGmChunk);
+    // the reduce partials, then one residual-norm row per chunk
+    const size_t ngp = (size_t)(m + 1) * q.nchunk * 4;
+    x.w->gm.ensure(ctx, (ngp + (size_t)q.nchunk * kNumPart) * sizeof(double));
     x.w->small.ensure(ctx, (size_t)lay.size() * sizeof(double));
-    q.gpart = x.w->D.template as<double>();
+    q.gpart = x.w->gm.template as<double>();
+    q.part = q.gpart + ngp;
     q.sm = x.w->small.template as<double>();
     q.st = x.w->st.template as<RState>();
     maxchunk = std::max(maxchunk, (long long)q.nchunk);
@@ -1145,11 +1147,12 @@ void gmres_solve(Ctx* ctx, std::vector<SolveItem<T>>& it, const IterOpts& io, hi
     Problem& p = *it[i].p;
     Cx<T>* W = it[i].w->W.template as<Cx<T>>();
     const long long f = 3 * hv[i].nv;
+    double* tpart = it[i].w->part.template as<double>();  // the sweeps' tile partials
     mx.h.push_back(make_desc<T>(p, PRE_V, ZOP_GREEN, POST_PMATVEC, hv[i].x, nullptr, hv[i].K, W,
-                                hv[i].part, hv[i].st));
+                                tpart, hv[i].st));
     for (int k = 0; k < m; ++k)
       mk_[k].h.push_back(make_desc<T>(p, PRE_V, ZOP_GREEN, POST_PMATVEC, hv[i].K + k * f, nullptr,
-                                      hv[i].K + (k + 1) * f, W, hv[i].part, hv[i].st));
+                                      hv[i].K + (k + 1) * f, W, tpart, hv[i].st));
     sts.push_back(hv[i].st);
     parts.push_back(hv[i].part);
     nt_vec.push_back(hv[i].nchunk);
