@@ -135,7 +135,10 @@ typedef struct {
   int64_t voxel_iters;    /* applications * N_vox                            */
   int32_t applications;   /* operator applications (Born: one per sweep;
                            * BiCGStab: two per iteration plus the warm start;
-                           * GMRES: one per step plus one per cycle start)   */
+                           * GMRES: one per step plus one per cycle start,
+                           * plus one in a cycle that meets the tolerance:
+                           * the delayed re-orthogonalisation learns it one
+                           * step late)                                      */
 } cbs_solve_stats;
 
 /* ---- context ------------------------------------------------------------ */
