@@ -206,3 +206,21 @@ def test_gmres_fp32_even_and_odd_voxel_counts(fm, oracle, grid):
     assert st.iters == st_o.iters == budget
     assert st.applications == st_o.applications == budget + 3  # cycles of 4, 4, 3 steps
     assert rel(E, x_o) <= 2e-5
+
+
+@pytest.mark.parametrize("budget, restart", [(1, 0), (2, 0), (3, 1), (5, 2)])
+def test_gmres_smallest_budgets_and_cycles(fm, oracle, budget, restart):
+    """The delayed re-orthogonalisation at its edges: a budget of one or two
+    Arnoldi steps (the cycle's only columns are closed by the final reduce)
+    and cycles of one or two steps, against the oracle's iterates (FP64)."""
+    print(f"case gmres budget {budget} restart {restart}", flush=True)
+    init, S, rhs, prob = _setup(fm, oracle, (12, 10, 8), oracle.PeriodicBC(), seed=38)
+    x_o, st_o = oracle.solve_gmres(rhs, rhs, init.potential, init.eps_imag, init.green, 0.0,
+                                   budget, restart=restart)
+    E, _, st = prob.solve(S, fm.make_opts(method="gmres", tol=0.0, maxiters=budget,
+                                          gmres_restart=restart), want_h=False)
+    cycles = 1 if restart == 0 else -(-budget // restart)
+    assert st.iters == st_o.iters == budget
+    assert st.applications == st_o.applications == budget + cycles
+    assert rel(E, x_o) <= 1e-9
+    assert st.resid == pytest.approx(st_o.resid, rel=1e-7)
