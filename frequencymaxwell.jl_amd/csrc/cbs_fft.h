@@ -254,10 +254,13 @@ struct NoTileIO {};
 // per group of lanes, a last stage with N/R = NS writes NS consecutive ones);
 // otherwise (t = b div Q, q = b mod Q), consecutive lanes take consecutive
 // pencils (conflict-free on the padded tile rows).
+// pre(), if given, runs between a GIN stage's loads and its barrier (a tile
+// stages its tables there, so their loads overlap those of the stage).
 template <class T, int N, int Q, int QS, int DIR, int R, int NS, int NT, bool GIN, bool GOUT,
-          class LD, class ST, bool TMAJ = false>
+          class LD, class ST, bool TMAJ = false, class PRE = NoTileIO>
 __device__ __forceinline__ void stage_ct_io(Cx<T>* __restrict__ s, const Cx<T>* __restrict__ tw,
-                                            int tid, const LD& ld, const ST& st) {
+                                            int tid, const LD& ld, const ST& st,
+                                            const PRE& pre = PRE{}) {
   constexpr int NB = N / R;
   constexpr int TOTAL = Q * NB;
   constexpr int RD = (TOTAL + NT - 1) / NT;
@@ -275,6 +278,7 @@ __device__ __forceinline__ void stage_ct_io(Cx<T>* __restrict__ s, const Cx<T>* 
       }
     }
   }
+  if constexpr (GIN && !std::is_same<PRE, NoTileIO>::value) pre();
   // orders the tile (and, after GIN, the staged twiddles and tables) for the
   // writes and butterflies below; a GOUT stage reading the tile writes none of it
   if constexpr (GIN || !GOUT) __syncthreads();
@@ -343,14 +347,16 @@ __device__ __forceinline__ void fft_ct_io(Cx<T>* __restrict__ s, const Cx<T>* __
 // outputs m = t + r N/R are exactly the inputs of the first stage of the
 // forward-order transform, so a caller can fuse work between the two.
 template <class T, int N, int Q, int QS, int DIR, int NT, bool GIN, int PK = 1, int S = 0,
-          int NS = 1, class LD, class ST>
+          int NS = 1, class LD, class ST, class PRE = NoTileIO>
 __device__ __forceinline__ void fft_ct_rev_head(Cx<T>* __restrict__ s,
                                                 const Cx<T>* __restrict__ tw, int tid,
-                                                const LD& ld, const ST& st) {
+                                                const LD& ld, const ST& st,
+                                                const PRE& pre = PRE{}) {
   constexpr int nst = plan_nst(N, PK);
   if constexpr (S < nst - 1) {
     constexpr int R = plan_radix(N, nst - 1 - S, PK);
-    stage_ct_io<T, N, Q, QS, DIR, R, NS, NT, GIN && S == 0, false>(s, tw, tid, ld, st);
+    stage_ct_io<T, N, Q, QS, DIR, R, NS, NT, GIN && S == 0, false, LD, ST, false, PRE>(
+        s, tw, tid, ld, st, pre);
     fft_ct_rev_head<T, N, Q, QS, DIR, NT, GIN, PK, S + 1, NS * R>(s, tw, tid, ld, st);
   }
 }

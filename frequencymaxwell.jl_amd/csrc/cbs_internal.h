@@ -24,6 +24,11 @@ constexpr int kLdsBudget = 65536; // LDS bytes a pass tile may use (two blocks p
 #ifndef CBS_XE
 #define CBS_XE 0
 #endif
+// CBS_ZLATE: merged z tiles stage their twiddles and k values after issuing
+// their first stage's HBM loads
+#ifndef CBS_ZLATE
+#define CBS_ZLATE 1
+#endif
 #ifndef CBS_XOPS_LATE
 #define CBS_XOPS_LATE 1
 #endif

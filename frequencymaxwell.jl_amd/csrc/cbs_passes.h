@@ -672,7 +672,7 @@ __device__ __forceinline__ void zpass_tile(const DevProb<T>& pr, int tile, int t
   const int n = CT ? NC : pr.n[2];
   const int P = zcols(n, eb, !CT);
   const int Q = 3 * P;
-  Cx<T>* lds = stage_twiddles<T, NT>(pr.tw[2], n, tid);
+  Cx<T>* lds = reinterpret_cast<Cx<T>*>(g_smem + tw_bytes(n, eb));  // behind the twiddles
   const int nx = pr.n[0], ny = pr.n[1];
   const int nxb = (nx + P - 1) / P;
   const int xb = tile % nxb;
@@ -682,15 +682,14 @@ __device__ __forceinline__ void zpass_tile(const DevProb<T>& pr, int tile, int t
   const int x0 = xb * P;
   const int npx = min(P, nx - x0);
   Cx<T>* base = pr.w + 3 * s * pr.ws[2] + (long long)y * pr.ws[0] + x0;
-  if constexpr (!CT) {
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-      tile_load<T, NT>(lds, base + c * pr.ws[2], P, npx, n, pr.ws[1], Q, c * P, tid);
-  }
   // k values of this tile: kt[m] along z, kt[n + p] along x, kt[n + P] at y,
   // rounded to the tile's precision (complex-FP32 tiles compute in FP32)
   T* kt = reinterpret_cast<T*>(reinterpret_cast<unsigned char*>(lds) + zk_offset(n, P, eb, !CT));
-  {
+  // the twiddles and k values into LDS; merged multi-stage tiles do it after
+  // issuing their first stage's HBM loads (the table loads then overlap them
+  // instead of delaying them by two round trips)
+  const auto stage_tables = [&]() {
+    stage_twiddles<T, NT>(pr.tw[2], n, tid);
     const bool green = pr.zop == ZOP_GREEN;
     const double* tz = green ? pr.kk[2][s] : pr.kc[2];
     const double* tx = green ? pr.kk[0][s] : pr.kc[0];
@@ -702,6 +701,48 @@ __device__ __forceinline__ void zpass_tile(const DevProb<T>& pr, int tile, int t
       else v = gld(ty + y);
       kt[i] = (T)v;
     }
+  };
+  constexpr int PK0 = zplan_kind(CT ? NC : 8, eb);
+  constexpr bool LATE = CT && CBS_ZMERGE && CBS_ZLATE && plan_nst(CT ? NC : 8, PK0) > 1;
+  if constexpr (!LATE) stage_tables();
+  // the same for a specialised length: every table load is issued before
+  // the first LDS store waits for them
+  const auto stage_tables_ct = [&]() {
+    constexpr int NCc = CT ? NC : 8;
+    constexpr int Pc = zcols(NCc, eb, false), NKc = NCc + Pc + 1;
+    constexpr int RT = (NCc + NT - 1) / NT, RK = (NKc + NT - 1) / NT;
+    const bool green = pr.zop == ZOP_GREEN;
+    const double* tz = green ? pr.kk[2][s] : pr.kc[2];
+    const double* tx = green ? pr.kk[0][s] : pr.kc[0];
+    const double* ty = green ? pr.kk[1][s] : pr.kc[1];
+    Cx<T> tv[RT];
+    double kv[RK];
+#pragma unroll
+    for (int r = 0; r < RT; ++r) {
+      const int i = tid + r * NT;
+      tv[r] = i < NCc ? gld(pr.tw[2] + i) : mk<T>(T(0), T(0));
+    }
+#pragma unroll
+    for (int r = 0; r < RK; ++r) {
+      const int i = tid + r * NT;
+      double v = 0.0;
+      if (i < NCc) v = gld(tz + i);
+      else if (i < NCc + Pc) v = (i - NCc < npx) ? gld(tx + x0 + i - NCc) : 0.0;
+      else if (i < NKc) v = gld(ty + y);
+      kv[r] = v;
+    }
+    Cx<T>* t = reinterpret_cast<Cx<T>*>(g_smem);
+#pragma unroll
+    for (int r = 0; r < RT; ++r)
+      if (tid + r * NT < NCc) t[tid + r * NT] = tv[r];
+#pragma unroll
+    for (int r = 0; r < RK; ++r)
+      if (tid + r * NT < NKc) kt[tid + r * NT] = (T)kv[r];
+  };
+  if constexpr (!CT) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      tile_load<T, NT>(lds, base + c * pr.ws[2], P, npx, n, pr.ws[1], Q, c * P, tid);
   }
   // specialised lengths: the forward transform's first stage reads HBM and the
   // inverse transform's last stage writes it (pencil q = component c, column p)
@@ -722,7 +763,11 @@ __device__ __forceinline__ void zpass_tile(const DevProb<T>& pr, int tile, int t
   if constexpr (CT && CBS_ZMERGE) {
     constexpr int nst = plan_nst(CT ? NC : 8, PK);
     const Cx<T>* tw = lds_twiddles<T>();
-    fft_ct_rev_head<T, NC, 3 * Pz, 3 * Pz, -1, NT, true, PK>(lds, tw, tid, ld, st);
+    if constexpr (LATE)
+      fft_ct_rev_head<T, NC, 3 * Pz, 3 * Pz, -1, NT, true, PK>(lds, tw, tid, ld, st,
+                                                               stage_tables_ct);
+    else
+      fft_ct_rev_head<T, NC, 3 * Pz, 3 * Pz, -1, NT, true, PK>(lds, tw, tid, ld, st);
     zpass_mid<T, NC, Pz, NT, PK, nst == 1, nst == 1>(pr, lds, tw, kt, npx, tid, ld, st);
     if constexpr (nst > 1)
       fft_ct_io<T, NC, 3 * Pz, 3 * Pz, 1, NT, false, true, 1, plan_radix(CT ? NC : 8, 0, PK),
