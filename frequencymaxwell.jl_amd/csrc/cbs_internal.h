@@ -24,6 +24,11 @@ constexpr int kLdsBudget = 65536; // LDS bytes a pass tile may use (two blocks p
 #ifndef CBS_XE
 #define CBS_XE 0
 #endif
+// CBS_XSEL: the x inverse zeroes the spectra of pencils beyond the tile's
+// lines as it loads them (1), or leaves them to the epilogue's element mask (0)
+#ifndef CBS_XSEL
+#define CBS_XSEL 0
+#endif
 // CBS_ZLATE: merged z tiles stage their twiddles and k values after issuing
 // their first stage's HBM loads
 #ifndef CBS_ZLATE

@@ -1041,11 +1041,19 @@ __device__ __forceinline__ void xinv_tile(const DevProb<T>& pr, int tile, int ti
     for (int s0 = 0; s0 < S; s0 += J) {
       if (s0 > 0) __syncthreads();  // the next pipeline reuses the tile
       if constexpr (XD) {
+        // pencils beyond the tile's lines read line l0 (x_pencil_off) and are
+        // transformed like the others; the epilogue drops their elements
+        // (xe[k].ok), so no select waits on each item's loads before the next
+        // item's loads issue
         const auto ld = [&](int q, int m) -> Cx<T> {
           bool in;
           const long long off = x_pencil_off(pr, q, Pc, s0, l0, np, in);
-          const Cx<T> v = gld(pr.w + off + m);
-          return in ? v : mk<T>(T(0), T(0));
+          if constexpr (CBS_XSEL) {
+            const Cx<T> v = gld(pr.w + off + m);
+            return in ? v : mk<T>(T(0), T(0));
+          } else {
+            return gld(pr.w + off + m);
+          }
         };
         // the epilogue operands load after the first stage (its registers are
         // free again) and arrive during the second
