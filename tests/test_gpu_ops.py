@@ -8,7 +8,7 @@ relative (the FFT roundoff is ~1e-15); FP32 <= 2e-6 relative.
 import numpy as np
 import pytest
 
-from cases import RES, oracle_setup, random_field, rel, spec_from
+from cases import N_BG, RES, oracle_setup, random_field, rel, spec_from
 
 pytestmark = pytest.mark.gpu
 
@@ -153,3 +153,41 @@ def test_grid_beyond_32bit_tile_offsets_rejected(fm):
     with pytest.raises(ValueError, match="2\\^32"):
         L.check(lib.cbs_problem_create(L.default_context().handle, C.byref(d), L.dptr(V), C.byref(h)))
     assert not h.value
+
+
+# x lengths whose inverse transform reads the spectra in its first stage
+# (two-stage plans with at least 16 butterflies per pencil), with line counts
+# (ny * nz of the padded grid) that leave the last x tile partly filled
+PARTIAL_TILE_GRIDS = [((64, 7, 5), False), ((96, 5, 3), False), ((160, 3, 7), False),
+                      ((56, 5, 3), True)]
+
+
+@pytest.mark.parametrize("grid, absorb", PARTIAL_TILE_GRIDS)
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_x_tiles_partly_filled(fm, oracle, grid, absorb, precision):
+    """The x passes on tiles holding fewer lines than their width: the x
+    inverse reads the spectra of the missing lines from the tile's first line
+    and drops their elements in the epilogue.  The operator (plain x inverse),
+    fused Born sweeps and BiCGStab (preconditioned-matvec partial sums) match
+    the oracle at a fixed budget."""
+    print(f"case x tiles partly filled {grid} absorb={absorb} {precision}", flush=True)
+    bc = _absorbing(oracle) if absorb else oracle.PeriodicBC()
+    s, _, init = oracle_setup(oracle, grid, bc, seed=17)
+    tol = FP64_TOL if precision == "fp64" else FP32_TOL
+    prob = fm.CBSProblem(init.potential, spec_from(fm, init), precision=precision)
+    x = random_field(init.potential.shape, 18)
+    if precision == "fp64":
+        got = prob.operator_apply(x)
+        assert rel(got, oracle.cbs_operator(x, init.potential, init.green)) <= tol
+    src = oracle.PlaneWave(wavelength=500e-9, polarization=[1, 0, 0], k_vector=[0, 0, 1])
+    E_inc, _ = oracle.incident_field(s, src, N_BG ** 2)
+    S = oracle.source_term(init.potential, E_inc, init.eps_imag)
+    rhs = oracle.green_conv(S, init.green)
+    solve_tol = 1e-9 if precision == "fp64" else 2e-5
+    for method, budget in (("born", 6), ("bicgstab", 3)):
+        fn = oracle.solve_born if method == "born" else oracle.solve_bicgstab
+        x_o, st_o = fn(rhs, rhs, init.potential, init.eps_imag, init.green, 0.0, budget)
+        E, _, st = prob.solve(S, fm.make_opts(method=method, precision=precision, tol=0.0,
+                                              maxiters=budget), want_h=False)
+        assert st.iters == st_o.iters == budget, method
+        assert rel(E, x_o) <= solve_tol, method
