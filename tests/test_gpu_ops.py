@@ -157,17 +157,20 @@ def test_grid_beyond_32bit_tile_offsets_rejected(fm):
 
 # x lengths whose inverse transform reads the spectra in its first stage
 # (two-stage plans with at least 16 butterflies per pencil), with line counts
-# (ny * nz of the padded grid) that leave the last x tile partly filled
+# (ny * nz of the padded grid) that leave the last x tile partly filled; and
+# two-stage z lengths (32, 96) under an x extent of 20, which leaves the last
+# z tile (16 or 8 x columns) partly filled while it stages its k values
 PARTIAL_TILE_GRIDS = [((64, 7, 5), False), ((96, 5, 3), False), ((160, 3, 7), False),
-                      ((56, 5, 3), True)]
+                      ((56, 5, 3), True), ((20, 6, 32), False), ((20, 4, 96), False)]
 
 
 @pytest.mark.parametrize("grid, absorb", PARTIAL_TILE_GRIDS)
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_x_tiles_partly_filled(fm, oracle, grid, absorb, precision):
-    """The x passes on tiles holding fewer lines than their width: the x
-    inverse reads the spectra of the missing lines from the tile's first line
-    and drops their elements in the epilogue.  The operator (plain x inverse),
+    """The x and z passes on tiles holding fewer lines (x) or columns (z) than
+    their width: the x inverse reads the spectra of the missing lines from the
+    tile's first line and drops their elements in the epilogue; the z tile
+    zeroes the k values of its missing columns.  The operator (plain x inverse),
     fused Born sweeps and BiCGStab (preconditioned-matvec partial sums) match
     the oracle at a fixed budget."""
     print(f"case x tiles partly filled {grid} absorb={absorb} {precision}", flush=True)
